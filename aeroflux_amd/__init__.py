@@ -1,0 +1,379 @@
+"""Python mirror of the AeroFlux Go API over the MI355X C ABI.
+
+The names, argument meanings and error behaviour follow the reference's
+exported API (AeroFlex/aeroflux.go:14-439, grid.go:8-193, project.go:210-258,
+stencil.go:319-338): ``Config`` + ``Config.Normalize()``, ``New(cfg)``,
+``Engine.Step()``, ``MaxDivergence()``, ``KineticEnergy()``,
+``MaxVelocity()``, ``CFL()``, ``Close()``, ``EnableSparseSolver()``,
+``Engine.Mesh`` with its ``Field`` descriptors and ``Data``.  Go ``error``
+returns become :class:`AeroFluxError` exceptions carrying the same
+``"aeroflux: ..."`` messages.
+
+Every call goes through ``libaeroflux.so`` (hand-written HIP for gfx950).
+There is no CPU fallback: without the built library or a GPU the calls
+raise.
+
+Differences forced by device memory, by design:
+
+* ``Mesh.Data(field)`` returns a host *copy* of the padded field shaped
+  ``(NZ, NY, NX)`` (Go returns a live view aliasing the arena,
+  grid.go:147-149); write a modified array back with ``Mesh.SetData``.
+* ``Field`` values name a role (P, U, ...); ``Data`` always reads the
+  current descriptor of that role, as ``m.Data(&m.U)`` does in Go.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, fields as dc_fields
+
+import numpy as np
+
+__all__ = [
+    "AeroFluxError", "Config", "Field", "MAC", "Engine", "New", "MACBytes",
+    "FIELD_P", "FIELD_U", "FIELD_V", "FIELD_W", "FIELD_DIV", "FIELD_TMP0",
+    "FIELD_TMP1", "FIELD_TMP2", "AXIS_X", "AXIS_Y", "AXIS_Z", "COLOR_RED",
+    "COLOR_BLACK", "PRECISION_FP64", "PRECISION_MIXED", "lib_path", "load_library",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libaeroflux.so")
+
+AF_OK, AF_ERR_CONFIG, AF_ERR_FAULT, AF_ERR_OOM, AF_ERR_HIP, AF_ERR_ARG, AF_ERR_CLOSED, \
+    AF_ERR_UNSUPPORTED = range(8)
+
+FIELD_P, FIELD_U, FIELD_V, FIELD_W, FIELD_DIV, FIELD_TMP0, FIELD_TMP1, FIELD_TMP2 = range(8)
+FIELD_NAMES = ("P", "U", "V", "W", "Div", "Tmp0", "Tmp1", "Tmp2")
+AXIS_X, AXIS_Y, AXIS_Z = range(3)
+COLOR_RED, COLOR_BLACK = range(2)
+PRECISION_FP64, PRECISION_MIXED = range(2)
+
+
+class AeroFluxError(RuntimeError):
+    """A Go ``error`` from the reference API, raised.  ``code`` is the C ABI
+    return code (AF_ERR_*)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class _CConfig(C.Structure):
+    _fields_ = [
+        ("NX", C.c_int64), ("NY", C.c_int64), ("NZ", C.c_int64), ("Ghost", C.c_int64),
+        ("H", C.c_double), ("Viscosity", C.c_double), ("Density", C.c_double),
+        ("DT", C.c_double), ("Workers", C.c_int64), ("GaussSeidelIters", C.c_int64),
+        ("UseSparseSolver", C.c_int32), ("Lock", C.c_int32),
+        ("Precision", C.c_int32), ("MixedMaxCycles", C.c_int32), ("MixedSweeps", C.c_int32),
+        ("MixedFixSweeps", C.c_int32), ("MixedTarget", C.c_double),
+    ]
+
+
+class _CField(C.Structure):
+    _fields_ = [("Off", C.c_int64), ("NX", C.c_int64), ("NY", C.c_int64), ("NZ", C.c_int64),
+                ("SY", C.c_int64), ("SZ", C.c_int64), ("Count", C.c_int64)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported symbol in include/aeroflux.h.
+_SIGNATURES = [
+    ("af_config_normalize", C.c_int, [C.POINTER(_CConfig)]),
+    ("af_mac_bytes", C.c_int64, [C.c_int64] * 4),
+    ("af_new", C.c_int, [C.POINTER(_CConfig), C.POINTER(C.c_void_p)]),
+    ("af_close", C.c_int, [C.c_void_p]),
+    ("af_free", None, [C.c_void_p]),
+    ("af_get_config", C.c_int, [C.c_void_p, C.POINTER(_CConfig)]),
+    ("af_step", C.c_int, [C.c_void_p]),
+    ("af_step_n", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    ("af_max_divergence", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("af_kinetic_energy", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("af_max_velocity", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("af_cfl", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("af_enable_sparse_solver", C.c_int, [C.c_void_p]),
+    ("af_field_desc", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_CField)]),
+    ("af_field_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
+    ("af_field_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
+    ("af_field_device_ptr", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
+    ("af_arena_bytes", C.c_int64, [C.c_void_p]),
+    ("af_apply_velocity_bc", C.c_int, [C.c_void_p]),
+    ("af_sweep_axis", C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    ("af_commit_velocity", C.c_int, [C.c_void_p]),
+    ("af_compute_divergence", C.c_int, [C.c_void_p]),
+    ("af_gs_sweep", C.c_int, [C.c_void_p, C.c_int]),
+    ("af_apply_pressure_bc", C.c_int, [C.c_void_p]),
+    ("af_solve_pressure", C.c_int, [C.c_void_p]),
+    ("af_subtract_gradient", C.c_int, [C.c_void_p]),
+    ("af_project", C.c_int, [C.c_void_p]),
+    ("af_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    ("af_mixed_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("af_last_error", C.c_char_p, []),
+]
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def load_library():
+    """Load ``libaeroflux.so`` (built by ``python -m aeroflux_amd.build``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(
+                f"aeroflux: {_LIB_PATH} is not built; run `python -m aeroflux_amd.build`")
+        lib = C.CDLL(_LIB_PATH)
+        for name, res, args in _SIGNATURES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != AF_OK:
+        msg = load_library().af_last_error().decode()
+        raise AeroFluxError(rc, msg or f"aeroflux: error code {rc}")
+
+
+@dataclass
+class Config:
+    """Config, aeroflux.go:14-41 (plus the MI355X-build extensions)."""
+    NX: int = 0
+    NY: int = 0
+    NZ: int = 0
+    Ghost: int = 0
+    H: float = 0.0
+    Viscosity: float = 0.0
+    Density: float = 0.0
+    DT: float = 0.0
+    Workers: int = 0
+    GaussSeidelIters: int = 0
+    UseSparseSolver: bool = False
+    Lock: bool = False
+    Precision: int = PRECISION_FP64
+    MixedMaxCycles: int = 0
+    MixedSweeps: int = 0
+    MixedFixSweeps: int = 0
+    MixedTarget: float = 0.0
+
+    def _to_c(self) -> _CConfig:
+        c = _CConfig()
+        for f in dc_fields(self):
+            v = getattr(self, f.name)
+            setattr(c, f.name, int(v) if isinstance(v, bool) else v)
+        return c
+
+    def _from_c(self, c: _CConfig) -> None:
+        for f in dc_fields(self):
+            v = getattr(c, f.name)
+            if f.type in ("bool", bool):
+                v = bool(v)
+            setattr(self, f.name, v)
+
+    def Normalize(self) -> None:
+        """Config.Normalize, aeroflux.go:47-113: fills defaults in place,
+        raises AeroFluxError on a geometry the solver cannot represent."""
+        c = self._to_c()
+        rc = load_library().af_config_normalize(C.byref(c))
+        self._from_c(c)
+        _check(rc)
+
+
+@dataclass(frozen=True)
+class Field:
+    """Field, grid.go:12-21.  ``id`` names the role (FIELD_*)."""
+    id: int
+    Off: int
+    NX: int
+    NY: int
+    NZ: int
+    SY: int
+    SZ: int
+    Count: int
+
+    def Index(self, x: int, y: int, z: int) -> int:  # grid.go:24-26
+        return x + y * self.SY + z * self.SZ
+
+    def ByteOffset(self, x: int, y: int, z: int) -> int:  # grid.go:29-31
+        return self.Off + self.Index(x, y, z) * 8
+
+    @property
+    def shape(self) -> tuple[int, int, int]:
+        return (self.NZ, self.NY, self.NX)
+
+
+def MACBytes(nx: int, ny: int, nz: int, ghost: int) -> int:
+    """MACBytes, grid.go:138-144."""
+    return int(load_library().af_mac_bytes(nx, ny, nz, ghost))
+
+
+class MAC:
+    """The engine's staggered grid (MAC, grid.go:52-73) over HBM."""
+
+    def __init__(self, engine: "Engine"):
+        self._e = engine
+        cfg = engine.Cfg
+        self.NX, self.NY, self.NZ, self.Ghost, self.H = cfg.NX, cfg.NY, cfg.NZ, cfg.Ghost, cfg.H
+
+    def field(self, fid: int) -> Field:
+        f = _CField()
+        _check(load_library().af_field_desc(self._e._handle(), fid, C.byref(f)))
+        return Field(fid, f.Off, f.NX, f.NY, f.NZ, f.SY, f.SZ, f.Count)
+
+    P = property(lambda s: s.field(FIELD_P))
+    U = property(lambda s: s.field(FIELD_U))
+    V = property(lambda s: s.field(FIELD_V))
+    W = property(lambda s: s.field(FIELD_W))
+    Div = property(lambda s: s.field(FIELD_DIV))
+    Tmp0 = property(lambda s: s.field(FIELD_TMP0))
+    Tmp1 = property(lambda s: s.field(FIELD_TMP1))
+    Tmp2 = property(lambda s: s.field(FIELD_TMP2))
+
+    @property
+    def Bytes(self) -> int:
+        return int(load_library().af_arena_bytes(self._e._handle()))
+
+    def Data(self, f: Field) -> np.ndarray:
+        """Host copy of the current padded field of f's role, (NZ, NY, NX)."""
+        cur = self.field(f.id)
+        out = np.empty(cur.shape, dtype=np.float64)
+        _check(load_library().af_field_read(self._e._handle(), cur.id,
+                                            out.ctypes.data_as(C.c_void_p), cur.Count))
+        return out
+
+    def SetData(self, f: Field, data: np.ndarray) -> None:
+        """Write a whole padded field (any shape with Count float64s)."""
+        cur = self.field(f.id)
+        arr = np.ascontiguousarray(data, dtype=np.float64).reshape(-1)
+        if arr.size != cur.Count:
+            raise AeroFluxError(AF_ERR_ARG, f"aeroflux: field has {cur.Count} elements, "
+                                            f"got {arr.size}")
+        _check(load_library().af_field_write(self._e._handle(), cur.id,
+                                             arr.ctypes.data_as(C.c_void_p), cur.Count))
+
+    def DevicePtr(self, f: Field) -> int:
+        out = C.c_uint64()
+        _check(load_library().af_field_device_ptr(self._e._handle(), f.id, C.byref(out)))
+        return int(out.value)
+
+
+class Engine:
+    """Engine, aeroflux.go:133-172: owns the HBM arena and the step graph."""
+
+    def __init__(self, handle: int, cfg: Config):
+        self._h = handle
+        self.Cfg = cfg
+        self.Mesh = MAC(self)
+
+    def _handle(self) -> int:
+        if self._h is None:
+            raise AeroFluxError(AF_ERR_CLOSED, "aeroflux: engine is closed")
+        return self._h
+
+    # -- public API ---------------------------------------------------------
+    def Step(self) -> None:
+        """Step, aeroflux.go:348-409."""
+        _check(load_library().af_step(self._handle()))
+
+    def StepN(self, n: int) -> None:
+        """n Steps with one synchronisation (no reference counterpart)."""
+        first = C.c_int64(-1)
+        _check(load_library().af_step_n(self._handle(), n, C.byref(first)))
+
+    def _scalar(self, fn) -> float:
+        out = C.c_double()
+        _check(fn(self._handle(), C.byref(out)))
+        return float(out.value)
+
+    def MaxDivergence(self) -> float:  # project.go:210-232
+        return self._scalar(load_library().af_max_divergence)
+
+    def KineticEnergy(self) -> float:  # project.go:238-258
+        return self._scalar(load_library().af_kinetic_energy)
+
+    def MaxVelocity(self) -> float:  # stencil.go:319-331
+        return self._scalar(load_library().af_max_velocity)
+
+    def CFL(self) -> float:  # stencil.go:336-338
+        return self._scalar(load_library().af_cfl)
+
+    def EnableSparseSolver(self) -> None:  # sparse_fallback.go:17-19
+        _check(load_library().af_enable_sparse_solver(self._handle()))
+
+    def Close(self) -> None:
+        """Close, aeroflux.go:419-439; idempotent."""
+        if self._h is not None:
+            lib = load_library()
+            rc = lib.af_close(self._h)
+            lib.af_free(self._h)
+            self._h = None
+            _check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.Close()
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                load_library().af_free(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    # -- unexported methods the reference's in-package tests call ------------
+    def apply_velocity_bc(self) -> None:  # stencil.go:246-315
+        _check(load_library().af_apply_velocity_bc(self._handle()))
+
+    def sweep_axis(self, axis: int, dt: float) -> None:  # stencil.go:21-25
+        _check(load_library().af_sweep_axis(self._handle(), axis, dt))
+
+    def commit_velocity(self) -> None:  # stencil.go:230-235
+        _check(load_library().af_commit_velocity(self._handle()))
+
+    def compute_divergence(self) -> None:  # project.go:42-62
+        _check(load_library().af_compute_divergence(self._handle()))
+
+    def gs_sweep(self, color: int) -> None:  # project.go:103-130
+        _check(load_library().af_gs_sweep(self._handle(), color))
+
+    def apply_pressure_bc(self) -> None:  # project.go:139-163
+        _check(load_library().af_apply_pressure_bc(self._handle()))
+
+    def solve_pressure(self) -> None:  # project.go:73-100
+        _check(load_library().af_solve_pressure(self._handle()))
+
+    def subtract_gradient(self) -> None:  # project.go:172-205
+        _check(load_library().af_subtract_gradient(self._handle()))
+
+    def project(self) -> None:  # project.go:12-34
+        _check(load_library().af_project(self._handle()))
+
+    # -- measurement ----------------------------------------------------------
+    def time_kernel(self, kernel: int, reps: int) -> float:
+        out = C.c_double()
+        _check(load_library().af_time_kernel(self._handle(), kernel, reps, C.byref(out)))
+        return float(out.value)
+
+    def mixed_stats(self) -> tuple[float, int]:
+        d = C.c_double()
+        n = C.c_int32()
+        _check(load_library().af_mixed_stats(self._handle(), C.byref(d), C.byref(n)))
+        return float(d.value), int(n.value)
+
+
+def New(cfg: Config) -> Engine:
+    """New, aeroflux.go:176-213.  ``cfg`` is normalised in a copy."""
+    lib = load_library()
+    c = cfg._to_c()
+    h = C.c_void_p()
+    _check(lib.af_new(C.byref(c), C.byref(h)))
+    out = _CConfig()
+    lib.af_get_config(h, C.byref(out))
+    ncfg = Config()
+    ncfg._from_c(out)
+    return Engine(h.value, ncfg)

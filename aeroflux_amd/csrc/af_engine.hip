@@ -1,0 +1,790 @@
+// af_engine.hip -- the engine behind the C ABI (include/aeroflux.h).
+//
+// One engine = one zeroed HBM arena laid out exactly like the reference's
+// MAC arena (grid.go:79-134), one HIP stream, and one hipGraph per Step
+// (aeroflux.go:348-409) that replays the whole Strang palindrome.  The
+// reference's per-phase goroutine dispatch + join becomes stream order; its
+// serial boundary-condition fills become gather kernels (or are fused into
+// the red-black passes); its guard-page fault containment becomes a device
+// flag raised by a NaN departure coordinate and returned by Step.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unistd.h>
+
+#include "../../include/aeroflux.h"
+#include "af_internal.h"
+#include "af_mixed.h"
+
+using namespace af;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t err, const char *what)
+{
+    return fail(AF_ERR_HIP, "aeroflux: %s: %s", what, hipGetErrorString(err));
+}
+
+#define HIP_TRY(expr, what)                         \
+    do {                                            \
+        hipError_t _e = (expr);                     \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+constexpr int kCacheLine = 128;  // layout.go:13
+
+af_field field_layout(int64_t *off, int64_t nx, int64_t ny, int64_t nz)
+{
+    af_field f;
+    f.Off = *off;
+    f.NX = nx; f.NY = ny; f.NZ = nz;
+    f.SY = nx; f.SZ = nx * ny;
+    f.Count = nx * ny * nz;
+    int64_t sz = f.Count * 8;
+    if (int64_t r = sz % kCacheLine) sz += kCacheLine - r;
+    *off += sz;
+    return f;
+}
+
+// NewMAC field order and padding, grid.go:101-134.
+int64_t mac_layout(af_field *f, int64_t nx, int64_t ny, int64_t nz, int64_t g)
+{
+    const int64_t g2 = 2 * g, cx = nx + g2, cy = ny + g2, cz = nz + g2;
+    int64_t off = 0;
+    f[AF_FIELD_P] = field_layout(&off, cx, cy, cz);
+    f[AF_FIELD_U] = field_layout(&off, cx + 1, cy, cz);
+    f[AF_FIELD_V] = field_layout(&off, cx, cy + 1, cz);
+    f[AF_FIELD_W] = field_layout(&off, cx, cy, cz + 1);
+    f[AF_FIELD_DIV] = field_layout(&off, cx, cy, cz);
+    f[AF_FIELD_TMP0] = field_layout(&off, cx + 1, cy, cz);
+    f[AF_FIELD_TMP1] = field_layout(&off, cx, cy + 1, cz);
+    f[AF_FIELD_TMP2] = field_layout(&off, cx, cy, cz + 1);
+    return off;
+}
+
+// Current descriptors of the six velocity-sized fields, as the step walks
+// through its commitVelocity swaps.
+struct Desc {
+    af_field f[AF_FIELD_COUNT];
+};
+
+void commit(Desc &d)  // commitVelocity, stencil.go:230-235
+{
+    std::swap(d.f[AF_FIELD_U], d.f[AF_FIELD_TMP0]);
+    std::swap(d.f[AF_FIELD_V], d.f[AF_FIELD_TMP1]);
+    std::swap(d.f[AF_FIELD_W], d.f[AF_FIELD_TMP2]);
+}
+
+}  // namespace
+
+struct af_engine {
+    af_config cfg;
+    int device = 0;
+    bool closed = false;
+    hipStream_t stream = nullptr;
+    unsigned char *arena = nullptr;
+    int64_t bytes = 0;
+    Desc cur;          // descriptors after any stage-level swaps
+    int64_t orig_off[AF_FIELD_COUNT];
+    Geom G;
+    double invH, h2, div_scale, grad_scale;
+    int *flag = nullptr;        // device fault flag
+    int *flag_host = nullptr;   // pinned mirror
+    double *partials = nullptr;
+    double *red_out = nullptr;
+    double *red_host = nullptr; // pinned
+    hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
+    MixedState mixed;
+};
+
+namespace {
+
+DField dfield(const af_engine *e, const af_field &f)
+{
+    DField d;
+    d.d = reinterpret_cast<double *>(e->arena + f.Off);
+    d.sy = (int)f.SY;
+    d.sz = (int)f.SZ;
+    d.nx = (int)f.NX;
+    d.ny = (int)f.NY;
+    d.nz = (int)f.NZ;
+    return d;
+}
+
+int swap_state(const af_engine *e, const Desc &d)
+{
+    int s = 0;
+    if (d.f[AF_FIELD_U].Off != e->orig_off[AF_FIELD_U]) s |= 1;
+    if (d.f[AF_FIELD_V].Off != e->orig_off[AF_FIELD_V]) s |= 2;
+    if (d.f[AF_FIELD_W].Off != e->orig_off[AF_FIELD_W]) s |= 4;
+    return s;
+}
+
+int check_open(const af_engine *e)
+{
+    if (!e) return fail(AF_ERR_ARG, "aeroflux: null engine");
+    if (e->closed) return fail(AF_ERR_CLOSED, "aeroflux: engine is closed");
+    return AF_OK;
+}
+
+// ---- enqueue helpers (used both eagerly and under stream capture) -------
+
+hipError_t enq_velocity_bc(af_engine *e, const Desc &d)
+{
+    return launch_velocity_bc(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                              dfield(e, d.f[AF_FIELD_W]), e->G, e->stream);
+}
+
+hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
+{
+    SweepArgs a;
+    a.U = dfield(e, d.f[AF_FIELD_U]);
+    a.V = dfield(e, d.f[AF_FIELD_V]);
+    a.W = dfield(e, d.f[AF_FIELD_W]);
+    a.T0 = dfield(e, d.f[AF_FIELD_TMP0]);
+    a.T1 = dfield(e, d.f[AF_FIELD_TMP1]);
+    a.T2 = dfield(e, d.f[AF_FIELD_TMP2]);
+    a.G = e->G;
+    a.dt = dt;
+    // advectComponent, stencil.go:47,73: invH = 1.0/h; lam = nu*dt*invH*invH.
+    a.invH = e->invH;
+    a.lam = e->cfg.Viscosity * dt * e->invH * e->invH;
+    a.flag = e->flag;
+    return launch_sweep(axis, a, e->stream);
+}
+
+hipError_t enq_divergence(af_engine *e, const Desc &d)
+{
+    return launch_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                             dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_DIV]), e->G,
+                             e->div_scale, e->stream);
+}
+
+hipError_t enq_gradient(af_engine *e, const Desc &d)
+{
+    return launch_gradient(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                           dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_P]), e->G,
+                           e->grad_scale, e->stream);
+}
+
+// solvePressureRedBlack (project.go:73-100), FP64, production form: the
+// zeroing is folded into the first red pass and every intermediate
+// applyPressureBC into the passes themselves; the final fill is materialised.
+hipError_t enq_solve_fp64(af_engine *e, const Desc &d)
+{
+    const DField P = dfield(e, d.f[AF_FIELD_P]);
+    const DField D = dfield(e, d.f[AF_FIELD_DIV]);
+    hipError_t err = launch_rbgs_first(P, D, e->G, e->h2, e->stream);
+    if (err != hipSuccess) return err;
+    err = launch_rbgs(P, D, e->G, e->h2, AF_COLOR_BLACK, true, e->stream);
+    for (int64_t it = 1; it < e->cfg.GaussSeidelIters && err == hipSuccess; it++) {
+        err = launch_rbgs(P, D, e->G, e->h2, AF_COLOR_RED, true, e->stream);
+        if (err == hipSuccess) err = launch_rbgs(P, D, e->G, e->h2, AF_COLOR_BLACK, true, e->stream);
+    }
+    if (err == hipSuccess) err = launch_pressure_bc(P, e->G, e->G.g > 1, e->stream);
+    return err;
+}
+
+hipError_t enq_solve(af_engine *e, const Desc &d)
+{
+    if (e->cfg.Precision == AF_PRECISION_MIXED) {
+        MixedCtx c;
+        c.P = dfield(e, d.f[AF_FIELD_P]);
+        c.D = dfield(e, d.f[AF_FIELD_DIV]);
+        c.G = e->G;
+        c.h2 = e->h2;
+        c.sweeps = e->cfg.MixedSweeps;
+        c.fix_sweeps = e->cfg.MixedFixSweeps;
+        c.cycles = e->cfg.MixedMaxCycles;
+        c.target = e->cfg.MixedTarget;
+        c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
+        return enqueue_mixed_solve(e->mixed, c, e->stream);
+    }
+    return enq_solve_fp64(e, d);
+}
+
+// project, project.go:12-34
+hipError_t enq_project(af_engine *e, const Desc &d)
+{
+    hipError_t err = enq_velocity_bc(e, d);
+    if (err == hipSuccess) err = enq_divergence(e, d);
+    if (err == hipSuccess) err = enq_solve(e, d);
+    if (err == hipSuccess) err = enq_gradient(e, d);
+    if (err == hipSuccess) err = enq_velocity_bc(e, d);
+    return err;
+}
+
+// Step, aeroflux.go:348-409.  d is advanced through the six swaps, which
+// return it to its starting state.
+hipError_t enq_step(af_engine *e, Desc &d)
+{
+    static const int fwd[3] = {AF_AXIS_X, AF_AXIS_Y, AF_AXIS_Z};
+    static const int rev[3] = {AF_AXIS_Z, AF_AXIS_Y, AF_AXIS_X};
+    const double half = e->cfg.DT * 0.5;
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < 3 && err == hipSuccess; k++) {
+        err = enq_velocity_bc(e, d);
+        if (err == hipSuccess) err = enq_sweep(e, d, fwd[k], half);
+        commit(d);
+    }
+    if (err == hipSuccess) err = enq_project(e, d);
+    for (int k = 0; k < 3 && err == hipSuccess; k++) {
+        err = enq_velocity_bc(e, d);
+        if (err == hipSuccess) err = enq_sweep(e, d, rev[k], half);
+        commit(d);
+    }
+    if (err == hipSuccess) err = enq_project(e, d);
+    return err;
+}
+
+int get_graph(af_engine *e, hipGraphExec_t *out)
+{
+    const int s = swap_state(e, e->cur);
+    if (!e->graphs[s]) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal),
+                "begin step capture");
+        Desc d = e->cur;
+        hipError_t err = enq_step(e, d);
+        hipError_t err2 = hipStreamEndCapture(e->stream, &g);
+        if (err != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(err, "capture step");
+        }
+        if (err2 != hipSuccess) return hip_fail(err2, "end step capture");
+        hipGraphExec_t x = nullptr;
+        hipError_t err3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err3 != hipSuccess) return hip_fail(err3, "instantiate step graph");
+        e->graphs[s] = x;
+    }
+    *out = e->graphs[s];
+    return AF_OK;
+}
+
+int sync_stream(af_engine *e, const char *what)
+{
+    HIP_TRY(hipStreamSynchronize(e->stream), what);
+    return AF_OK;
+}
+
+int run_reduction(af_engine *e, double *out)
+{
+    HIP_TRY(hipMemcpyAsync(e->red_host, e->red_out, sizeof(double), hipMemcpyDeviceToHost,
+                           e->stream),
+            "copy reduction");
+    if (int rc = sync_stream(e, "reduction")) return rc;
+    *out = *e->red_host;
+    return AF_OK;
+}
+
+void release(af_engine *e)
+{
+    if (e->closed) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &g : e->graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    release_mixed(e->mixed);
+    if (e->arena) (void)hipFree(e->arena);
+    if (e->flag) (void)hipFree(e->flag);
+    if (e->partials) (void)hipFree(e->partials);
+    if (e->red_out) (void)hipFree(e->red_out);
+    if (e->flag_host) (void)hipHostFree(e->flag_host);
+    if (e->red_host) (void)hipHostFree(e->red_host);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    e->arena = nullptr;
+    e->flag = nullptr;
+    e->partials = e->red_out = e->red_host = nullptr;
+    e->flag_host = nullptr;
+    e->stream = nullptr;
+    e->closed = true;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+const char *af_last_error(void) { return g_err.c_str(); }
+
+int af_config_normalize(af_config *c)
+{
+    if (!c) return fail(AF_ERR_ARG, "aeroflux: null config");
+    // Config.Normalize, aeroflux.go:47-113, check for check.
+    if (c->NX <= 0 || c->NY <= 0 || c->NZ <= 0)
+        return fail(AF_ERR_CONFIG, "aeroflux: interior extents must be positive, got %ldx%ldx%ld",
+                    (long)c->NX, (long)c->NY, (long)c->NZ);
+    if (c->Ghost == 0) c->Ghost = 1;
+    if (c->Ghost < 1)
+        return fail(AF_ERR_CONFIG, "aeroflux: ghost thickness must be >= 1, got %ld", (long)c->Ghost);
+    if (c->H == 0) {
+        int64_t m = c->NX > c->NY ? c->NX : c->NY;
+        m = m > c->NZ ? m : c->NZ;
+        c->H = 1.0 / (double)m;
+    }
+    if (c->H <= 0 || std::isnan(c->H) || std::isinf(c->H))
+        return fail(AF_ERR_CONFIG, "aeroflux: cell size must be finite and positive, got %g", c->H);
+    if (c->Density == 0) c->Density = 1.0;
+    if (c->Density <= 0 || std::isnan(c->Density) || std::isinf(c->Density))
+        return fail(AF_ERR_CONFIG, "aeroflux: density must be finite and positive, got %g",
+                    c->Density);
+    if (c->Viscosity < 0 || std::isnan(c->Viscosity) || std::isinf(c->Viscosity))
+        return fail(AF_ERR_CONFIG, "aeroflux: viscosity must be finite and non-negative, got %g",
+                    c->Viscosity);
+    if (c->DT == 0) {
+        if (c->Viscosity > 0)
+            c->DT = 0.5 * c->H * c->H / (6 * c->Viscosity);
+        else
+            c->DT = 0.25 * c->H;
+    }
+    if (c->DT <= 0 || std::isnan(c->DT) || std::isinf(c->DT))
+        return fail(AF_ERR_CONFIG, "aeroflux: timestep must be finite and positive, got %g", c->DT);
+    if (c->Workers == 0) c->Workers = (int64_t)sysconf(_SC_NPROCESSORS_ONLN);
+    if (c->Workers < 1)
+        return fail(AF_ERR_CONFIG, "aeroflux: worker count must be >= 1, got %ld", (long)c->Workers);
+    if (c->Workers > c->NZ) c->Workers = c->NZ;
+    if (c->GaussSeidelIters == 0) c->GaussSeidelIters = 24;
+    if (c->GaussSeidelIters < 1)
+        return fail(AF_ERR_CONFIG, "aeroflux: gauss-seidel iterations must be >= 1, got %ld",
+                    (long)c->GaussSeidelIters);
+    if (c->Viscosity > 0) {
+        const double limit = c->H * c->H / (6 * c->Viscosity);
+        if (c->DT > limit)
+            return fail(AF_ERR_CONFIG,
+                        "aeroflux: dt %g exceeds explicit diffusion stability limit %g (h=%g, "
+                        "nu=%g); reduce dt or h",
+                        c->DT, limit, c->H, c->Viscosity);
+    }
+    // Extensions (no reference counterpart).
+    if (c->Precision != AF_PRECISION_FP64 && c->Precision != AF_PRECISION_MIXED)
+        return fail(AF_ERR_CONFIG, "aeroflux: unknown precision mode %d", (int)c->Precision);
+    if (c->MixedMaxCycles == 0) c->MixedMaxCycles = 4;
+    if (c->MixedSweeps == 0) c->MixedSweeps = (int32_t)c->GaussSeidelIters;
+    if (c->MixedFixSweeps == 0) c->MixedFixSweeps = 2;
+    if (c->MixedMaxCycles < 1 || c->MixedSweeps < 1 || c->MixedFixSweeps < 1)
+        return fail(AF_ERR_CONFIG, "aeroflux: mixed-precision cycles and sweeps must be >= 1");
+    if (c->MixedTarget < 0 || std::isnan(c->MixedTarget))
+        return fail(AF_ERR_CONFIG, "aeroflux: mixed-precision target must be >= 0");
+    return AF_OK;
+}
+
+int64_t af_mac_bytes(int64_t nx, int64_t ny, int64_t nz, int64_t ghost)
+{
+    if (nx < 1 || ny < 1 || nz < 1 || ghost < 1) return 0;
+    af_field f[AF_FIELD_COUNT];
+    return mac_layout(f, nx, ny, nz, ghost);
+}
+
+int af_new(const af_config *cfg_in, af_engine **out)
+{
+    if (!cfg_in || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *out = nullptr;
+    af_config cfg = *cfg_in;
+    if (int rc = af_config_normalize(&cfg)) return rc;
+    if (cfg.UseSparseSolver)
+        return fail(AF_ERR_UNSUPPORTED,
+                    "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
+                    "Red-Black iterative solver is used instead");
+
+    af_engine *e = new (std::nothrow) af_engine();
+    if (!e) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
+    e->cfg = cfg;
+    e->bytes = mac_layout(e->cur.f, cfg.NX, cfg.NY, cfg.NZ, cfg.Ghost);
+    for (int i = 0; i < AF_FIELD_COUNT; i++) {
+        e->orig_off[i] = e->cur.f[i].Off;
+        if (e->cur.f[i].Count >= (int64_t)1 << 31) {
+            delete e;
+            return fail(AF_ERR_CONFIG,
+                        "aeroflux: a %ldx%ldx%ld grid exceeds 2^31 elements per field",
+                        (long)cfg.NX, (long)cfg.NY, (long)cfg.NZ);
+        }
+    }
+    e->G.NX = (int)cfg.NX;
+    e->G.NY = (int)cfg.NY;
+    e->G.NZ = (int)cfg.NZ;
+    e->G.g = (int)cfg.Ghost;
+    e->invH = 1.0 / cfg.H;                          // stencil.go:47, project.go:216
+    e->h2 = cfg.H * cfg.H;                          // project.go:107
+    e->div_scale = cfg.Density / (cfg.DT * cfg.H);  // project.go:49
+    e->grad_scale = cfg.DT / (cfg.Density * cfg.H); // project.go:179
+
+    hipError_t err = hipGetDevice(&e->device);
+    if (err != hipSuccess) {
+        delete e;
+        return hip_fail(err, "no HIP device");
+    }
+    e->closed = false;
+    auto bail = [&](hipError_t er, const char *what, int code) {
+        int rc = code == AF_ERR_OOM
+                     ? fail(AF_ERR_OOM, "aeroflux: %s (%ld bytes): %s", what, (long)e->bytes,
+                            hipGetErrorString(er))
+                     : hip_fail(er, what);
+        release(e);
+        delete e;
+        return rc;
+    };
+    if ((err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(err, "create stream", AF_ERR_HIP);
+    if ((err = hipMalloc(&e->arena, (size_t)e->bytes)) != hipSuccess)
+        return bail(err, "allocate arena", AF_ERR_OOM);
+    if ((err = hipMemsetAsync(e->arena, 0, (size_t)e->bytes, e->stream)) != hipSuccess)
+        return bail(err, "zero arena", AF_ERR_HIP);  // MAC.Zero, grid.go:158-163
+    if ((err = hipMalloc(&e->flag, sizeof(int))) != hipSuccess)
+        return bail(err, "allocate flag", AF_ERR_OOM);
+    if ((err = hipMemsetAsync(e->flag, 0, sizeof(int), e->stream)) != hipSuccess)
+        return bail(err, "zero flag", AF_ERR_HIP);
+    const int np = reduce_partials_needed(e->G);
+    if ((err = hipMalloc(&e->partials, sizeof(double) * (size_t)np)) != hipSuccess)
+        return bail(err, "allocate partials", AF_ERR_OOM);
+    if ((err = hipMalloc(&e->red_out, sizeof(double) * 4)) != hipSuccess)
+        return bail(err, "allocate reduction", AF_ERR_OOM);
+    if ((err = hipHostMalloc(&e->flag_host, sizeof(int), hipHostMallocDefault)) != hipSuccess)
+        return bail(err, "allocate pinned flag", AF_ERR_OOM);
+    if ((err = hipHostMalloc(&e->red_host, sizeof(double) * 4, hipHostMallocDefault)) != hipSuccess)
+        return bail(err, "allocate pinned result", AF_ERR_OOM);
+    if (cfg.Precision == AF_PRECISION_MIXED) {
+        if ((err = init_mixed(e->mixed, e->G, dfield(e, e->cur.f[AF_FIELD_P]), e->stream)) !=
+            hipSuccess)
+            return bail(err, "allocate mixed-precision buffers", AF_ERR_OOM);
+    }
+    if ((err = hipStreamSynchronize(e->stream)) != hipSuccess)
+        return bail(err, "initialise engine", AF_ERR_HIP);
+    *out = e;
+    return AF_OK;
+}
+
+int af_close(af_engine *e)
+{
+    if (!e) return fail(AF_ERR_ARG, "aeroflux: null engine");
+    release(e);
+    return AF_OK;
+}
+
+void af_free(af_engine *e)
+{
+    if (!e) return;
+    release(e);
+    delete e;
+}
+
+int af_get_config(const af_engine *e, af_config *out)
+{
+    if (!e || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *out = e->cfg;
+    return AF_OK;
+}
+
+int af_step(af_engine *e) { return af_step_n(e, 1, nullptr); }
+
+int af_step_n(af_engine *e, int64_t n, int64_t *first_fault_step)
+{
+    if (int rc = check_open(e)) return rc;
+    if (n < 0) return fail(AF_ERR_ARG, "aeroflux: negative step count");
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    hipGraphExec_t g;
+    if (int rc = get_graph(e, &g)) return rc;
+    // The flag is cleared once; a fault in step k stays raised.  To name the
+    // first faulting step, the per-step flag is read only when one is asked
+    // for and n > 1.
+    const bool per_step = first_fault_step && n > 1;
+    HIP_TRY(launch_set_int(e->flag, 0, e->stream), "clear fault flag");
+    for (int64_t k = 0; k < n; k++) {
+        HIP_TRY(hipGraphLaunch(g, e->stream), "launch step graph");
+        if (per_step) {
+            HIP_TRY(hipMemcpyAsync(e->flag_host, e->flag, sizeof(int), hipMemcpyDeviceToHost,
+                                   e->stream),
+                    "read fault flag");
+            if (int rc = sync_stream(e, "step")) return rc;
+            if (*e->flag_host) {
+                *first_fault_step = k;
+                return fail(AF_ERR_FAULT,
+                            "aeroflux: device fault contained: non-finite departure coordinate "
+                            "in an advection sweep (step %ld)",
+                            (long)k);
+            }
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(e->flag_host, e->flag, sizeof(int), hipMemcpyDeviceToHost, e->stream),
+            "read fault flag");
+    if (int rc = sync_stream(e, "step")) return rc;
+    if (*e->flag_host) {
+        if (first_fault_step) *first_fault_step = 0;
+        return fail(AF_ERR_FAULT,
+                    "aeroflux: device fault contained: non-finite departure coordinate in an "
+                    "advection sweep");
+    }
+    return AF_OK;
+}
+
+int af_max_divergence(af_engine *e, double *out)
+{
+    if (int rc = check_open(e)) return rc;
+    if (!out) return fail(AF_ERR_ARG, "aeroflux: null output");
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    const Desc &d = e->cur;
+    HIP_TRY(launch_max_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                                  dfield(e, d.f[AF_FIELD_W]), e->G, e->invH, e->partials,
+                                  e->red_out, e->stream),
+            "max divergence");
+    return run_reduction(e, out);
+}
+
+int af_kinetic_energy(af_engine *e, double *out)
+{
+    if (int rc = check_open(e)) return rc;
+    if (!out) return fail(AF_ERR_ARG, "aeroflux: null output");
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    const Desc &d = e->cur;
+    HIP_TRY(launch_kinetic_energy(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                                  dfield(e, d.f[AF_FIELD_W]), e->G, e->partials, e->red_out,
+                                  e->stream),
+            "kinetic energy");
+    double sum;
+    if (int rc = run_reduction(e, &sum)) return rc;
+    const double vol = e->cfg.H * e->cfg.H * e->cfg.H;
+    *out = 0.5 * sum * vol;  // project.go:257
+    return AF_OK;
+}
+
+int af_max_velocity(af_engine *e, double *out)
+{
+    if (int rc = check_open(e)) return rc;
+    if (!out) return fail(AF_ERR_ARG, "aeroflux: null output");
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    const Desc &d = e->cur;
+    const DField U = dfield(e, d.f[AF_FIELD_U]), V = dfield(e, d.f[AF_FIELD_V]),
+                 W = dfield(e, d.f[AF_FIELD_W]);
+    HIP_TRY(launch_max_abs3(U.d, d.f[AF_FIELD_U].Count, V.d, d.f[AF_FIELD_V].Count, W.d,
+                            d.f[AF_FIELD_W].Count, e->partials, e->red_out, e->stream),
+            "max velocity");
+    return run_reduction(e, out);
+}
+
+int af_cfl(af_engine *e, double *out)
+{
+    double mv;
+    if (int rc = af_max_velocity(e, &mv)) return rc;
+    *out = mv * e->cfg.DT / e->cfg.H;  // stencil.go:337
+    return AF_OK;
+}
+
+int af_enable_sparse_solver(af_engine *e)
+{
+    (void)e;
+    return fail(AF_ERR_UNSUPPORTED,
+                "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
+                "Red-Black iterative solver is used instead");
+}
+
+int af_field_desc(const af_engine *e, int id, af_field *out)
+{
+    if (!e || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    if (id < 0 || id >= AF_FIELD_COUNT) return fail(AF_ERR_ARG, "aeroflux: bad field id %d", id);
+    *out = e->cur.f[id];
+    return AF_OK;
+}
+
+int af_field_read(af_engine *e, int id, double *host, int64_t count)
+{
+    if (int rc = check_open(e)) return rc;
+    if (id < 0 || id >= AF_FIELD_COUNT || !host)
+        return fail(AF_ERR_ARG, "aeroflux: bad field read arguments");
+    const af_field &f = e->cur.f[id];
+    if (count != f.Count)
+        return fail(AF_ERR_ARG, "aeroflux: field %d has %ld elements, got %ld", id, (long)f.Count,
+                    (long)count);
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    HIP_TRY(hipMemcpyAsync(host, e->arena + f.Off, (size_t)count * 8, hipMemcpyDeviceToHost,
+                           e->stream),
+            "read field");
+    return sync_stream(e, "read field");
+}
+
+int af_field_write(af_engine *e, int id, const double *host, int64_t count)
+{
+    if (int rc = check_open(e)) return rc;
+    if (id < 0 || id >= AF_FIELD_COUNT || !host)
+        return fail(AF_ERR_ARG, "aeroflux: bad field write arguments");
+    const af_field &f = e->cur.f[id];
+    if (count != f.Count)
+        return fail(AF_ERR_ARG, "aeroflux: field %d has %ld elements, got %ld", id, (long)f.Count,
+                    (long)count);
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    HIP_TRY(hipMemcpyAsync(e->arena + f.Off, host, (size_t)count * 8, hipMemcpyHostToDevice,
+                           e->stream),
+            "write field");
+    return sync_stream(e, "write field");
+}
+
+int af_field_device_ptr(const af_engine *e, int id, uint64_t *out)
+{
+    if (!e || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    if (e->closed) return fail(AF_ERR_CLOSED, "aeroflux: engine is closed");
+    if (id < 0 || id >= AF_FIELD_COUNT) return fail(AF_ERR_ARG, "aeroflux: bad field id %d", id);
+    *out = (uint64_t)(uintptr_t)(e->arena + e->cur.f[id].Off);
+    return AF_OK;
+}
+
+int64_t af_arena_bytes(const af_engine *e) { return e ? e->bytes : 0; }
+
+// ---- stage entry points ------------------------------------------------
+
+#define STAGE_BEGIN(e)                                  \
+    if (int rc = check_open(e)) return rc;              \
+    HIP_TRY(hipSetDevice((e)->device), "set device")
+
+int af_apply_velocity_bc(af_engine *e)
+{
+    STAGE_BEGIN(e);
+    HIP_TRY(enq_velocity_bc(e, e->cur), "velocity bc");
+    return sync_stream(e, "velocity bc");
+}
+
+int af_sweep_axis(af_engine *e, int axis, double dt)
+{
+    STAGE_BEGIN(e);
+    if (axis < 0 || axis > 2) return fail(AF_ERR_ARG, "aeroflux: bad axis %d", axis);
+    HIP_TRY(launch_set_int(e->flag, 0, e->stream), "clear fault flag");
+    HIP_TRY(enq_sweep(e, e->cur, axis, dt), "advection sweep");
+    HIP_TRY(hipMemcpyAsync(e->flag_host, e->flag, sizeof(int), hipMemcpyDeviceToHost, e->stream),
+            "read fault flag");
+    if (int rc = sync_stream(e, "advection sweep")) return rc;
+    if (*e->flag_host)
+        return fail(AF_ERR_FAULT,
+                    "aeroflux: device fault contained: non-finite departure coordinate in an "
+                    "advection sweep");
+    return AF_OK;
+}
+
+int af_commit_velocity(af_engine *e)
+{
+    if (int rc = check_open(e)) return rc;
+    commit(e->cur);
+    return AF_OK;
+}
+
+int af_compute_divergence(af_engine *e)
+{
+    STAGE_BEGIN(e);
+    HIP_TRY(enq_divergence(e, e->cur), "divergence");
+    return sync_stream(e, "divergence");
+}
+
+int af_gs_sweep(af_engine *e, int color)
+{
+    STAGE_BEGIN(e);
+    if (color != 0 && color != 1) return fail(AF_ERR_ARG, "aeroflux: bad colour %d", color);
+    HIP_TRY(launch_rbgs(dfield(e, e->cur.f[AF_FIELD_P]), dfield(e, e->cur.f[AF_FIELD_DIV]), e->G,
+                        e->h2, color, false, e->stream),
+            "gauss-seidel sweep");
+    return sync_stream(e, "gauss-seidel sweep");
+}
+
+int af_apply_pressure_bc(af_engine *e)
+{
+    STAGE_BEGIN(e);
+    HIP_TRY(launch_pressure_bc(dfield(e, e->cur.f[AF_FIELD_P]), e->G, false, e->stream),
+            "pressure bc");
+    return sync_stream(e, "pressure bc");
+}
+
+int af_solve_pressure(af_engine *e)
+{
+    STAGE_BEGIN(e);
+    HIP_TRY(enq_solve(e, e->cur), "pressure solve");
+    return sync_stream(e, "pressure solve");
+}
+
+int af_subtract_gradient(af_engine *e)
+{
+    STAGE_BEGIN(e);
+    HIP_TRY(enq_gradient(e, e->cur), "gradient");
+    return sync_stream(e, "gradient");
+}
+
+int af_project(af_engine *e)
+{
+    STAGE_BEGIN(e);
+    HIP_TRY(enq_project(e, e->cur), "project");
+    return sync_stream(e, "project");
+}
+
+int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles)
+{
+    STAGE_BEGIN(e);
+    if (e->cfg.Precision != AF_PRECISION_MIXED)
+        return fail(AF_ERR_ARG, "aeroflux: engine is not in mixed precision");
+    double d = 0;
+    int c = 0;
+    HIP_TRY(hipMemcpyAsync(e->red_host, e->mixed.last_res, sizeof(double), hipMemcpyDeviceToHost,
+                           e->stream),
+            "read mixed stats");
+    if (int rc = sync_stream(e, "mixed stats")) return rc;
+    d = *e->red_host;
+    HIP_TRY(hipMemcpyAsync(e->flag_host, e->mixed.cycles, sizeof(int), hipMemcpyDeviceToHost,
+                           e->stream),
+            "read mixed stats");
+    if (int rc = sync_stream(e, "mixed stats")) return rc;
+    c = *e->flag_host;
+    if (last_div) *last_div = d;
+    if (cycles) *cycles = c;
+    return AF_OK;
+}
+
+int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
+{
+    STAGE_BEGIN(e);
+    if (reps < 1 || !avg_ms) return fail(AF_ERR_ARG, "aeroflux: bad timing arguments");
+    const Desc &d = e->cur;
+    const DField P = dfield(e, d.f[AF_FIELD_P]), D = dfield(e, d.f[AF_FIELD_DIV]);
+    auto one = [&]() -> hipError_t {
+        switch (kernel) {
+        case 0: return launch_rbgs(P, D, e->G, e->h2, AF_COLOR_RED, true, e->stream);
+        case 1: return enq_sweep(e, d, AF_AXIS_X, e->cfg.DT * 0.5);
+        case 2: return enq_divergence(e, d);
+        case 3: return enq_gradient(e, d);
+        default: return hipErrorInvalidValue;
+        }
+    };
+    if (kernel < 0 || kernel > 3) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
+    HIP_TRY(one(), "warm-up launch");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), "create event");
+    HIP_TRY(hipEventCreate(&b), "create event");
+    hipError_t err = hipEventRecord(a, e->stream);
+    for (int r = 0; r < reps && err == hipSuccess; r++) err = one();
+    if (err == hipSuccess) err = hipEventRecord(b, e->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(b);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (err != hipSuccess) return hip_fail(err, "time kernel");
+    *avg_ms = (double)ms / reps;
+    return AF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+// af_internal.h -- shared declarations between the engine (af_engine.hip)
+// and the kernel launchers (af_kernels.hip).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace af {
+
+// One field as the kernels see it: base pointer and padded geometry.
+// Element counts are checked on the host to fit int32 (max 2^31-1 per field).
+struct DField {
+    double *d;
+    int sy, sz;
+    int nx, ny, nz;
+};
+
+// Interior extents and ghost width.
+struct Geom {
+    int NX, NY, NZ, g;
+};
+
+struct SweepArgs {
+    DField U, V, W;        // current velocity descriptors (sources)
+    DField T0, T1, T2;     // destinations (Tmp0/1/2)
+    Geom G;
+    double dt, invH, lam;
+    int *flag;             // set to 1 on a NaN departure coordinate
+};
+
+// Launchers.  All enqueue on `s` and return the hipGetLastError() code.
+hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t s);
+hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s);
+hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
+                             hipStream_t s);
+// Red-black colour pass.  fused_bc: a neighbour outside the interior reads the
+// cell's own value (the Neumann mirror applied just before the pass).
+hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fused_bc,
+                       hipStream_t s);
+// First red pass of a solve from P == 0: writes every interior cell.
+hipError_t launch_rbgs_first(DField P, DField D, Geom G, double h2, hipStream_t s);
+// Pressure ghost fill (Neumann mirror); outer layers (ghost > 1) are zeroed
+// when zero_outer is set.
+hipError_t launch_pressure_bc(DField P, Geom G, bool zero_outer, hipStream_t s);
+hipError_t launch_zero(double *d, int64_t n, hipStream_t s);
+hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
+                           hipStream_t s);
+hipError_t launch_set_int(int *p, int v, hipStream_t s);
+
+// Reductions: two launches, fixed tree, no atomics.  `partials` must hold
+// reduce_partials_needed(G) doubles; the result lands in *out (device).
+int reduce_partials_needed(Geom G);
+hipError_t launch_max_divergence(DField U, DField V, DField W, Geom G, double invH,
+                                 double *partials, double *out, hipStream_t s);
+hipError_t launch_kinetic_energy(DField U, DField V, DField W, Geom G, double *partials,
+                                 double *out, hipStream_t s);
+hipError_t launch_max_abs3(const double *u, int64_t nu, const double *v, int64_t nv,
+                           const double *w, int64_t nw, double *partials, double *out,
+                           hipStream_t s);
+
+}  // namespace af

@@ -1,0 +1,235 @@
+#!/usr/bin/env python3
+"""Benchmark of the AeroFlux step data plane on one MI355X.
+
+Default workload: 256^3 MAC grid, H=1/256, DT=0.001, nu=1e-5, seeded
+Taylor-Green + U(+-1e-3) noise, FP64, the reference's default 24 red-black
+sweep pairs per projection.  W warm-up Step()s, then K timed Step()s
+bracketed by device synchronisation.  Prints ONE JSON line:
+
+* value: cell-updates/s = NX*NY*NZ*K / wall seconds (bench_test.go:32);
+* hbm_gbps_effective: algorithmic bytes (288 + 2*(88 + 48*I) per cell per
+  step, I = sweep pairs) / wall seconds;
+* roofline: the dominant kernel (the red-black colour pass, 24 B/cell of
+  algorithmic traffic: read P, read Div, write P) timed with HIP events on
+  the engine's stream;
+* cpu_baseline: the C restatement of the Go reference timed on this host
+  on a bounded sample of the same workload;
+* mixed: the same workload with FP32 red-black smoothing inside FP64 defect
+  correction, iterated to the FP64 run's final max divergence, with its
+  speed-up and per-field error against the FP64 run.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def bytes_per_cell_step(iters: int) -> int:
+    """Algorithmic FP64 bytes per interior cell per Step: 6 sweeps x (3 reads
+    + 3 writes) x 8 B, plus per projection divergence 32 B, 2*iters colour
+    passes of 24 B, gradient 56 B."""
+    return 288 + 2 * (88 + 48 * iters)
+
+
+def hip_sync() -> None:
+    rc = ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def load_ic(e, f) -> None:
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), f[name])
+    e.apply_velocity_bc()
+
+
+def traffic_from_profile(name: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy kernel of known size) for this workload,
+    read from the committed profile summary when present."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("traffic_bytes_per_launch")
+
+
+def cpu_baseline(cfg: dict, f: dict, seconds: float, workers: int) -> dict:
+    from oracle import oracle
+
+    ocfg = dict(cfg, Workers=workers)
+    o = oracle.OracleEngine(ocfg)
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        o.field(fid)[...] = f[name]
+    o.apply_velocity_bc()
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        if o.step() != 0:
+            raise RuntimeError("oracle step faulted")
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    o.close()
+    cells = cfg["NX"] * cfg["NY"] * cfg["NZ"]
+    return {
+        "value": cells * steps / el,
+        "unit": "cell-updates/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": f"{steps} Step() calls of the same {cfg['NX']}x{cfg['NY']}x{cfg['NZ']} workload "
+                  f"(first steps from the same initial condition), {el:.1f} s, "
+                  f"C restatement of stencil.go/project.go, Z-slab pool of {workers} threads",
+    }
+
+
+def run_gpu(af, cfg: dict, f: dict, warmup: int, steps: int):
+    e = af.New(af.Config(**cfg))
+    load_ic(e, f)
+    if warmup:
+        e.StepN(warmup)
+    hip_sync()
+    t0 = time.perf_counter()
+    e.StepN(steps)
+    hip_sync()
+    el = time.perf_counter() - t0
+    return e, el
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256, help="grid edge (default 256)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-workers", type=int, default=int(os.environ.get("AF_CPU_WORKERS", "16")))
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only; the step data plane is a single-GPU "
+              "path here (multi-GPU sharding is a design note, not a measured mode).",
+              file=sys.stderr)
+        return 2
+
+    import aeroflux_amd as af
+    from aeroflux_amd import ic
+
+    af.load_library()
+    n = args.n
+    h = 1.0 / n
+    cfg = dict(NX=n, NY=n, NZ=n, H=h, DT=0.001, Viscosity=1e-5)
+    iters = 24
+    cells = n ** 3
+    f = ic.taylor_green(n, n, n, 1, h, seed=1)
+
+    e, el = run_gpu(af, cfg, f, args.warmup, args.steps)
+    d64 = e.MaxDivergence()
+    ms_per_step = 1e3 * el / args.steps
+    value = cells * args.steps / el
+    eff_gbps = bytes_per_cell_step(iters) * cells * args.steps / el / 1e9
+
+    # Dominant kernel: one red-black colour pass.
+    reps = 40
+    t_rb = e.time_kernel(0, reps)
+    rb_bytes = 24 * cells
+    achieved = rb_bytes / (t_rb * 1e-3) / 1e9
+    kernels = {
+        "rbgs_pass_ms": t_rb,
+        "advect_sweep_ms": e.time_kernel(1, 10),
+        "divergence_ms": e.time_kernel(2, 10),
+        "gradient_ms": e.time_kernel(3, 10),
+    }
+    fp64_fields = None
+    if not args.no_mixed:
+        fp64_fields = {k: e.Mesh.Data(e.Mesh.field(i)) for i, k in enumerate("PUVW")}
+    e.Close()
+
+    out = {
+        "metric": "cell-updates/s & HBM GB/s at 256³ FP64 vs mixed; p50/p99 step-job latency",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded Taylor-Green + U(-1e-3,1e-3) noise (SplitMix64 seed 1)",
+        "config": {
+            "workload": f"plain FP64 step, {n}^3 MAC grid, H=1/{n}, DT=0.001, nu=1e-5, "
+                        f"{iters} red-black sweep pairs, Strang X,Y,Z,[P],Z,Y,X,[P]",
+            "grid": [n, n, n],
+            "gauss_seidel_iters": iters,
+            "precision": "fp64",
+        },
+        "hbm_gbps_effective": eff_gbps,
+        "roofline_frac_effective": eff_gbps / HBM_PEAK_GBPS,
+        "final_max_divergence": d64,
+        "roofline": {
+            "kernel": "k_rbgs<true> (red-black colour pass, Neumann mirror fused)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": traffic_from_profile(f"rbgs_{n}_pmc.json"),
+            "algorithmic_bytes_per_launch": rb_bytes,
+            "avg_launch_ms": t_rb,
+        },
+        "kernels": kernels,
+    }
+
+    if not args.no_mixed:
+        mcfg = dict(cfg, Precision=af.PRECISION_MIXED, MixedTarget=d64)
+        m, mel = run_gpu(af, mcfg, f, args.warmup, args.steps)
+        dmx = m.MaxDivergence()
+        errs = {}
+        for i, k in enumerate("PUVW"):
+            a = m.Mesh.Data(m.Mesh.field(i))
+            ref = fp64_fields[k]
+            errs[k] = float(np.abs(a - ref).max() / max(np.abs(ref).max(), 1e-300))
+        _, cyc = m.mixed_stats()
+        m.Close()
+        mval = cells * args.steps / mel
+        out["mixed"] = {
+            "value": mval,
+            "unit": "cell-updates/s",
+            "ms_per_step": 1e3 * mel / args.steps,
+            "speedup_vs_fp64": mval / value,
+            "final_max_divergence": dmx,
+            "fp64_final_max_divergence": d64,
+            "target": "per projection: max|div| implied by the FP64 residual <= the FP64 run's "
+                      "final max|div|",
+            "cycles_last_solve": cyc,
+            "max_rel_error_vs_fp64": errs,
+        }
+    del fp64_fields
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, f, args.cpu_seconds, args.cpu_workers)
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,202 @@
+/*
+ * aeroflux.h -- C ABI of the MI355X-native AeroFlux step data plane.
+ *
+ * This is the drop-in boundary beneath the reference's Go API: a cgo shim
+ * keeps `aeroflux.New(Config) / Step() / MaxDivergence() / Close()` unchanged
+ * and calls these functions.  Every entry point names the reference function
+ * it replaces (file:line in the AeroFlex Go package).  Signatures use only
+ * plain C types: fixed-width integers, doubles, and host pointers; device
+ * addresses cross as uint64_t, mirroring the reference's uintptr discipline
+ * (accelerate_darwin.go:21-24).
+ *
+ * Threading: one engine is driven from one host thread at a time (as
+ * Step/project/diagnostics are called from one goroutine in the reference).
+ * Each engine owns one HIP stream at default priority; no host threads are
+ * created.  Errors are integer codes; af_last_error() returns the message of
+ * the most recent failure on the calling thread, prefixed "aeroflux: ".
+ */
+#ifndef AEROFLUX_H
+#define AEROFLUX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- codes */
+enum {
+    AF_OK = 0,
+    AF_ERR_CONFIG = 1,      /* Config.Normalize rejection (aeroflux.go:47-113) */
+    AF_ERR_FAULT = 2,       /* contained device fault: non-finite departure
+                               coordinate (the reference's guard-page /
+                               bounds-panic path, arena.go:230-243) */
+    AF_ERR_OOM = 3,         /* device arena allocation failed */
+    AF_ERR_HIP = 4,         /* HIP runtime error */
+    AF_ERR_ARG = 5,         /* bad argument (field id, size, null) */
+    AF_ERR_CLOSED = 6,      /* engine already closed */
+    AF_ERR_UNSUPPORTED = 7  /* feature absent on this platform */
+};
+
+/* Field ids in arena order, NewMAC (grid.go:119-127). */
+enum {
+    AF_FIELD_P = 0,
+    AF_FIELD_U = 1,
+    AF_FIELD_V = 2,
+    AF_FIELD_W = 3,
+    AF_FIELD_DIV = 4,
+    AF_FIELD_TMP0 = 5,
+    AF_FIELD_TMP1 = 6,
+    AF_FIELD_TMP2 = 7,
+    AF_FIELD_COUNT = 8
+};
+
+enum { AF_AXIS_X = 0, AF_AXIS_Y = 1, AF_AXIS_Z = 2 };      /* stencil.go:7-11 */
+enum { AF_COLOR_RED = 0, AF_COLOR_BLACK = 1 };              /* layout.go:30-33 */
+
+/* Pressure-solve precision (no counterpart in the reference, whose solve is
+ * FP64 only).  AF_PRECISION_FP64 is the reference algorithm, bit for bit. */
+enum {
+    AF_PRECISION_FP64 = 0,
+    AF_PRECISION_MIXED = 1  /* FP32 red-black smoothing inside an FP64
+                               defect-correction loop */
+};
+
+/* Config, aeroflux.go:14-41.  The first block mirrors the Go struct field
+ * for field (Go int -> int64_t, bool -> int32_t).  The trailing block holds
+ * MI355X-build extensions; their zero values select reference behaviour. */
+typedef struct af_config {
+    int64_t NX, NY, NZ;
+    int64_t Ghost;
+    double H;
+    double Viscosity;
+    double Density;
+    double DT;
+    int64_t Workers;          /* CPU worker count in the reference; kept and
+                                 normalised identically, unused on the GPU */
+    int64_t GaussSeidelIters;
+    int32_t UseSparseSolver;  /* must stay 0: no direct solver here */
+    int32_t Lock;             /* accepted; device memory is always resident */
+
+    /* --- extensions --- */
+    int32_t Precision;        /* AF_PRECISION_* */
+    int32_t MixedMaxCycles;   /* mixed: max defect-correction cycles (0 -> 4) */
+    int32_t MixedSweeps;      /* mixed: FP32 sweep pairs per cycle (0 ->
+                                 GaussSeidelIters) */
+    int32_t MixedFixSweeps;   /* mixed: FP32 sweep pairs in each later
+                                 correction cycle (0 -> 2) */
+    double MixedTarget;       /* mixed: stop once the post-projection
+                                 max|div| implied by the FP64 residual is
+                                 <= this (0 -> run exactly one cycle) */
+} af_config;
+
+/* Field, grid.go:12-21 (descriptor after any commitVelocity swaps). */
+typedef struct af_field {
+    int64_t Off;   /* byte offset of element (0,0,0) in the arena */
+    int64_t NX, NY, NZ;
+    int64_t SY, SZ;
+    int64_t Count;
+} af_field;
+
+typedef struct af_engine af_engine;
+
+/* ------------------------------------------------------------ lifecycle */
+
+/* Config.Normalize, aeroflux.go:47-113: fills defaults, rejects bad
+ * geometry.  Workers defaults to the host's online CPU count (GOMAXPROCS). */
+int af_config_normalize(af_config *cfg);
+
+/* MACBytes, grid.go:138-144: arena footprint, 0 for an invalid geometry. */
+int64_t af_mac_bytes(int64_t nx, int64_t ny, int64_t nz, int64_t ghost);
+
+/* New, aeroflux.go:176-213: normalises a copy of *cfg, allocates the zeroed
+ * HBM arena on the current HIP device, and prepares the per-step graph. */
+int af_new(const af_config *cfg, af_engine **out);
+
+/* Close, aeroflux.go:419-439: releases device memory; idempotent.  The
+ * handle stays valid (closed) until af_free. */
+int af_close(af_engine *e);
+
+/* Frees a handle (closing it first if needed).  No reference counterpart:
+ * the Go engine struct is garbage-collected. */
+void af_free(af_engine *e);
+
+/* Copy of the engine's normalised config (Engine.Cfg, aeroflux.go:134). */
+int af_get_config(const af_engine *e, af_config *out);
+
+/* ------------------------------------------------------------- hot path */
+
+/* Step, aeroflux.go:348-409: one Strang-palindrome step X,Y,Z,[project],
+ * Z,Y,X,[project], replayed as one hipGraph.  Blocks until done; returns
+ * AF_ERR_FAULT if any departure coordinate was NaN (state then unspecified,
+ * as after a contained fault in the reference). */
+int af_step(af_engine *e);
+
+/* n Steps back to back with one host synchronisation at the end.  On a
+ * fault, *first_fault_step (if non-null) receives the index of the first
+ * faulting step.  No reference counterpart (a loop of Step calls). */
+int af_step_n(af_engine *e, int64_t n, int64_t *first_fault_step);
+
+/* ---------------------------------------------------------- diagnostics */
+
+int af_max_divergence(af_engine *e, double *out);   /* project.go:210-232 */
+int af_kinetic_energy(af_engine *e, double *out);   /* project.go:238-258 */
+int af_max_velocity(af_engine *e, double *out);     /* stencil.go:319-331 */
+int af_cfl(af_engine *e, double *out);              /* stencil.go:336-338 */
+
+/* EnableSparseSolver, sparse_fallback.go:17-19: always fails here, with the
+ * reference's non-Darwin behaviour. */
+int af_enable_sparse_solver(af_engine *e);
+
+/* --------------------------------------------------------- field access */
+
+/* Field descriptor (MAC.P/U/..., grid.go:60-67) after descriptor swaps. */
+int af_field_desc(const af_engine *e, int field_id, af_field *out);
+
+/* MAC.Data (grid.go:147-149) cannot alias HBM; these copy a whole padded
+ * field between host memory and the arena.  count must equal Field.Count. */
+int af_field_read(af_engine *e, int field_id, double *host, int64_t count);
+int af_field_write(af_engine *e, int field_id, const double *host, int64_t count);
+
+/* Device address of a field's element (0,0,0), for device-side consumers. */
+int af_field_device_ptr(const af_engine *e, int field_id, uint64_t *out);
+
+/* Total arena bytes (MAC.Bytes, grid.go:128). */
+int64_t af_arena_bytes(const af_engine *e);
+
+/* ----------------------------------------- stage entry points (parity) */
+/* These mirror the engine's unexported methods that the reference's own
+ * in-package tests call (engine_test.go:121, :334). */
+
+int af_apply_velocity_bc(af_engine *e);                 /* stencil.go:246-315 */
+int af_sweep_axis(af_engine *e, int axis, double dt);   /* stencil.go:21-25, one dispatch */
+int af_commit_velocity(af_engine *e);                   /* stencil.go:230-235 */
+int af_compute_divergence(af_engine *e);                /* project.go:42-62 */
+int af_gs_sweep(af_engine *e, int color);               /* project.go:103-130 */
+int af_apply_pressure_bc(af_engine *e);                 /* project.go:139-163 */
+int af_solve_pressure(af_engine *e);                    /* project.go:73-100 */
+int af_subtract_gradient(af_engine *e);                 /* project.go:172-205 */
+int af_project(af_engine *e);                           /* project.go:12-34 */
+
+/* -------------------------------------------------------- measurement */
+
+/* Average device time (ms, HIP events on the engine's stream) of `reps`
+ * back-to-back launches of one kernel of the step, launched exactly as the
+ * step launches it.  kernel: 0 = red-black pressure pass, 1 = advection
+ * sweep (X axis), 2 = divergence, 3 = gradient subtract.  No reference
+ * counterpart (measurement only); it overwrites P and the Tmp fields. */
+int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
+
+/* Mixed precision only: max|div| implied by the last FP64 residual check
+ * (0 if MixedTarget == 0) and the cycles the last solve ran.  No reference
+ * counterpart. */
+int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *af_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AEROFLUX_H */

@@ -1,0 +1,112 @@
+"""CPU checks of the C ABI library: it loads, exports every symbol that
+include/aeroflux.h declares, and its host-side logic (Config.Normalize,
+MACBytes) matches the reference's tests.  No compute calls (no GPU here)."""
+import math
+import os
+import re
+
+import pytest
+
+import aeroflux_amd as af
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(af.lib_path()):
+        from aeroflux_amd import build
+        build.build()
+    return af.load_library()
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "aeroflux.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(af_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_reference_boundary():
+    syms = declared_symbols()
+    for s in ("af_new", "af_step", "af_max_divergence", "af_close", "af_config_normalize",
+              "af_project", "af_field_read", "af_field_write"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing
+
+
+def test_python_mirror_binds_every_symbol():
+    bound = {name for name, _, _ in af._SIGNATURES}
+    assert bound == set(declared_symbols())
+
+
+def test_normalize_defaults(lib):
+    # engine_test.go:18-35
+    c = af.Config(NX=16, NY=16, NZ=16)
+    c.Normalize()
+    assert c.Ghost == 1 and c.Density == 1.0 and c.H > 0 and c.DT > 0 and c.Workers >= 1
+    assert c.GaussSeidelIters == 24 and c.H == 1 / 16 and c.DT == 0.25 / 16
+
+
+@pytest.mark.parametrize("kw", [
+    dict(NX=0, NY=8, NZ=8), dict(NX=-4, NY=8, NZ=8), dict(NX=8, NY=8, NZ=8, Ghost=-1),
+    dict(NX=8, NY=8, NZ=8, Viscosity=-1.0), dict(NX=8, NY=8, NZ=8, H=math.nan),
+    dict(NX=8, NY=8, NZ=8, DT=math.inf), dict(NX=8, NY=8, NZ=8, Density=-2.0),
+    dict(NX=8, NY=8, NZ=8, H=0.1, Viscosity=1.0, DT=1.0),
+])
+def test_normalize_rejects(lib, kw):
+    # engine_test.go:37-59
+    with pytest.raises(af.AeroFluxError, match="^aeroflux: ") as ei:
+        af.Config(**kw).Normalize()
+    assert ei.value.code == af.AF_ERR_CONFIG
+
+
+def test_normalize_viscous_default_dt(lib):
+    c = af.Config(NX=32, NY=32, NZ=32, Viscosity=1e-3)
+    c.Normalize()
+    assert c.DT == 0.5 * c.H * c.H / (6 * 1e-3)
+
+
+def test_workers_capped_by_depth(lib):
+    c = af.Config(NX=8, NY=8, NZ=2, Workers=64)
+    c.Normalize()
+    assert c.Workers <= c.NZ
+
+
+def test_normalize_matches_oracle(lib, oracle_lib):
+    for kw in (dict(NX=64, NY=64, NZ=64, H=1 / 64, DT=0.004, Viscosity=1e-5),
+               dict(NX=5, NY=7, NZ=3, Viscosity=2e-4), dict(NX=9, NY=2, NZ=4, Ghost=2)):
+        c = af.Config(**kw, Workers=4)
+        c.Normalize()
+        o = oracle_lib.normalize(dict(kw, Workers=4))
+        for k in ("NX", "NY", "NZ", "Ghost", "H", "Viscosity", "Density", "DT", "Workers",
+                  "GaussSeidelIters"):
+            assert getattr(c, k) == o[k], k
+
+
+def test_mac_bytes_matches_oracle(lib, oracle_lib):
+    o = oracle_lib.load()
+    for dims in ((64, 32, 16, 1), (256, 256, 256, 1), (5, 7, 3, 2), (1, 1, 1, 1)):
+        assert af.MACBytes(*dims) == o.orc_mac_bytes(*dims)
+    assert af.MACBytes(0, 4, 4, 1) == 0
+
+
+def test_new_without_gpu_raises_cleanly(lib):
+    """On a host with no HIP device, New must fail loudly (no CPU fallback)."""
+    import ctypes
+    cnt = ctypes.c_int(0)
+    hip = ctypes.CDLL("libamdhip64.so")
+    if hip.hipGetDeviceCount(ctypes.byref(cnt)) == 0 and cnt.value > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(af.AeroFluxError) as ei:
+        af.New(af.Config(NX=8, NY=8, NZ=8))
+    assert ei.value.code == af.AF_ERR_HIP
+
+
+def test_sparse_solver_is_rejected(lib):
+    # sparse_fallback.go:17-19 -- checked before any device work
+    with pytest.raises(af.AeroFluxError, match="sparse direct solver"):
+        af.New(af.Config(NX=8, NY=8, NZ=8, UseSparseSolver=True))

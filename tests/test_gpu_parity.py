@@ -1,0 +1,324 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+FP64 path: bit-for-bit on every stage and on whole steps (the bar for the
+reference's unfused IEEE evaluation), signed zeros and ghost layers
+included.  KineticEnergy sums in a fixed tree instead of the reference's
+serial order: relative tolerance 1e-12.  Mixed precision: error bounds
+stated per field in the test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import aeroflux_amd as afm
+from aeroflux_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("P", "U", "V", "W", "Div", "Tmp0", "Tmp1", "Tmp2")
+
+
+def make_pair(af, oracle_lib, fields=None, **kw):
+    kw.setdefault("Workers", 4)
+    g = af.New(af.Config(**kw))
+    o = oracle_lib.OracleEngine(dict(kw))
+    if fields is not None:
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            g.Mesh.SetData(g.Mesh.field(fid), fields[name])
+            o.field(fid)[...] = fields[name]
+    return g, o
+
+
+def assert_bitwise(g, o, ids=range(8), what=""):
+    for fid in ids:
+        a = g.Mesh.Data(g.Mesh.field(fid))
+        b = o.field(fid)
+        assert a.shape == b.shape
+        if a.tobytes() != b.tobytes():
+            bad = np.argwhere(a.view(np.uint64) != b.view(np.uint64))
+            z, y, x = bad[0]
+            raise AssertionError(
+                f"{what} field {NAMES[fid]}: {len(bad)} mismatches, first at (x={x},y={y},z={z}) "
+                f"gpu={a[z, y, x]!r} oracle={b[z, y, x]!r}")
+
+
+def random_fields(shape_cfg, seed, scale=1.0, ghosts=True):
+    nx, ny, nz, g = shape_cfg
+    rng = np.random.default_rng(seed)
+    out = {}
+    for name, shp in ic.shapes(nx, ny, nz, g).items():
+        a = rng.standard_normal(shp) * scale
+        a[rng.random(shp) < 0.02] = 0.0
+        a[rng.random(shp) < 0.02] = -0.0
+        if not ghosts:
+            a2 = np.zeros(shp)
+            xr, yr, zr = ic.face_ranges(nx, ny, nz, g)[name]
+            a2[zr.start:zr.stop, yr.start:yr.stop, xr.start:xr.stop] = \
+                a[zr.start:zr.stop, yr.start:yr.stop, xr.start:xr.stop]
+            a = a2
+        out[name] = a
+    return out
+
+
+SHAPES = [(7, 5, 6, 1), (5, 4, 3, 2), (16, 9, 11, 1), (66, 5, 4, 1)]
+
+
+@pytest.mark.parametrize("shp", SHAPES)
+def test_velocity_bc_bitwise(af, oracle_lib, shp):
+    nx, ny, nz, g = shp
+    f = random_fields(shp, 1)
+    ge, oe = make_pair(af, oracle_lib, f, NX=nx, NY=ny, NZ=nz, Ghost=g)
+    ge.apply_velocity_bc()
+    oe.apply_velocity_bc()
+    assert_bitwise(ge, oe, (1, 2, 3), "velocity bc")
+    ge.Close()
+
+
+@pytest.mark.parametrize("shp", SHAPES)
+@pytest.mark.parametrize("ax", [0, 1, 2])
+@pytest.mark.parametrize("nu", [0.0, 3e-3])
+def test_sweep_bitwise(af, oracle_lib, shp, ax, nu):
+    nx, ny, nz, g = shp
+    h = 1.0 / max(nx, ny, nz)
+    # velocities of several cells per half step: exercises the clamps
+    f = random_fields(shp, 2 + ax, scale=2.0)
+    dt = 0.5 * h
+    ge, oe = make_pair(af, oracle_lib, f, NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, DT=dt,
+                       Viscosity=nu if nu == 0 else 0.1 * h * h / dt)
+    ge.sweep_axis(ax, dt * 0.5)
+    assert oe.sweep_axis(ax, dt * 0.5) == 0
+    assert_bitwise(ge, oe, (5, 6, 7), f"sweep ax={ax}")
+    ge.Close()
+
+
+@pytest.mark.parametrize("shp", SHAPES)
+def test_projection_stages_bitwise(af, oracle_lib, shp):
+    nx, ny, nz, g = shp
+    f = random_fields(shp, 5)
+    kw = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=0.37, DT=0.021, Density=1.3, GaussSeidelIters=7)
+    ge, oe = make_pair(af, oracle_lib, f, **kw)
+    rng = np.random.default_rng(11)
+    p = rng.standard_normal(oe.field(0).shape)
+    ge.Mesh.SetData(ge.Mesh.P, p)
+    oe.field(0)[...] = p
+    ge.compute_divergence(); oe.compute_divergence()
+    assert_bitwise(ge, oe, (4,), "divergence")
+    for col in (0, 1, 0):  # non-fused colour passes read the ghosts as stored
+        ge.gs_sweep(col); oe.gs_sweep(col)
+        assert_bitwise(ge, oe, (0,), f"gs colour {col}")
+    ge.apply_pressure_bc(); oe.apply_pressure_bc()
+    assert_bitwise(ge, oe, (0,), "pressure bc")
+    ge.subtract_gradient(); oe.subtract_gradient()
+    assert_bitwise(ge, oe, (1, 2, 3), "gradient")
+    # production solve: zeroing and per-pass BCs fused, final fill materialised
+    ge.Mesh.SetData(ge.Mesh.P, p)
+    oe.field(0)[...] = p
+    ge.solve_pressure(); oe.solve_pressure()
+    assert_bitwise(ge, oe, (0,), "solve")
+    ge.project(); oe.project()
+    assert_bitwise(ge, oe, range(8), "project")
+    ge.Close()
+
+
+STEP_CASES = [
+    # name, config, initial condition, steps
+    ("vortex12_inviscid", dict(NX=12, NY=12, NZ=12, H=1 / 12, DT=0.005, Viscosity=0.0,
+                               GaussSeidelIters=30), "vortex", 5),
+    ("tg_16x12x10", dict(NX=16, NY=12, NZ=10, H=1 / 16, DT=0.004, Viscosity=1e-4,
+                         GaussSeidelIters=24), "tg", 6),
+    ("ghost2", dict(NX=9, NY=7, NZ=8, Ghost=2, H=1 / 9, DT=0.01, Viscosity=1e-4,
+                    GaussSeidelIters=10), "tg", 4),
+    ("fast_flow_clamps", dict(NX=10, NY=10, NZ=10, H=0.1, DT=0.05, Viscosity=1e-4,
+                              GaussSeidelIters=12), "tg_strong", 4),
+]
+
+
+def initial(kind, cfg):
+    nx, ny, nz, g, h = cfg["NX"], cfg["NY"], cfg["NZ"], cfg.get("Ghost", 1), cfg["H"]
+    if kind == "vortex":
+        return ic.seed_vortex(nx, ny, nz, g, 1.0)
+    if kind == "tg_strong":
+        return ic.taylor_green(nx, ny, nz, g, h, seed=9, amp=8.0, noise=0.1)
+    return ic.taylor_green(nx, ny, nz, g, h, seed=4)
+
+
+@pytest.mark.parametrize("case", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_step_bitwise(af, oracle_lib, case):
+    name, cfg, kind, steps = case
+    ge, oe = make_pair(af, oracle_lib, initial(kind, cfg), **cfg)
+    ge.apply_velocity_bc(); oe.apply_velocity_bc()
+    for s in range(steps):
+        ge.Step()
+        assert oe.step() == 0
+        assert_bitwise(ge, oe, range(8), f"{name} step {s}")
+    assert ge.MaxDivergence() == oe.max_divergence()
+    assert ge.MaxVelocity() == oe.max_velocity()
+    assert ge.CFL() == oe.cfl()
+    ke_g, ke_o = ge.KineticEnergy(), oe.kinetic_energy()
+    assert abs(ke_g - ke_o) <= 1e-12 * abs(ke_o)
+    ge.Close()
+
+
+def test_config0_64cubed_100_steps_bitwise(af, oracle_lib):
+    """The reference CPU case: 64^3, H=1/64, DT=0.004, nu=1e-5, seeded
+    Taylor-Green + 1e-3 noise, default 24 sweeps, 100 Step() calls."""
+    cfg = dict(NX=64, NY=64, NZ=64, H=1 / 64, DT=0.004, Viscosity=1e-5, Workers=16)
+    f = ic.taylor_green(64, 64, 64, 1, 1 / 64, seed=1)
+    ge, oe = make_pair(af, oracle_lib, f, **cfg)
+    ge.apply_velocity_bc(); oe.apply_velocity_bc()
+    for s in range(100):
+        ge.Step()
+        assert oe.step() == 0
+        if s % 25 == 24:
+            assert_bitwise(ge, oe, range(8), f"64^3 step {s}")
+    assert ge.MaxDivergence() == oe.max_divergence()
+    ge.Close()
+
+
+def test_step_n_equals_repeated_step_and_replay_is_repeatable(af):
+    cfg = dict(NX=24, NY=20, NZ=16, H=1 / 24, DT=0.004, Viscosity=1e-5)
+    f = ic.taylor_green(24, 20, 16, 1, 1 / 24, seed=5)
+    outs = []
+    for mode in ("step", "stepn", "step"):
+        e = af.New(af.Config(**cfg))
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            e.Mesh.SetData(e.Mesh.field(fid), f[name])
+        if mode == "step":
+            for _ in range(7):
+                e.Step()
+        else:
+            e.StepN(7)
+        outs.append([e.Mesh.Data(e.Mesh.field(i)).tobytes() for i in range(8)])
+        e.Close()
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_descriptor_swap_state_is_honoured(af, oracle_lib):
+    """After an odd number of commitVelocity swaps the step graph must run on
+    the swapped descriptors (a second graph is captured)."""
+    cfg = dict(NX=8, NY=6, NZ=7, H=1 / 8, DT=0.01, Viscosity=1e-4, GaussSeidelIters=6)
+    f = ic.taylor_green(8, 6, 7, 1, 1 / 8, seed=6)
+    ge, oe = make_pair(af, oracle_lib, f, **cfg)
+    ge.sweep_axis(1, 0.005); assert oe.sweep_axis(1, 0.005) == 0
+    ge.commit_velocity(); oe.commit_velocity()
+    assert ge.Mesh.U.Off == oe.desc(1)["Off"]
+    for s in range(3):
+        ge.Step(); assert oe.step() == 0
+    assert_bitwise(ge, oe, range(8), "swapped")
+    ge.Close()
+
+
+def test_nan_departure_is_a_contained_fault(af):
+    """The reference's guard-page/bounds-panic path: a NaN departure
+    coordinate makes Step return an error; the engine survives and can be
+    reseeded and stepped again."""
+    cfg = dict(NX=8, NY=8, NZ=8, H=1 / 8, DT=0.01)
+    e = af.New(af.Config(**cfg))
+    f = ic.taylor_green(8, 8, 8, 1, 1 / 8, seed=2)
+    f["U"][4, 4, 4] = math.nan
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), f[name])
+    with pytest.raises(afm.AeroFluxError, match="^aeroflux: device fault contained") as ei:
+        e.Step()
+    assert ei.value.code == afm.AF_ERR_FAULT
+    clean = ic.taylor_green(8, 8, 8, 1, 1 / 8, seed=2)
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), clean[name])
+    e.Step()
+    assert math.isfinite(e.MaxDivergence())
+    e.Close()
+    e.Close()  # idempotent
+
+
+def test_close_then_use_errors(af):
+    e = af.New(af.Config(NX=4, NY=4, NZ=4))
+    e.Close()
+    with pytest.raises(afm.AeroFluxError):
+        e.Step()
+
+
+def test_sparse_solver_errors(af):
+    e = af.New(af.Config(NX=4, NY=4, NZ=4))
+    with pytest.raises(afm.AeroFluxError, match="sparse direct solver requires darwin"):
+        e.EnableSparseSolver()
+    e.Close()
+
+
+def test_reference_example_on_gpu(af):
+    """example_test.go:13-61 on the GPU path."""
+    e = af.New(af.Config(NX=16, NY=16, NZ=16, H=1 / 16, DT=0.004, Viscosity=1e-5, Density=1.0,
+                         GaussSeidelIters=400))
+    f = ic.shear_layer(16, 16, 16, 1)
+    e.Mesh.SetData(e.Mesh.U, f["U"])
+    for _ in range(10):
+        e.Step()
+    assert not math.isnan(e.KineticEnergy())
+    assert e.MaxDivergence() < 1e-3
+    e.Close()
+
+
+# ------------------------------------------------------------ mixed precision
+
+def run_engine(af, cfg, f, steps):
+    e = af.New(af.Config(**cfg))
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), f[name])
+    e.apply_velocity_bc()
+    e.StepN(steps)
+    out = {n: e.Mesh.Data(e.Mesh.field(i)) for i, n in enumerate(NAMES) if i < 4}
+    d = e.MaxDivergence()
+    stats = e.mixed_stats() if cfg.get("Precision") == afm.PRECISION_MIXED else None
+    e.Close()
+    return out, d, stats
+
+
+@pytest.mark.parametrize("target_mode", ["one_cycle", "fp64_target"])
+def test_mixed_precision_matches_fp64(af, target_mode):
+    """FP32 smoothing + FP64 defect correction vs the FP64 path (itself
+    bit-exact to the oracle).  Tolerances: velocities 1e-6 of max|u|,
+    pressure 1e-5 of max|p|; final max|div| within 1% of the FP64 path's,
+    and not above it when the FP64 value is the target."""
+    n = 32
+    base = dict(NX=n, NY=n, NZ=n, H=1 / n, DT=0.004, Viscosity=1e-5)
+    f = ic.taylor_green(n, n, n, 1, 1 / n, seed=8)
+    ref, d64, _ = run_engine(af, base, f, 5)
+    mcfg = dict(base, Precision=afm.PRECISION_MIXED)
+    if target_mode == "fp64_target":
+        mcfg["MixedTarget"] = d64
+    mix, dmx, stats = run_engine(af, mcfg, f, 5)
+    vmax = max(np.abs(ref[k]).max() for k in "UVW")
+    for k in "UVW":
+        assert np.abs(mix[k] - ref[k]).max() <= 1e-6 * vmax, k
+    assert np.abs(mix["P"] - ref["P"]).max() <= 1e-5 * np.abs(ref["P"]).max()
+    assert abs(dmx - d64) <= 0.01 * d64
+    if target_mode == "fp64_target":
+        assert dmx <= d64 * (1 + 1e-9)
+        assert stats[1] >= 1
+
+
+# ------------------------------------------------------ full size (256^3)
+
+def test_config1_256cubed_first_step_bitwise_and_properties(af, oracle_lib):
+    """256^3, H=1/256, DT=0.001, nu=1e-5, Taylor-Green + noise, 24 sweeps:
+    the first step bit for bit against the oracle, then size-independent
+    properties over further steps: finite, exact zero normal velocity on
+    walls, divergence reduced, identical replays."""
+    n = 256
+    cfg = dict(NX=n, NY=n, NZ=n, H=1 / n, DT=0.001, Viscosity=1e-5, Workers=16)
+    f = ic.taylor_green(n, n, n, 1, 1 / n, seed=1)
+    ge, oe = make_pair(af, oracle_lib, f, **cfg)
+    ge.apply_velocity_bc(); oe.apply_velocity_bc()
+    d0 = ge.MaxDivergence()
+    ge.Step()
+    assert oe.step() == 0
+    assert_bitwise(ge, oe, range(8), "256^3 step 0")
+    oe.close()
+    ge.StepN(9)
+    u = ge.Mesh.Data(ge.Mesh.U)
+    assert np.all(u[1:n + 1, 1:n + 1, 1] == 0) and np.all(u[1:n + 1, 1:n + 1, n + 1] == 0)
+    for fid in range(4):
+        assert np.all(np.isfinite(ge.Mesh.Data(ge.Mesh.field(fid))))
+    d10 = ge.MaxDivergence()
+    assert d10 < d0
+    ge.Close()
