@@ -66,6 +66,8 @@ class _CConfig(C.Structure):
         ("UseSparseSolver", C.c_int32), ("Lock", C.c_int32),
         ("Precision", C.c_int32), ("MixedMaxCycles", C.c_int32), ("MixedSweeps", C.c_int32),
         ("MixedFixSweeps", C.c_int32), ("MixedTarget", C.c_double),
+        ("TileK", C.c_int32), ("TileRows", C.c_int32), ("TileZ", C.c_int32),
+        ("Reserved1", C.c_int32),
     ]
 
 
@@ -107,6 +109,8 @@ _SIGNATURES = [
     ("af_project", C.c_int, [C.c_void_p]),
     ("af_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("af_mixed_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+    ("af_solver_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]),
     ("af_last_error", C.c_char_p, []),
 ]
 
@@ -157,6 +161,10 @@ class Config:
     MixedSweeps: int = 0
     MixedFixSweeps: int = 0
     MixedTarget: float = 0.0
+    TileK: int = 0
+    TileRows: int = 0
+    TileZ: int = 0
+    Reserved1: int = 0
 
     def _to_c(self) -> _CConfig:
         c = _CConfig()
@@ -358,6 +366,11 @@ class Engine:
         out = C.c_double()
         _check(load_library().af_time_kernel(self._handle(), kernel, reps, C.byref(out)))
         return float(out.value)
+
+    def solver_tile(self) -> tuple[int, int, int]:
+        k, r, z = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(load_library().af_solver_tile(self._handle(), C.byref(k), C.byref(r), C.byref(z)))
+        return int(k.value), int(r.value), int(z.value)
 
     def mixed_stats(self) -> tuple[float, int]:
         d = C.c_double()

@@ -20,6 +20,11 @@
 #include "../../include/aeroflux.h"
 #include "af_internal.h"
 #include "af_mixed.h"
+#include "af_rbgs_tb.h"
+
+#include <map>
+#include <mutex>
+#include <tuple>
 
 using namespace af;
 
@@ -111,6 +116,9 @@ struct af_engine {
     double *partials = nullptr;
     double *red_out = nullptr;
     double *red_host = nullptr; // pinned
+    double *pbuf = nullptr;     // FP64 ping-pong partner of P for the blocked solve
+    Tile tile64;                // FP64 red-black tiling (k <= 0: one pass per launch)
+    Tile tile32;                // FP32 tiling of the mixed-precision smoother
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
 };
@@ -173,9 +181,10 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
 
 hipError_t enq_divergence(af_engine *e, const Desc &d)
 {
+    float *r32 = e->cfg.Precision == AF_PRECISION_MIXED ? e->mixed.r : nullptr;
     return launch_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
                              dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_DIV]), e->G,
-                             e->div_scale, e->stream);
+                             e->div_scale, e->stream, r32, e->h2);
 }
 
 hipError_t enq_gradient(af_engine *e, const Desc &d)
@@ -188,10 +197,50 @@ hipError_t enq_gradient(af_engine *e, const Desc &d)
 // solvePressureRedBlack (project.go:73-100), FP64, production form: the
 // zeroing is folded into the first red pass and every intermediate
 // applyPressureBC into the passes themselves; the final fill is materialised.
+TbArgs<double> tb_args(const af_engine *e, const DField &P, const DField &D)
+{
+    TbArgs<double> a;
+    a.src = P.d;
+    a.dst = P.d;
+    a.rhs = D.d;
+    a.NX = e->G.NX; a.NY = e->G.NY; a.NZ = e->G.NZ; a.g = e->G.g;
+    a.sy = P.sy; a.sz = P.sz;
+    a.h2 = e->h2;
+    a.color0 = AF_COLOR_RED;
+    a.cz = e->tile64.cz;
+    return a;
+}
+
+// Temporally blocked form: 2*GaussSeidelIters colour passes in launches of
+// tile64.k passes, ping-ponging between P and pbuf so that the last launch
+// writes P; the first launch starts from P == 0 without reading it (the
+// solve's zeroing).  Bit-identical to the single-pass form.
+hipError_t enq_solve_fp64_blocked(af_engine *e, const DField &P, const DField &D)
+{
+    const int passes = (int)(2 * e->cfg.GaussSeidelIters);
+    const int K = e->tile64.k;
+    const int launches = (passes + K - 1) / K;
+    TbArgs<double> a = tb_args(e, P, D);
+    // Launch i (0-based) writes P when (launches - 1 - i) is even.
+    double *bufs[2] = {P.d, e->pbuf};
+    int done = 0;
+    for (int i = 0; i < launches; i++) {
+        const int k = (passes - done) < K ? (passes - done) : K;
+        double *dst = bufs[(launches - 1 - i) & 1];
+        a.dst = dst;
+        hipError_t err = launch_rbgs_tb<double>(a, k, e->tile64.rows, i == 0, e->stream);
+        if (err != hipSuccess) return err;
+        a.src = dst;
+        done += k;
+    }
+    return launch_pressure_bc(P, e->G, e->G.g > 1, e->stream);
+}
+
 hipError_t enq_solve_fp64(af_engine *e, const Desc &d)
 {
     const DField P = dfield(e, d.f[AF_FIELD_P]);
     const DField D = dfield(e, d.f[AF_FIELD_DIV]);
+    if (e->tile64.k > 0) return enq_solve_fp64_blocked(e, P, D);
     hipError_t err = launch_rbgs_first(P, D, e->G, e->h2, e->stream);
     if (err != hipSuccess) return err;
     err = launch_rbgs(P, D, e->G, e->h2, AF_COLOR_BLACK, true, e->stream);
@@ -216,6 +265,7 @@ hipError_t enq_solve(af_engine *e, const Desc &d)
         c.cycles = e->cfg.MixedMaxCycles;
         c.target = e->cfg.MixedTarget;
         c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
+        c.tile = e->tile32;
         return enqueue_mixed_solve(e->mixed, c, e->stream);
     }
     return enq_solve_fp64(e, d);
@@ -308,6 +358,8 @@ void release(af_engine *e)
         }
     release_mixed(e->mixed);
     if (e->arena) (void)hipFree(e->arena);
+    if (e->pbuf) (void)hipFree(e->pbuf);
+    e->pbuf = nullptr;
     if (e->flag) (void)hipFree(e->flag);
     if (e->partials) (void)hipFree(e->partials);
     if (e->red_out) (void)hipFree(e->red_out);
@@ -320,6 +372,116 @@ void release(af_engine *e)
     e->flag_host = nullptr;
     e->stream = nullptr;
     e->closed = true;
+}
+
+// ---- tile selection -------------------------------------------------------
+
+// Tuned tiles are cached per (grid, ghost, precision) for the process, so
+// engines of the same shape (batches, served jobs) tune once.
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, Tile> g_tune_cache;
+
+bool tile_fits(const Tile &t, const Geom &G)
+{
+    (void)G;
+    return rbgs_tb_supported(t.k, t.rows) && 16 * t.rows - 2 * t.k > 0 && t.cz >= 1;
+}
+
+// Times `reps` launches of every candidate on the engine's own buffers
+// (src is never written) and keeps the fastest per colour pass.
+template <typename T>
+int tune_tile(af_engine *e, const T *src, T *dst, const T *rhs, int sy, int sz, Tile *out)
+{
+    const Geom &G = e->G;
+    const int ks[] = {2, 4, 6};
+    const int rowss[] = {1, 2};
+    const int czs[] = {8, 16, 32, 64};
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), "create event");
+    HIP_TRY(hipEventCreate(&b), "create event");
+    float best = 1e30f;
+    Tile bt;
+    bt.k = 4; bt.rows = 2; bt.cz = 16;
+    for (int k : ks)
+        for (int rows : rowss)
+            for (int cz : czs) {
+                Tile t;
+                t.k = k; t.rows = rows; t.cz = cz;
+                if (!tile_fits(t, G)) continue;
+                if (cz > 2 * G.NZ && cz != 8) continue;
+                TbArgs<T> ta;
+                ta.src = src; ta.dst = dst; ta.rhs = rhs;
+                ta.NX = G.NX; ta.NY = G.NY; ta.NZ = G.NZ; ta.g = G.g;
+                ta.sy = sy; ta.sz = sz; ta.h2 = e->h2; ta.color0 = 0; ta.cz = cz;
+                hipError_t err = launch_rbgs_tb<T>(ta, k, rows, false, e->stream);
+                const int reps = 3;
+                if (err == hipSuccess) err = hipEventRecord(a, e->stream);
+                for (int r = 0; r < reps && err == hipSuccess; r++)
+                    err = launch_rbgs_tb<T>(ta, k, rows, false, e->stream);
+                if (err == hipSuccess) err = hipEventRecord(b, e->stream);
+                if (err == hipSuccess) err = hipEventSynchronize(b);
+                float ms = 0;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+                if (err != hipSuccess) {
+                    (void)hipEventDestroy(a);
+                    (void)hipEventDestroy(b);
+                    return hip_fail(err, "tune red-black tile");
+                }
+                const float per_pass = ms / (reps * k);
+                if (per_pass < best) {
+                    best = per_pass;
+                    bt = t;
+                }
+            }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *out = bt;
+    return AF_OK;
+}
+
+int choose_tiles(af_engine *e)
+{
+    const af_config &c = e->cfg;
+    const Geom &G = e->G;
+    const DField P = dfield(e, e->cur.f[AF_FIELD_P]);
+    const DField D = dfield(e, e->cur.f[AF_FIELD_DIV]);
+    for (int prec = 0; prec < 2; prec++) {
+        if (prec == 1 && c.Precision != AF_PRECISION_MIXED) break;
+        Tile &t = prec == 0 ? e->tile64 : e->tile32;
+        if (c.TileK < 0) {
+            t.k = -1;
+            continue;
+        }
+        if (c.TileK > 0) {
+            t.k = c.TileK;
+            t.rows = c.TileRows > 0 ? c.TileRows : 2;
+            t.cz = c.TileZ > 0 ? c.TileZ : 16;
+            if (!tile_fits(t, G))
+                return fail(AF_ERR_CONFIG,
+                            "aeroflux: unsupported red-black tile k=%d rows=%d z=%d (k in {2,4,6}, "
+                            "rows in {1,2}, 16*rows > 2*k)",
+                            t.k, t.rows, t.cz);
+            continue;
+        }
+        const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g, prec);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_tune_cache.find(key);
+            if (it != g_tune_cache.end()) {
+                t = it->second;
+                continue;
+            }
+        }
+        int rc;
+        if (prec == 0)
+            rc = tune_tile<double>(e, P.d, e->pbuf, D.d, P.sy, P.sz, &t);
+        else
+            rc = tune_tile<float>(e, e->mixed.e[0], e->mixed.e[1], e->mixed.r, P.sy, P.sz, &t);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = t;
+    }
+    return AF_OK;
 }
 
 }  // namespace
@@ -470,6 +632,29 @@ int af_new(const af_config *cfg_in, af_engine **out)
             hipSuccess)
             return bail(err, "allocate mixed-precision buffers", AF_ERR_OOM);
     }
+    if (cfg.TileK >= 0) {
+        if ((err = hipMalloc(&e->pbuf, (size_t)e->cur.f[AF_FIELD_P].Count * 8)) != hipSuccess)
+            return bail(err, "allocate pressure ping-pong buffer", AF_ERR_OOM);
+        if ((err = hipMemsetAsync(e->pbuf, 0, (size_t)e->cur.f[AF_FIELD_P].Count * 8,
+                                  e->stream)) != hipSuccess)
+            return bail(err, "zero pressure ping-pong buffer", AF_ERR_HIP);
+        if ((err = rbgs_tb_init()) != hipSuccess)
+            return bail(err, "configure red-black kernels", AF_ERR_HIP);
+    }
+    if ((err = hipStreamSynchronize(e->stream)) != hipSuccess)
+        return bail(err, "initialise engine", AF_ERR_HIP);
+    if (int rc = choose_tiles(e)) {
+        std::string msg = g_err;
+        release(e);
+        delete e;
+        g_err = msg;
+        return rc;
+    }
+    // Tuning launches wrote only pbuf / the FP32 buffers; restore their zeros.
+    if (e->pbuf &&
+        (err = hipMemsetAsync(e->pbuf, 0, (size_t)e->cur.f[AF_FIELD_P].Count * 8, e->stream)) !=
+            hipSuccess)
+        return bail(err, "zero pressure ping-pong buffer", AF_ERR_HIP);
     if ((err = hipStreamSynchronize(e->stream)) != hipSuccess)
         return bail(err, "initialise engine", AF_ERR_HIP);
     *out = e;
@@ -732,6 +917,15 @@ int af_project(af_engine *e)
     return sync_stream(e, "project");
 }
 
+int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchunk)
+{
+    if (!e) return fail(AF_ERR_ARG, "aeroflux: null engine");
+    if (k) *k = e->tile64.k;
+    if (rows) *rows = e->tile64.rows;
+    if (zchunk) *zchunk = e->tile64.cz;
+    return AF_OK;
+}
+
 int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles)
 {
     STAGE_BEGIN(e);
@@ -766,10 +960,17 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
         case 1: return enq_sweep(e, d, AF_AXIS_X, e->cfg.DT * 0.5);
         case 2: return enq_divergence(e, d);
         case 3: return enq_gradient(e, d);
+        case 4: {
+            TbArgs<double> a = tb_args(e, P, D);
+            a.dst = e->pbuf;
+            return launch_rbgs_tb<double>(a, e->tile64.k, e->tile64.rows, false, e->stream);
+        }
         default: return hipErrorInvalidValue;
         }
     };
-    if (kernel < 0 || kernel > 3) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
+    if (kernel < 0 || kernel > 4) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
+    if (kernel == 4 && e->tile64.k <= 0)
+        return fail(AF_ERR_ARG, "aeroflux: the engine does not use the blocked red-black kernel");
     HIP_TRY(one(), "warm-up launch");
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a), "create event");

@@ -15,6 +15,7 @@
 // target.  Cycles after the target is met run as empty launches (a device
 // flag), which keeps the whole solve capturable in one graph.
 #include "af_mixed.h"
+#include "af_rbgs_tb.h"
 
 #pragma clang fp contract(off)
 
@@ -32,29 +33,20 @@ __device__ __forceinline__ int idx(const DField &f, int x, int y, int z)
 
 inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
 
-// First red pass of a cycle from e == 0.  Cycle 0 also forms the FP32
-// right-hand side r = -(H^2 div) from the FP64 divergence; later cycles
-// read the residual written by the previous cycle.
-template <bool FROM_DIV>
-__global__ __launch_bounds__(BX *BY) void k_e_first(DField P, const double *__restrict__ div,
-                                                    float *__restrict__ e, float *__restrict__ r,
-                                                    Geom G, double h2, const int *done)
+// First red pass of a cycle from e == 0: e = r/6 on red cells, 0 on black.
+__global__ __launch_bounds__(BX *BY) void k_e_first(DField P, float *__restrict__ e,
+                                                    const float *__restrict__ r, Geom G,
+                                                    const int *done, int check_done)
 {
-    if (!FROM_DIV && *done) return;
+    if (check_done && *done) return;
     const int g = G.g;
     const int x = g + blockIdx.x * BX + threadIdx.x;
     const int y = g + blockIdx.y * BY + threadIdx.y;
     const int z = g + blockIdx.z;
     if (x >= g + G.NX || y >= g + G.NY) return;
     const int i = idx(P, x, y, z);
-    float rhs;
-    if (FROM_DIV) {
-        rhs = (float)(-(h2 * div[i]));
-        r[i] = rhs;
-    } else {
-        rhs = r[i];
-    }
-    e[i] = (((x + y + z) & 1) == 0) ? rhs * (1.0f / 6.0f) : 0.0f;
+    // (0 + r) keeps the sign of zero that a pass over all-zero neighbours gives.
+    e[i] = (((x + y + z) & 1) == 0) ? (0.0f + r[i]) * (1.0f / 6.0f) : 0.0f;
 }
 
 // FP32 red-black pass on the correction, Neumann mirror fused.
@@ -181,6 +173,49 @@ __global__ void k_mixed_reset(int *done, int *cycles)
 
 __global__ void k_mixed_count(int *cycles) { *cycles = *cycles + 1; }
 
+// `passes` FP32 red-black passes (red first) on A e = r from e = 0.
+// Returns (through *out) the buffer index holding the result.
+hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipStream_t s,
+                  int *out)
+{
+    const Geom &G = c.G;
+    const int *done = gated ? m.done : nullptr;
+    if (c.tile.k > 0) {
+        TbArgs<float> a;
+        a.rhs = m.r;
+        a.NX = G.NX; a.NY = G.NY; a.NZ = G.NZ; a.g = G.g;
+        a.sy = c.P.sy; a.sz = c.P.sz;
+        a.h2 = 0;
+        a.color0 = 0;
+        a.cz = c.tile.cz;
+        a.done = done;
+        int cur = 0;
+        int done_passes = 0;
+        bool first = true;
+        while (done_passes < passes) {
+            const int k = (passes - done_passes) < c.tile.k ? (passes - done_passes) : c.tile.k;
+            a.src = m.e[cur];
+            a.dst = m.e[first ? cur : 1 - cur];
+            hipError_t err = launch_rbgs_tb<float>(a, k, c.tile.rows, first, s);
+            if (err != hipSuccess) return err;
+            if (!first) cur = 1 - cur;
+            first = false;
+            done_passes += k;
+        }
+        *out = cur;
+        return hipSuccess;
+    }
+    const dim3 blk(BX, BY);
+    const dim3 gfull = grid3(G.NX, G.NY, G.NZ);
+    const dim3 ghalf = grid3((G.NX + 1) / 2, G.NY, G.NZ);
+    hipLaunchKernelGGL(k_e_first, gfull, blk, 0, s, c.P, m.e[0], m.r, G, m.done, gated ? 1 : 0);
+    for (int p = 1; p < passes; p++)
+        hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e[0], m.r, G, p & 1, m.done,
+                           gated ? 1 : 0);
+    *out = 0;
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t init_mixed(MixedState &m, Geom G, DField P, hipStream_t s)
@@ -188,13 +223,15 @@ hipError_t init_mixed(MixedState &m, Geom G, DField P, hipStream_t s)
     const size_t n = (size_t)P.nx * P.ny * P.nz;
     m.np = ((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY);
     hipError_t err;
-    if ((err = hipMalloc(&m.e, n * sizeof(float))) != hipSuccess) return err;
+    for (int b = 0; b < 2; b++) {
+        if ((err = hipMalloc(&m.e[b], n * sizeof(float))) != hipSuccess) return err;
+        if ((err = hipMemsetAsync(m.e[b], 0, n * sizeof(float), s)) != hipSuccess) return err;
+    }
     if ((err = hipMalloc(&m.r, n * sizeof(float))) != hipSuccess) return err;
     if ((err = hipMalloc(&m.partials, (size_t)m.np * sizeof(double))) != hipSuccess) return err;
     if ((err = hipMalloc(&m.done, 2 * sizeof(int))) != hipSuccess) return err;
     if ((err = hipMalloc(&m.last_res, sizeof(double))) != hipSuccess) return err;
     m.cycles = m.done + 1;
-    if ((err = hipMemsetAsync(m.e, 0, n * sizeof(float), s)) != hipSuccess) return err;
     if ((err = hipMemsetAsync(m.r, 0, n * sizeof(float), s)) != hipSuccess) return err;
     if ((err = hipMemsetAsync(m.done, 0, 2 * sizeof(int), s)) != hipSuccess) return err;
     return hipMemsetAsync(m.last_res, 0, sizeof(double), s);
@@ -202,7 +239,8 @@ hipError_t init_mixed(MixedState &m, Geom G, DField P, hipStream_t s)
 
 void release_mixed(MixedState &m)
 {
-    if (m.e) (void)hipFree(m.e);
+    for (auto &b : m.e)
+        if (b) (void)hipFree(b);
     if (m.r) (void)hipFree(m.r);
     if (m.partials) (void)hipFree(m.partials);
     if (m.done) (void)hipFree(m.done);
@@ -215,19 +253,15 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
     const Geom &G = c.G;
     const dim3 blk(BX, BY);
     const dim3 gfull = grid3(G.NX, G.NY, G.NZ);
-    const dim3 ghalf = grid3((G.NX + 1) / 2, G.NY, G.NZ);
     const dim3 gcol((G.NX + BX - 1) / BX, (G.NY + BY - 1) / BY);
     const bool check = c.target > 0;
+    hipError_t err;
+    int buf = 0;
 
     hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles);
-    // Cycle 0: p = 0, r = b.
-    hipLaunchKernelGGL(k_e_first<true>, gfull, blk, 0, s, c.P, c.D.d, m.e, m.r, G, c.h2, m.done);
-    hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e, m.r, G, 1, m.done, 0);
-    for (int it = 1; it < c.sweeps; it++) {
-        hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e, m.r, G, 0, m.done, 0);
-        hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e, m.r, G, 1, m.done, 0);
-    }
-    hipLaunchKernelGGL(k_correct<true>, gfull, blk, 0, s, c.P, m.e, G, m.done, 0);
+    // Cycle 0: p = 0, r = b (written by the divergence kernel).
+    if ((err = smooth(m, c, 2 * c.sweeps, false, s, &buf)) != hipSuccess) return err;
+    hipLaunchKernelGGL(k_correct<true>, gfull, blk, 0, s, c.P, m.e[buf], G, m.done, 0);
     if (!check) {
         hipLaunchKernelGGL(k_mixed_count, dim3(1), dim3(1), 0, s, m.cycles);
     } else {
@@ -236,22 +270,15 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
         hipLaunchKernelGGL(k_res_final, dim3(1), dim3(RB), 0, s, m.partials, m.np, c.res_to_div,
                            c.target, m.done, m.last_res, m.cycles, 0);
         for (int cyc = 1; cyc < c.cycles; cyc++) {
-            hipLaunchKernelGGL(k_e_first<false>, gfull, blk, 0, s, c.P, c.D.d, m.e, m.r, G, c.h2,
-                               m.done);
-            hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e, m.r, G, 1, m.done, 1);
-            for (int it = 1; it < c.fix_sweeps; it++) {
-                hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e, m.r, G, 0, m.done, 1);
-                hipLaunchKernelGGL(k_e_pass, ghalf, blk, 0, s, c.P, m.e, m.r, G, 1, m.done, 1);
-            }
-            hipLaunchKernelGGL(k_correct<false>, gfull, blk, 0, s, c.P, m.e, G, m.done, 1);
+            if ((err = smooth(m, c, 2 * c.fix_sweeps, true, s, &buf)) != hipSuccess) return err;
+            hipLaunchKernelGGL(k_correct<false>, gfull, blk, 0, s, c.P, m.e[buf], G, m.done, 1);
             hipLaunchKernelGGL(k_residual, gcol, blk, 0, s, c.P, c.D.d, m.r, G, c.h2, m.partials,
                                m.done, 1);
             hipLaunchKernelGGL(k_res_final, dim3(1), dim3(RB), 0, s, m.partials, m.np,
                                c.res_to_div, c.target, m.done, m.last_res, m.cycles, 1);
         }
     }
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
+    if ((err = hipGetLastError()) != hipSuccess) return err;
     // Final Neumann fill of the FP64 pressure (project.go:98).
     return launch_pressure_bc(c.P, G, G.g > 1, s);
 }

@@ -88,6 +88,11 @@ typedef struct af_config {
     double MixedTarget;       /* mixed: stop once the post-projection
                                  max|div| implied by the FP64 residual is
                                  <= this (0 -> run exactly one cycle) */
+    int32_t TileK;            /* red-black colour passes per launch: 2, 4 or
+                                 6; -1 -> one pass per launch; 0 -> tuned */
+    int32_t TileRows;         /* rows per thread, 1 or 2 (0 -> tuned) */
+    int32_t TileZ;            /* z-planes per workgroup (0 -> tuned) */
+    int32_t Reserved1;
 } af_config;
 
 /* Field, grid.go:12-21 (descriptor after any commitVelocity swaps). */
@@ -182,8 +187,9 @@ int af_project(af_engine *e);                           /* project.go:12-34 */
 
 /* Average device time (ms, HIP events on the engine's stream) of `reps`
  * back-to-back launches of one kernel of the step, launched exactly as the
- * step launches it.  kernel: 0 = red-black pressure pass, 1 = advection
- * sweep (X axis), 2 = divergence, 3 = gradient subtract.  No reference
+ * step launches it.  kernel: 0 = one red-black colour pass, 1 = advection
+ * sweep (X axis), 2 = divergence, 3 = gradient subtract, 4 = one launch
+ * of the temporally blocked red-black kernel (TileK passes).  No reference
  * counterpart (measurement only); it overwrites P and the Tmp fields. */
 int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
 
@@ -191,6 +197,11 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
  * (0 if MixedTarget == 0) and the cycles the last solve ran.  No reference
  * counterpart. */
 int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles);
+
+/* Tile shape the engine's red-black solve runs with (after tuning):
+ * passes per launch (-1: one pass per launch), rows per thread, z-chunk.
+ * No reference counterpart. */
+int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchunk);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *af_last_error(void);

@@ -276,25 +276,37 @@ def run_engine(af, cfg, f, steps):
 @pytest.mark.parametrize("target_mode", ["one_cycle", "fp64_target"])
 def test_mixed_precision_matches_fp64(af, target_mode):
     """FP32 smoothing + FP64 defect correction vs the FP64 path (itself
-    bit-exact to the oracle).  Tolerances: velocities 1e-6 of max|u|,
-    pressure 1e-5 of max|p|; final max|div| within 1% of the FP64 path's,
-    and not above it when the FP64 value is the target."""
+    bit-exact to the oracle).
+
+    one_cycle (same 24 sweep pairs, FP32): differences are FP32 rounding
+    only -- velocities within 1e-6 of max|u|, pressure within 1e-5 of
+    max|p|, final max|div| within 1% of the FP64 path's.
+
+    fp64_target (cycles until max|div| <= the FP64 path's final value):
+    extra correction sweeps carry the iterate closer to the converged
+    solution than the FP64 path's fixed 24 sweeps, so the difference is the
+    FP64 path's own unconverged remainder -- velocities within 1e-4 of
+    max|u|, pressure within 1e-3 of max|p| -- and the final max|div| must
+    not exceed the FP64 path's."""
     n = 32
     base = dict(NX=n, NY=n, NZ=n, H=1 / n, DT=0.004, Viscosity=1e-5)
     f = ic.taylor_green(n, n, n, 1, 1 / n, seed=8)
     ref, d64, _ = run_engine(af, base, f, 5)
     mcfg = dict(base, Precision=afm.PRECISION_MIXED)
+    tol_u, tol_p = 1e-6, 1e-5
     if target_mode == "fp64_target":
         mcfg["MixedTarget"] = d64
+        tol_u, tol_p = 1e-4, 1e-3
     mix, dmx, stats = run_engine(af, mcfg, f, 5)
     vmax = max(np.abs(ref[k]).max() for k in "UVW")
     for k in "UVW":
-        assert np.abs(mix[k] - ref[k]).max() <= 1e-6 * vmax, k
-    assert np.abs(mix["P"] - ref["P"]).max() <= 1e-5 * np.abs(ref["P"]).max()
-    assert abs(dmx - d64) <= 0.01 * d64
+        assert np.abs(mix[k] - ref[k]).max() <= tol_u * vmax, k
+    assert np.abs(mix["P"] - ref["P"]).max() <= tol_p * np.abs(ref["P"]).max()
     if target_mode == "fp64_target":
         assert dmx <= d64 * (1 + 1e-9)
         assert stats[1] >= 1
+    else:
+        assert abs(dmx - d64) <= 0.01 * d64
 
 
 # ------------------------------------------------------ full size (256^3)
@@ -322,3 +334,70 @@ def test_config1_256cubed_first_step_bitwise_and_properties(af, oracle_lib):
     d10 = ge.MaxDivergence()
     assert d10 < d0
     ge.Close()
+
+
+# ---------------------------------------------- temporally blocked red-black
+
+TILES = [(-1, 0, 0), (2, 1, 1), (2, 2, 3), (4, 1, 8), (4, 2, 16), (6, 1, 5), (6, 2, 32)]
+TILE_SHAPES = [(37, 29, 23, 1), (130, 9, 7, 1), (10, 70, 9, 2)]
+
+
+@pytest.mark.parametrize("tile", TILES, ids=[f"k{t[0]}r{t[1]}z{t[2]}" for t in TILES])
+@pytest.mark.parametrize("shp", TILE_SHAPES, ids=["37x29x23", "130x9x7", "10x70x9g2"])
+def test_blocked_solve_bitwise_across_tiles(af, oracle_lib, tile, shp):
+    """Every tile shape (passes per launch, rows per thread, z-chunk) gives
+    the reference's solve bit for bit -- the GPU form of the reference's
+    worker-count determinism test (engine_test.go:134-170)."""
+    nx, ny, nz, g = shp
+    k, rows, cz = tile
+    kw = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=0.41, DT=0.017, GaussSeidelIters=5,
+              TileK=k, TileRows=rows, TileZ=cz)
+    ge = af.New(af.Config(**kw))
+    assert ge.solver_tile()[0] == k
+    oe = oracle_lib.OracleEngine({k2: v for k2, v in kw.items() if not k2.startswith("Tile")})
+    rng = np.random.default_rng(13)
+    d = rng.standard_normal(oe.field(4).shape)
+    p = rng.standard_normal(oe.field(0).shape)
+    for fid, arr in ((4, d), (0, p)):
+        ge.Mesh.SetData(ge.Mesh.field(fid), arr)
+        oe.field(fid)[...] = arr
+    ge.solve_pressure(); oe.solve_pressure()
+    assert_bitwise(ge, oe, (0, 4), f"blocked solve {tile}")
+    ge.Close()
+
+
+@pytest.mark.parametrize("tile", TILES[1:], ids=[f"k{t[0]}r{t[1]}z{t[2]}" for t in TILES[1:]])
+def test_step_bitwise_across_tiles(af, oracle_lib, tile):
+    k, rows, cz = tile
+    cfg = dict(NX=40, NY=36, NZ=20, H=1 / 40, DT=0.004, Viscosity=1e-4, GaussSeidelIters=9)
+    f = ic.taylor_green(40, 36, 20, 1, 1 / 40, seed=12)
+    ge, oe = make_pair(af, oracle_lib, f, **cfg)
+    ge.Close()
+    ge = af.New(af.Config(**cfg, TileK=k, TileRows=rows, TileZ=cz))
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        ge.Mesh.SetData(ge.Mesh.field(fid), f[name])
+    for s in range(3):
+        ge.Step(); assert oe.step() == 0
+    assert_bitwise(ge, oe, range(8), f"step tile {tile}")
+    ge.Close()
+
+
+def test_autotuned_tile_is_reported(af):
+    e = af.New(af.Config(NX=48, NY=40, NZ=32))
+    k, rows, cz = e.solver_tile()
+    assert k in (2, 4, 6) and rows in (1, 2) and cz >= 1
+    e.Close()
+
+
+def test_mixed_blocked_smoother_matches_unblocked(af):
+    """The FP32 smoother gives the same bits blocked or one pass per launch."""
+    n = 24
+    f = ic.taylor_green(n, n, n, 1, 1 / n, seed=3)
+    base = dict(NX=n, NY=n, NZ=n, H=1 / n, DT=0.004, Viscosity=1e-5,
+                Precision=afm.PRECISION_MIXED)
+    outs = []
+    for tile in ((-1, 0, 0), (4, 2, 8), (6, 1, 16)):
+        cfg = dict(base, TileK=tile[0], TileRows=tile[1], TileZ=tile[2])
+        out, d, _ = run_engine(af, cfg, f, 3)
+        outs.append([out[k].tobytes() for k in "PUVW"])
+    assert outs[0] == outs[1] == outs[2]
