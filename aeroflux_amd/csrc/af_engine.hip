@@ -20,6 +20,7 @@
 #include "../../include/aeroflux.h"
 #include "af_internal.h"
 #include "af_mixed.h"
+#include "af_rbgs_split.h"
 #include "af_rbgs_tb.h"
 
 #include <map>
@@ -98,6 +99,15 @@ void commit(Desc &d)  // commitVelocity, stencil.go:230-235
     std::swap(d.f[AF_FIELD_W], d.f[AF_FIELD_TMP2]);
 }
 
+// How the FP64 red-black solve is scheduled.  k == 0: one pass per launch on
+// the colour-split layout (the default); k < 0: one pass per launch on the
+// arena layout; k > 0: k passes per launch, temporally blocked, arena layout.
+struct Tile {
+    int k = 0;
+    int rows = 2;   // rows per thread (blocked)
+    int cz = 16;    // z-planes per workgroup (blocked)
+};
+
 }  // namespace
 
 struct af_engine {
@@ -117,8 +127,9 @@ struct af_engine {
     double *red_out = nullptr;
     double *red_host = nullptr; // pinned
     double *pbuf = nullptr;     // FP64 ping-pong partner of P for the blocked solve
-    Tile tile64;                // FP64 red-black tiling (k <= 0: one pass per launch)
-    Tile tile32;                // FP32 tiling of the mixed-precision smoother
+    // Colour-split FP64 solve buffers: P and Div, red and black, interior only.
+    double *ps_red = nullptr, *ps_black = nullptr, *ds_red = nullptr, *ds_black = nullptr;
+    Tile tile64;                // red-black schedule (see Tile)
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
 };
@@ -181,10 +192,9 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
 
 hipError_t enq_divergence(af_engine *e, const Desc &d)
 {
-    float *r32 = e->cfg.Precision == AF_PRECISION_MIXED ? e->mixed.r : nullptr;
     return launch_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
                              dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_DIV]), e->G,
-                             e->div_scale, e->stream, r32, e->h2);
+                             e->div_scale, e->stream);
 }
 
 hipError_t enq_gradient(af_engine *e, const Desc &d)
@@ -236,11 +246,43 @@ hipError_t enq_solve_fp64_blocked(af_engine *e, const DField &P, const DField &D
     return launch_pressure_bc(P, e->G, e->G.g > 1, e->stream);
 }
 
+SplitArgs<double> split_args(const af_engine *e)
+{
+    SplitArgs<double> a;
+    a.red = e->ps_red;
+    a.black = e->ps_black;
+    a.rhs_red = e->ds_red;
+    a.rhs_black = e->ds_black;
+    a.NX = e->G.NX; a.NY = e->G.NY; a.NZ = e->G.NZ; a.NXH = (e->G.NX + 1) / 2; a.g = e->G.g;
+    a.h2 = e->h2;
+    return a;
+}
+
+// Colour-split form: Div is split once, the 2*GaussSeidelIters passes run on
+// the compacted colour arrays (the first from P == 0, which is the solve's
+// zeroing), and the result is merged back into P's interior before the final
+// ghost fill.  Bit-identical to the reference's sequence.
+hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D)
+{
+    hipError_t err = launch_to_split<double, double>(D.d, D.sy, D.sz, e->G, e->ds_red,
+                                                     e->ds_black, e->stream);
+    const SplitArgs<double> a = split_args(e);
+    const int passes = (int)(2 * e->cfg.GaussSeidelIters);
+    for (int p = 0; p < passes && err == hipSuccess; p++)
+        err = launch_split_pass<double>(a, p & 1, p == 0 ? 1 : (p == 1 ? 2 : 0), e->stream);
+    if (err == hipSuccess)
+        err = launch_from_split<double>(P.d, P.sy, P.sz, e->G, e->ps_red, e->ps_black, false,
+                                        nullptr, e->stream);
+    if (err == hipSuccess) err = launch_pressure_bc(P, e->G, e->G.g > 1, e->stream);
+    return err;
+}
+
 hipError_t enq_solve_fp64(af_engine *e, const Desc &d)
 {
     const DField P = dfield(e, d.f[AF_FIELD_P]);
     const DField D = dfield(e, d.f[AF_FIELD_DIV]);
     if (e->tile64.k > 0) return enq_solve_fp64_blocked(e, P, D);
+    if (e->tile64.k == 0) return enq_solve_fp64_split(e, P, D);
     hipError_t err = launch_rbgs_first(P, D, e->G, e->h2, e->stream);
     if (err != hipSuccess) return err;
     err = launch_rbgs(P, D, e->G, e->h2, AF_COLOR_BLACK, true, e->stream);
@@ -265,7 +307,6 @@ hipError_t enq_solve(af_engine *e, const Desc &d)
         c.cycles = e->cfg.MixedMaxCycles;
         c.target = e->cfg.MixedTarget;
         c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
-        c.tile = e->tile32;
         return enqueue_mixed_solve(e->mixed, c, e->stream);
     }
     return enq_solve_fp64(e, d);
@@ -360,6 +401,11 @@ void release(af_engine *e)
     if (e->arena) (void)hipFree(e->arena);
     if (e->pbuf) (void)hipFree(e->pbuf);
     e->pbuf = nullptr;
+    double **split[4] = {&e->ps_red, &e->ps_black, &e->ds_red, &e->ds_black};
+    for (double **b : split) {
+        if (*b) (void)hipFree(*b);
+        *b = nullptr;
+    }
     if (e->flag) (void)hipFree(e->flag);
     if (e->partials) (void)hipFree(e->partials);
     if (e->red_out) (void)hipFree(e->red_out);
@@ -374,26 +420,25 @@ void release(af_engine *e)
     e->closed = true;
 }
 
-// ---- tile selection -------------------------------------------------------
+// ---- schedule selection ---------------------------------------------------
 
-// Tuned tiles are cached per (grid, ghost, precision) for the process, so
-// engines of the same shape (batches, served jobs) tune once.
+// Tuned blocked tiles are cached per (grid, ghost, passes per launch) for the
+// process, so engines of the same shape (batches, served jobs) tune once.
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int, int>, Tile> g_tune_cache;
 
-bool tile_fits(const Tile &t, const Geom &G)
+bool tile_fits(const Tile &t)
 {
-    (void)G;
     return rbgs_tb_supported(t.k, t.rows) && 16 * t.rows - 2 * t.k > 0 && t.cz >= 1;
 }
 
-// Times `reps` launches of every candidate on the engine's own buffers
-// (src is never written) and keeps the fastest per colour pass.
-template <typename T>
-int tune_tile(af_engine *e, const T *src, T *dst, const T *rhs, int sy, int sz, Tile *out)
+// Times a few launches of every (rows, z-chunk) candidate for k passes per
+// launch on the engine's own buffers (src is never written) and keeps the
+// fastest.
+int tune_blocked(af_engine *e, int k, Tile *out)
 {
-    const Geom &G = e->G;
-    const int ks[] = {2, 4, 6};
+    const DField P = dfield(e, e->cur.f[AF_FIELD_P]);
+    const DField D = dfield(e, e->cur.f[AF_FIELD_DIV]);
     const int rowss[] = {1, 2};
     const int czs[] = {8, 16, 32, 64};
     hipEvent_t a, b;
@@ -401,86 +446,70 @@ int tune_tile(af_engine *e, const T *src, T *dst, const T *rhs, int sy, int sz, 
     HIP_TRY(hipEventCreate(&b), "create event");
     float best = 1e30f;
     Tile bt;
-    bt.k = 4; bt.rows = 2; bt.cz = 16;
-    for (int k : ks)
-        for (int rows : rowss)
-            for (int cz : czs) {
-                Tile t;
-                t.k = k; t.rows = rows; t.cz = cz;
-                if (!tile_fits(t, G)) continue;
-                if (cz > 2 * G.NZ && cz != 8) continue;
-                TbArgs<T> ta;
-                ta.src = src; ta.dst = dst; ta.rhs = rhs;
-                ta.NX = G.NX; ta.NY = G.NY; ta.NZ = G.NZ; ta.g = G.g;
-                ta.sy = sy; ta.sz = sz; ta.h2 = e->h2; ta.color0 = 0; ta.cz = cz;
-                hipError_t err = launch_rbgs_tb<T>(ta, k, rows, false, e->stream);
-                const int reps = 3;
-                if (err == hipSuccess) err = hipEventRecord(a, e->stream);
-                for (int r = 0; r < reps && err == hipSuccess; r++)
-                    err = launch_rbgs_tb<T>(ta, k, rows, false, e->stream);
-                if (err == hipSuccess) err = hipEventRecord(b, e->stream);
-                if (err == hipSuccess) err = hipEventSynchronize(b);
-                float ms = 0;
-                if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
-                if (err != hipSuccess) {
-                    (void)hipEventDestroy(a);
-                    (void)hipEventDestroy(b);
-                    return hip_fail(err, "tune red-black tile");
-                }
-                const float per_pass = ms / (reps * k);
-                if (per_pass < best) {
-                    best = per_pass;
-                    bt = t;
-                }
+    bt.k = k;
+    for (int rows : rowss)
+        for (int cz : czs) {
+            Tile t;
+            t.k = k; t.rows = rows; t.cz = cz;
+            if (!tile_fits(t)) continue;
+            TbArgs<double> ta = tb_args(e, P, D);
+            ta.dst = e->pbuf;
+            ta.cz = cz;
+            hipError_t err = launch_rbgs_tb<double>(ta, k, rows, false, e->stream);
+            const int reps = 3;
+            if (err == hipSuccess) err = hipEventRecord(a, e->stream);
+            for (int r = 0; r < reps && err == hipSuccess; r++)
+                err = launch_rbgs_tb<double>(ta, k, rows, false, e->stream);
+            if (err == hipSuccess) err = hipEventRecord(b, e->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(b);
+            float ms = 0;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+            if (err != hipSuccess) {
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+                return hip_fail(err, "tune red-black tile");
             }
+            if (ms < best) {
+                best = ms;
+                bt = t;
+            }
+        }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     *out = bt;
     return AF_OK;
 }
 
-int choose_tiles(af_engine *e)
+int choose_schedule(af_engine *e)
 {
     const af_config &c = e->cfg;
     const Geom &G = e->G;
-    const DField P = dfield(e, e->cur.f[AF_FIELD_P]);
-    const DField D = dfield(e, e->cur.f[AF_FIELD_DIV]);
-    for (int prec = 0; prec < 2; prec++) {
-        if (prec == 1 && c.Precision != AF_PRECISION_MIXED) break;
-        Tile &t = prec == 0 ? e->tile64 : e->tile32;
-        if (c.TileK < 0) {
-            t.k = -1;
-            continue;
-        }
-        if (c.TileK > 0) {
-            t.k = c.TileK;
-            t.rows = c.TileRows > 0 ? c.TileRows : 2;
-            t.cz = c.TileZ > 0 ? c.TileZ : 16;
-            if (!tile_fits(t, G))
-                return fail(AF_ERR_CONFIG,
-                            "aeroflux: unsupported red-black tile k=%d rows=%d z=%d (k in {2,4,6}, "
-                            "rows in {1,2}, 16*rows > 2*k)",
-                            t.k, t.rows, t.cz);
-            continue;
-        }
-        const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g, prec);
-        {
-            std::lock_guard<std::mutex> lk(g_tune_mu);
-            auto it = g_tune_cache.find(key);
-            if (it != g_tune_cache.end()) {
-                t = it->second;
-                continue;
-            }
-        }
-        int rc;
-        if (prec == 0)
-            rc = tune_tile<double>(e, P.d, e->pbuf, D.d, P.sy, P.sz, &t);
-        else
-            rc = tune_tile<float>(e, e->mixed.e[0], e->mixed.e[1], e->mixed.r, P.sy, P.sz, &t);
-        if (rc) return rc;
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[key] = t;
+    Tile &t = e->tile64;
+    if (c.TileK <= 0) {
+        t.k = c.TileK < 0 ? -1 : 0;
+        return AF_OK;
     }
+    t.k = c.TileK;
+    t.rows = c.TileRows > 0 ? c.TileRows : 2;
+    t.cz = c.TileZ > 0 ? c.TileZ : 16;
+    if (!tile_fits(t))
+        return fail(AF_ERR_CONFIG,
+                    "aeroflux: unsupported red-black tile k=%d rows=%d z=%d (k in {2,4,6}, rows "
+                    "in {1,2}, 16*rows > 2*k)",
+                    t.k, t.rows, t.cz);
+    if (c.TileRows > 0 && c.TileZ > 0) return AF_OK;
+    const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g, t.k);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            t = it->second;
+            return AF_OK;
+        }
+    }
+    if (int rc = tune_blocked(e, t.k, &t)) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = t;
     return AF_OK;
 }
 
@@ -628,11 +657,18 @@ int af_new(const af_config *cfg_in, af_engine **out)
     if ((err = hipHostMalloc(&e->red_host, sizeof(double) * 4, hipHostMallocDefault)) != hipSuccess)
         return bail(err, "allocate pinned result", AF_ERR_OOM);
     if (cfg.Precision == AF_PRECISION_MIXED) {
-        if ((err = init_mixed(e->mixed, e->G, dfield(e, e->cur.f[AF_FIELD_P]), e->stream)) !=
-            hipSuccess)
+        if ((err = init_mixed(e->mixed, e->G, e->stream)) != hipSuccess)
             return bail(err, "allocate mixed-precision buffers", AF_ERR_OOM);
-    }
-    if (cfg.TileK >= 0) {
+    } else if (cfg.TileK == 0) {
+        const size_t n = (size_t)split_count(e->G) * sizeof(double);
+        double **split[4] = {&e->ps_red, &e->ps_black, &e->ds_red, &e->ds_black};
+        for (double **b : split) {
+            if ((err = hipMalloc(b, n)) != hipSuccess)
+                return bail(err, "allocate colour-split buffers", AF_ERR_OOM);
+            if ((err = hipMemsetAsync(*b, 0, n, e->stream)) != hipSuccess)
+                return bail(err, "zero colour-split buffers", AF_ERR_HIP);
+        }
+    } else if (cfg.TileK > 0) {
         if ((err = hipMalloc(&e->pbuf, (size_t)e->cur.f[AF_FIELD_P].Count * 8)) != hipSuccess)
             return bail(err, "allocate pressure ping-pong buffer", AF_ERR_OOM);
         if ((err = hipMemsetAsync(e->pbuf, 0, (size_t)e->cur.f[AF_FIELD_P].Count * 8,
@@ -643,7 +679,7 @@ int af_new(const af_config *cfg_in, af_engine **out)
     }
     if ((err = hipStreamSynchronize(e->stream)) != hipSuccess)
         return bail(err, "initialise engine", AF_ERR_HIP);
-    if (int rc = choose_tiles(e)) {
+    if (int rc = choose_schedule(e)) {
         std::string msg = g_err;
         release(e);
         delete e;
@@ -954,23 +990,24 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
     if (reps < 1 || !avg_ms) return fail(AF_ERR_ARG, "aeroflux: bad timing arguments");
     const Desc &d = e->cur;
     const DField P = dfield(e, d.f[AF_FIELD_P]), D = dfield(e, d.f[AF_FIELD_DIV]);
+    const bool split = e->tile64.k == 0 && e->cfg.Precision == AF_PRECISION_FP64;
     auto one = [&]() -> hipError_t {
         switch (kernel) {
-        case 0: return launch_rbgs(P, D, e->G, e->h2, AF_COLOR_RED, true, e->stream);
+        case 0:
+            if (split) return launch_split_pass<double>(split_args(e), AF_COLOR_RED, 0, e->stream);
+            if (e->tile64.k > 0) {
+                TbArgs<double> a = tb_args(e, P, D);
+                a.dst = e->pbuf;
+                return launch_rbgs_tb<double>(a, e->tile64.k, e->tile64.rows, false, e->stream);
+            }
+            return launch_rbgs(P, D, e->G, e->h2, AF_COLOR_RED, true, e->stream);
         case 1: return enq_sweep(e, d, AF_AXIS_X, e->cfg.DT * 0.5);
         case 2: return enq_divergence(e, d);
         case 3: return enq_gradient(e, d);
-        case 4: {
-            TbArgs<double> a = tb_args(e, P, D);
-            a.dst = e->pbuf;
-            return launch_rbgs_tb<double>(a, e->tile64.k, e->tile64.rows, false, e->stream);
-        }
         default: return hipErrorInvalidValue;
         }
     };
-    if (kernel < 0 || kernel > 4) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
-    if (kernel == 4 && e->tile64.k <= 0)
-        return fail(AF_ERR_ARG, "aeroflux: the engine does not use the blocked red-black kernel");
+    if (kernel < 0 || kernel > 3) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
     HIP_TRY(one(), "warm-up launch");
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a), "create event");

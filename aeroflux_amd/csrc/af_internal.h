@@ -31,9 +31,8 @@ struct SweepArgs {
 // Launchers.  All enqueue on `s` and return the hipGetLastError() code.
 hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t s);
 hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s);
-// r32 (optional): also write the FP32 right-hand side -(h2 * div).
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
-                             hipStream_t s, float *r32 = nullptr, double h2 = 0);
+                             hipStream_t s);
 // Red-black colour pass.  fused_bc: a neighbour outside the interior reads the
 // cell's own value (the Neumann mirror applied just before the pass).
 hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fused_bc,

@@ -245,10 +245,9 @@ __global__ __launch_bounds__(BX *BY) void k_sweep(SweepArgs a)
 
 // ------------------------------------------------------------- projection
 
-// computeDivergence, project.go:42-62.  In mixed precision the FP32
-// right-hand side -(h2 * div) of the correction smoother is written too.
+// computeDivergence, project.go:42-62.
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
-                                                       Geom G, double scale, float *r32, double h2)
+                                                       Geom G, double scale)
 {
     const int g = G.g;
     const int x = g + blockIdx.x * BX + threadIdx.x;
@@ -258,10 +257,7 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DFiel
     const double du = U.d[idx(U, x + 1, y, z)] - U.d[idx(U, x, y, z)];
     const double dv = V.d[idx(V, x, y + 1, z)] - V.d[idx(V, x, y, z)];
     const double dw = W.d[idx(W, x, y, z + 1)] - W.d[idx(W, x, y, z)];
-    const int i = idx(D, x, y, z);
-    const double div = scale * (du + dv + dw);
-    D.d[i] = div;
-    if (r32) r32[i] = (float)(-(h2 * div));
+    D.d[idx(D, x, y, z)] = scale * (du + dv + dw);
 }
 
 // gaussSeidelSweep, project.go:103-130: one lane per cell of the colour.
@@ -537,10 +533,10 @@ hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s)
 }
 
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
-                             hipStream_t s, float *r32, double h2)
+                             hipStream_t s)
 {
     hipLaunchKernelGGL(k_divergence, grid3(G.NX, G.NY, G.NZ), dim3(BX, BY), 0, s, U, V, W, D, G,
-                       scale, r32, h2);
+                       scale);
     return hipGetLastError();
 }
 

@@ -8,19 +8,13 @@
 
 namespace af {
 
-struct Tile {
-    int k = -1;     // colour passes per blocked launch; <= 0: one pass per launch
-    int rows = 2;   // rows per thread
-    int cz = 16;    // z-planes per workgroup
-};
-
 struct MixedState {
-    float *e[2] = {nullptr, nullptr};  // FP32 correction (ping-pong), P's padded layout
-    float *r = nullptr;                // FP32 right-hand side / residual
-    double *partials = nullptr;        // residual-max partials
-    int *done = nullptr;               // device: set once the target is met
-    double *last_res = nullptr;        // device: last max|div| implied by the residual
-    int *cycles = nullptr;             // device: cycles run since the last reset
+    float *e_red = nullptr, *e_black = nullptr;  // FP32 correction, colour-split
+    float *r_red = nullptr, *r_black = nullptr;  // FP32 right-hand side / residual, split
+    double *partials = nullptr;                  // residual-max partials
+    int *done = nullptr;                         // device: set once the target is met
+    double *last_res = nullptr;                  // device: last max|div| implied by the residual
+    int *cycles = nullptr;                       // device: cycles run since the last reset
     int np = 0;
 };
 
@@ -33,14 +27,11 @@ struct MixedCtx {
     int cycles;        // maximum cycles
     double target;     // max|div| target after the projection (0: one cycle)
     double res_to_div; // DT/(rho*H^2): max|div| = res_to_div * max|residual|
-    Tile tile;         // FP32 red-black tiling
 };
 
-hipError_t init_mixed(MixedState &m, Geom G, DField P, hipStream_t s);
+hipError_t init_mixed(MixedState &m, Geom G, hipStream_t s);
 void release_mixed(MixedState &m);
-// Enqueue one mixed-precision solve writing the FP64 P (ghosts filled).  The
-// FP32 right-hand side m.r must already hold -(H^2 div), which the
-// divergence kernel writes in mixed mode.
+// Enqueue one mixed-precision solve writing the FP64 P (ghosts filled).
 hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s);
 
 }  // namespace af

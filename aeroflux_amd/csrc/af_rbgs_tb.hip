@@ -58,8 +58,11 @@ template <typename T, int K, int R, bool ZERO>
 __global__ __launch_bounds__(LANES *WAVES) void k_rbgs_tb(TbArgs<T> a)
 {
     constexpr int TY = WAVES * R;
-    constexpr int W = K + 2;       // P window (planes t-K-1 .. t)
-    constexpr int WD = W;          // rhs window (planes t-K-1 .. t; same modulus)
+    // Planes are loaded PD steps ahead of their first use so that the loads
+    // of plane t+PD are in flight while step t computes.
+    constexpr int PD = 1;
+    constexpr int W = K + 2 + PD;  // P window (planes t-K-1 .. t+PD)
+    constexpr int WD = W;          // rhs window (same modulus)
     extern __shared__ unsigned char lds_raw[];
     T *lds = reinterpret_cast<T *>(lds_raw);  // [K][TY][LANES]
     if (a.done && *a.done) return;  // uniform across the workgroup
@@ -111,23 +114,40 @@ __global__ __launch_bounds__(LANES *WAVES) void k_rbgs_tb(TbArgs<T> a)
 
     const T h2 = (T)a.h2;
     const int tend = zl1 + K;
+    // Prologue: planes zl0 .. zl0+PD-1 into slots 0 .. PD-1.
+#pragma unroll
+    for (int p = 0; p < PD; p++) {
+        const int t = zl0 + p;
+        if (t < zl1) {
+            const int zo = t * sz;
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int s = 0; s < 2; s++)
+                    if (ld[r][s]) {
+                        pw[p][r][s] = ZERO ? T(0) : a.src[xe + s + off[r] + zo];
+                        dw[p][r][s] = a.rhs[xe + s + off[r] + zo];
+                    }
+        }
+    }
     for (int t0 = zl0; t0 < tend; t0 += W) {
         // Unrolled by W so every window slot is a compile-time constant.
 #pragma unroll
         for (int u = 0; u < W; u++) {
             const int t = t0 + u;
             if (t >= tend) break;  // uniform across the workgroup
-            // ---- load plane t ----
-            if (t < zl1) {
-                const int zo = t * sz;
+            // ---- prefetch plane t+PD (slot of plane t-K-2, no longer read) ----
+            if (t + PD < zl1) {
+                const int zo = (t + PD) * sz;
+                const int sl = (u + PD) % W;
 #pragma unroll
                 for (int r = 0; r < R; r++)
 #pragma unroll
                     for (int s = 0; s < 2; s++) {
                         const int i = xe + s + off[r] + zo;
                         if (ld[r][s]) {
-                            pw[u][r][s] = ZERO ? T(0) : a.src[i];
-                            dw[u][r][s] = a.rhs[i];
+                            pw[sl][r][s] = ZERO ? T(0) : a.src[i];
+                            dw[sl][r][s] = a.rhs[i];
                         }
                     }
             }
@@ -253,15 +273,12 @@ hipError_t launch_rbgs_tb(const TbArgs<T> &a, int k, int rows, bool zero_src, hi
 }
 
 template hipError_t launch_rbgs_tb<double>(const TbArgs<double> &, int, int, bool, hipStream_t);
-template hipError_t launch_rbgs_tb<float>(const TbArgs<float> &, int, int, bool, hipStream_t);
 
 hipError_t rbgs_tb_init()
 {
     hipError_t e;
     if ((e = allow_all<double, 1>()) != hipSuccess) return e;
-    if ((e = allow_all<double, 2>()) != hipSuccess) return e;
-    if ((e = allow_all<float, 1>()) != hipSuccess) return e;
-    return allow_all<float, 2>();
+    return allow_all<double, 2>();
 }
 
 bool rbgs_tb_supported(int k, int rows)

@@ -88,10 +88,13 @@ typedef struct af_config {
     double MixedTarget;       /* mixed: stop once the post-projection
                                  max|div| implied by the FP64 residual is
                                  <= this (0 -> run exactly one cycle) */
-    int32_t TileK;            /* red-black colour passes per launch: 2, 4 or
-                                 6; -1 -> one pass per launch; 0 -> tuned */
-    int32_t TileRows;         /* rows per thread, 1 or 2 (0 -> tuned) */
-    int32_t TileZ;            /* z-planes per workgroup (0 -> tuned) */
+    int32_t TileK;            /* FP64 red-black schedule: 0 -> one pass per
+                                 launch on the colour-split layout (default);
+                                 -1 -> one pass per launch on the arena
+                                 layout; 2, 4, 6 -> that many passes per
+                                 launch, temporally blocked, arena layout */
+    int32_t TileRows;         /* blocked: rows per thread, 1 or 2 (0 -> tuned) */
+    int32_t TileZ;            /* blocked: z-planes per workgroup (0 -> tuned) */
     int32_t Reserved1;
 } af_config;
 
@@ -187,9 +190,11 @@ int af_project(af_engine *e);                           /* project.go:12-34 */
 
 /* Average device time (ms, HIP events on the engine's stream) of `reps`
  * back-to-back launches of one kernel of the step, launched exactly as the
- * step launches it.  kernel: 0 = one red-black colour pass, 1 = advection
- * sweep (X axis), 2 = divergence, 3 = gradient subtract, 4 = one launch
- * of the temporally blocked red-black kernel (TileK passes).  No reference
+ * step launches it.  kernel: 0 = one launch of the red-black kernel the
+ * solve uses (one colour pass on the colour-split layout by default, one
+ * pass on the arena layout for TileK = -1, TileK passes for TileK > 0),
+ * 1 = advection sweep (X axis), 2 = divergence, 3 = gradient subtract.
+ * No reference
  * counterpart (measurement only); it overwrites P and the Tmp fields. */
 int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
 
