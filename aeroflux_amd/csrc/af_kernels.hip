@@ -6,6 +6,7 @@
 // unfused evaluation.  Thread mappings put x (the unit-stride axis) on the
 // 64 lanes of a wavefront so every row access is coalesced.
 #include "af_internal.h"
+#include "af_xcd.h"
 
 #pragma clang fp contract(off)
 
@@ -15,12 +16,18 @@ namespace {
 constexpr int BX = 64;  // lanes along x
 constexpr int BY = 4;   // rows along y per block
 
+// Block-count triple of a 3-D stencil launch; grids are launched 1-D in
+// XCD-aware order and decoded with xcd_block3.
+inline int3 nblocks(int nx, int ny, int nz)
+{
+    return make_int3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz);
+}
+inline dim3 grid1(int3 nb) { return dim3((unsigned)(nb.x * nb.y * nb.z)); }
+
 __device__ __forceinline__ int idx(const DField &f, int x, int y, int z)
 {
     return x + y * f.sy + z * f.sz;  // Field.Index, grid.go:24-26
 }
-
-__device__ __forceinline__ double qnan() { return __longlong_as_double(0x7ff8000000000000LL); }
 
 // ------------------------------------------------------------ velocity BC
 //
@@ -133,126 +140,20 @@ __global__ __launch_bounds__(256) void k_velocity_bc(DField U, DField V, DField 
 
 // -------------------------------------------------------------- advection
 
-// faceVelocity, stencil.go:128-155 (same neighbour order in each average).
-template <int C>
-__device__ __forceinline__ void face_velocity(const SweepArgs &a, int x, int y, int z, double &vx,
-                                              double &vy, double &vz)
-{
-    const double *u = a.U.d, *v = a.V.d, *w = a.W.d;
-    if (C == 0) {
-        vx = u[idx(a.U, x, y, z)];
-        vy = 0.25 * (v[idx(a.V, x - 1, y, z)] + v[idx(a.V, x, y, z)] +
-                     v[idx(a.V, x - 1, y + 1, z)] + v[idx(a.V, x, y + 1, z)]);
-        vz = 0.25 * (w[idx(a.W, x - 1, y, z)] + w[idx(a.W, x, y, z)] +
-                     w[idx(a.W, x - 1, y, z + 1)] + w[idx(a.W, x, y, z + 1)]);
-    } else if (C == 1) {
-        vx = 0.25 * (u[idx(a.U, x, y - 1, z)] + u[idx(a.U, x + 1, y - 1, z)] +
-                     u[idx(a.U, x, y, z)] + u[idx(a.U, x + 1, y, z)]);
-        vy = v[idx(a.V, x, y, z)];
-        vz = 0.25 * (w[idx(a.W, x, y - 1, z)] + w[idx(a.W, x, y, z)] +
-                     w[idx(a.W, x, y - 1, z + 1)] + w[idx(a.W, x, y, z + 1)]);
-    } else {
-        vx = 0.25 * (u[idx(a.U, x, y, z - 1)] + u[idx(a.U, x + 1, y, z - 1)] +
-                     u[idx(a.U, x, y, z)] + u[idx(a.U, x + 1, y, z)]);
-        vy = 0.25 * (v[idx(a.V, x, y, z - 1)] + v[idx(a.V, x, y + 1, z - 1)] +
-                     v[idx(a.V, x, y, z)] + v[idx(a.V, x, y + 1, z)]);
-        vz = w[idx(a.W, x, y, z)];
-    }
-}
-
-// sampleTrilinear, stencil.go:169-225.  All eight taps are evaluated even
-// when a weight is exactly 0 (0*Inf and signed zeros depend on it).  A NaN
-// coordinate -- the reference's bounds-panic trigger -- raises the fault
-// flag and yields NaN without touching memory out of range.
-__device__ __forceinline__ double sample_trilinear(const DField &f, double px, double py,
-                                                   double pz, int *flag)
-{
-    const double maxX = (double)(f.nx - 1);
-    const double maxY = (double)(f.ny - 1);
-    const double maxZ = (double)(f.nz - 1);
-    if (px < 0) px = 0; else if (px > maxX) px = maxX;
-    if (py < 0) py = 0; else if (py > maxY) py = maxY;
-    if (pz < 0) pz = 0; else if (pz > maxZ) pz = maxZ;
-    if (__builtin_isnan(px) || __builtin_isnan(py) || __builtin_isnan(pz)) {
-        *flag = 1;
-        return qnan();
-    }
-    int x0 = (int)px, y0 = (int)py, z0 = (int)pz;
-    if (x0 >= f.nx - 1) x0 = f.nx - 2;
-    if (y0 >= f.ny - 1) y0 = f.ny - 2;
-    if (z0 >= f.nz - 1) z0 = f.nz - 2;
-    const double tx = px - (double)x0;
-    const double ty = py - (double)y0;
-    const double tz = pz - (double)z0;
-
-    const double *d = f.d;
-    const int i000 = idx(f, x0, y0, z0);
-    const int i100 = i000 + 1;
-    const int i010 = i000 + f.sy;
-    const int i110 = i010 + 1;
-    const int i001 = i000 + f.sz;
-    const int i101 = i001 + 1;
-    const int i011 = i001 + f.sy;
-    const int i111 = i011 + 1;
-
-    const double c00 = d[i000] * (1 - tx) + d[i100] * tx;
-    const double c10 = d[i010] * (1 - tx) + d[i110] * tx;
-    const double c01 = d[i001] * (1 - tx) + d[i101] * tx;
-    const double c11 = d[i011] * (1 - tx) + d[i111] * tx;
-    const double c0 = c00 * (1 - ty) + c10 * ty;
-    const double c1 = c01 * (1 - ty) + c11 * ty;
-    return c0 * (1 - tz) + c1 * tz;
-}
-
-// One face of advectComponent (stencil.go:79-115).
-template <int AX, int C>
-__device__ __forceinline__ void advect_face(const SweepArgs &a, int x, int y, int z)
-{
-    const DField &src = (C == 0) ? a.U : (C == 1) ? a.V : a.W;
-    const DField &dst = (C == 0) ? a.T0 : (C == 1) ? a.T1 : a.T2;
-    double vx, vy, vz;
-    face_velocity<C>(a, x, y, z, vx, vy, vz);
-    double px = (double)x, py = (double)y, pz = (double)z;
-    if (AX == 0) px -= a.dt * vx * a.invH;
-    if (AX == 1) py -= a.dt * vy * a.invH;
-    if (AX == 2) pz -= a.dt * vz * a.invH;
-    const double adv = sample_trilinear(src, px, py, pz, a.flag);
-    double lap = 0;
-    if (a.lam != 0) {
-        const int i = idx(src, x, y, z);
-        const int st = (AX == 0) ? 1 : (AX == 1) ? src.sy : src.sz;
-        lap = a.lam * (src.d[i - st] - 2 * src.d[i] + src.d[i + st]);
-    }
-    dst.d[idx(dst, x, y, z)] = adv + lap;
-}
-
-// advectDiffuseAxis (stencil.go:21-25) for all three components in one
-// launch.  Threads cover x in [g,g+NX], y in [g,g+NY], z in [g,g+NZ]; each
-// computes every component face that exists at its coordinate.
-template <int AX>
-__global__ __launch_bounds__(BX *BY) void k_sweep(SweepArgs a)
-{
-    const int g = a.G.g;
-    const int x = g + blockIdx.x * BX + threadIdx.x;
-    const int y = g + blockIdx.y * BY + threadIdx.y;
-    const int z = g + blockIdx.z;
-    if (x > g + a.G.NX || y > g + a.G.NY) return;
-    const bool xin = x < g + a.G.NX, yin = y < g + a.G.NY, zin = z < g + a.G.NZ;
-    if (yin && zin) advect_face<AX, 0>(a, x, y, z);
-    if (xin && zin) advect_face<AX, 1>(a, x, y, z);
-    if (xin && yin) advect_face<AX, 2>(a, x, y, z);
-}
+// The advection sweeps live in af_advect.hip.
 
 // ------------------------------------------------------------- projection
 
 // computeDivergence, project.go:42-62.
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
-                                                       Geom G, double scale)
+                                                       Geom G, double scale, int3 nb)
 {
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    const int x = g + blockIdx.x * BX + threadIdx.x;
-    const int y = g + blockIdx.y * BY + threadIdx.y;
-    const int z = g + blockIdx.z;
+    const int x = g + bx * BX + threadIdx.x;
+    const int y = g + by * BY + threadIdx.y;
+    const int z = g + bz;
     if (x >= g + G.NX || y >= g + G.NY) return;
     const double du = U.d[idx(U, x + 1, y, z)] - U.d[idx(U, x, y, z)];
     const double dv = V.d[idx(V, x, y + 1, z)] - V.d[idx(V, x, y, z)];
@@ -266,14 +167,17 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DFiel
 // pass and sets that ghost to exactly this cell's current value, so the
 // substitution is bitwise identical and the ghost fill can be skipped.
 template <bool FUSED>
-__global__ __launch_bounds__(BX *BY) void k_rbgs(DField P, DField D, Geom G, double h2, int color)
+__global__ __launch_bounds__(BX *BY) void k_rbgs(DField P, DField D, Geom G, double h2, int color,
+                                                     int3 nb)
 {
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    const int y = g + blockIdx.y * BY + threadIdx.y;
-    const int z = g + blockIdx.z;
+    const int y = g + by * BY + threadIdx.y;
+    const int z = g + bz;
     if (y >= g + G.NY) return;
     const int start = g + ((((g + y + z) & 1) != color) ? 1 : 0);
-    const int x = start + 2 * (int)(blockIdx.x * BX + threadIdx.x);
+    const int x = start + 2 * (int)(bx * BX + threadIdx.x);
     if (x >= g + G.NX) return;
     double *p = P.d;
     const int i = idx(P, x, y, z);
@@ -300,12 +204,14 @@ __global__ __launch_bounds__(BX *BY) void k_rbgs(DField P, DField D, Geom G, dou
 // every neighbour, ghosts included, is +0, so a red cell becomes
 // ((((0+0)+0)+0)+0)+0 - h2*div, over 6, and a black cell stays +0.  Writing
 // both colours here also performs the zeroing of the interior.
-__global__ __launch_bounds__(BX *BY) void k_rbgs_first(DField P, DField D, Geom G, double h2)
+__global__ __launch_bounds__(BX *BY) void k_rbgs_first(DField P, DField D, Geom G, double h2, int3 nb)
 {
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    const int x = g + blockIdx.x * BX + threadIdx.x;
-    const int y = g + blockIdx.y * BY + threadIdx.y;
-    const int z = g + blockIdx.z;
+    const int x = g + bx * BX + threadIdx.x;
+    const int y = g + by * BY + threadIdx.y;
+    const int z = g + bz;
     if (x >= g + G.NX || y >= g + G.NY) return;
     const int i = idx(P, x, y, z);
     double v = 0;
@@ -377,28 +283,29 @@ __global__ __launch_bounds__(256) void k_zero(double *d, int64_t n)
 
 // subtractGradient, project.go:172-205, one thread per union-box coordinate.
 __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField W, DField P,
-                                                     Geom G, double scale)
+                                                     Geom G, double scale, int3 nb)
 {
+    // A thread owns interior column (x, y) for one z; the last column, row
+    // and z position also update the wall faces U(g+NX), V(g+NY), W(g+NZ).
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    const int x = g + blockIdx.x * BX + threadIdx.x;
-    const int y = g + blockIdx.y * BY + threadIdx.y;
-    const int z = g + blockIdx.z;
-    if (x > g + G.NX || y > g + G.NY) return;
-    const bool xin = x < g + G.NX, yin = y < g + G.NY, zin = z < g + G.NZ;
+    const int x = g + bx * BX + threadIdx.x;
+    const int y = g + by * BY + threadIdx.y;
+    const int z = g + bz;
+    if (x >= g + G.NX || y >= g + G.NY) return;
     const double *p = P.d;
     const int ip = idx(P, x, y, z);
-    if (yin && zin) {
-        const int i = idx(U, x, y, z);
-        U.d[i] -= scale * (p[ip] - p[ip - 1]);
+    if (z < g + G.NZ) {
+        const int iu = idx(U, x, y, z);
+        U.d[iu] -= scale * (p[ip] - p[ip - 1]);
+        if (x == g + G.NX - 1) U.d[iu + 1] -= scale * (p[ip + 1] - p[ip]);
+        const int iv = idx(V, x, y, z);
+        V.d[iv] -= scale * (p[ip] - p[ip - P.sy]);
+        if (y == g + G.NY - 1) V.d[iv + V.sy] -= scale * (p[ip + P.sy] - p[ip]);
     }
-    if (xin && zin) {
-        const int i = idx(V, x, y, z);
-        V.d[i] -= scale * (p[ip] - p[ip - P.sy]);
-    }
-    if (xin && yin) {
-        const int i = idx(W, x, y, z);
-        W.d[i] -= scale * (p[ip] - p[ip - P.sz]);
-    }
+    const int iw = idx(W, x, y, z);
+    W.d[iw] -= scale * (p[ip] - p[ip - P.sz]);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -503,8 +410,6 @@ __global__ __launch_bounds__(RB) void k_reduce_final(const double *partials, int
     if (threadIdx.x == 0) *out = r;
 }
 
-inline dim3 grid3(int nx, int ny, int nz) { return dim3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz); }
-
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -520,40 +425,29 @@ hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s)
-{
-    const dim3 grid = grid3(a.G.NX + 1, a.G.NY + 1, a.G.NZ + 1);
-    const dim3 block(BX, BY);
-    switch (axis) {
-    case 0: hipLaunchKernelGGL(k_sweep<0>, grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL(k_sweep<1>, grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL(k_sweep<2>, grid, block, 0, s, a); break;
-    }
-    return hipGetLastError();
-}
-
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_divergence, grid3(G.NX, G.NY, G.NZ), dim3(BX, BY), 0, s, U, V, W, D, G,
-                       scale);
+    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    hipLaunchKernelGGL(k_divergence, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale, nb);
     return hipGetLastError();
 }
 
 hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fused_bc,
                        hipStream_t s)
 {
-    const dim3 grid = grid3((G.NX + 1) / 2, G.NY, G.NZ);
+    const int3 nb = nblocks((G.NX + 1) / 2, G.NY, G.NZ);
     if (fused_bc)
-        hipLaunchKernelGGL(k_rbgs<true>, grid, dim3(BX, BY), 0, s, P, D, G, h2, color);
+        hipLaunchKernelGGL(k_rbgs<true>, grid1(nb), dim3(BX, BY), 0, s, P, D, G, h2, color, nb);
     else
-        hipLaunchKernelGGL(k_rbgs<false>, grid, dim3(BX, BY), 0, s, P, D, G, h2, color);
+        hipLaunchKernelGGL(k_rbgs<false>, grid1(nb), dim3(BX, BY), 0, s, P, D, G, h2, color, nb);
     return hipGetLastError();
 }
 
 hipError_t launch_rbgs_first(DField P, DField D, Geom G, double h2, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rbgs_first, grid3(G.NX, G.NY, G.NZ), dim3(BX, BY), 0, s, P, D, G, h2);
+    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    hipLaunchKernelGGL(k_rbgs_first, grid1(nb), dim3(BX, BY), 0, s, P, D, G, h2, nb);
     return hipGetLastError();
 }
 
@@ -577,8 +471,8 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s)
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gradient, grid3(G.NX + 1, G.NY + 1, G.NZ + 1), dim3(BX, BY), 0, s, U, V,
-                       W, P, G, scale);
+    const int3 nb = nblocks(G.NX, G.NY, G.NZ + 1);
+    hipLaunchKernelGGL(k_gradient, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale, nb);
     return hipGetLastError();
 }
 

@@ -17,6 +17,7 @@
 // other-colour row, and its y/z neighbours are entry x'>>1 of the adjacent
 // other-colour rows -- the same k.
 #include "af_rbgs_split.h"
+#include "af_xcd.h"
 
 #pragma clang fp contract(off)
 
@@ -51,12 +52,14 @@ __device__ __forceinline__ int parity(int xi, int yi, int zi, int g)
 // neighbour is +0).  MODE 2: second pass of a solve from zero (the cell's own
 // old value, used for mirrored ghosts, is +0).
 template <typename T, int MODE>
-__global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color)
+__global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color, int3 nb)
 {
     if (a.done && *a.done) return;
-    const int k = blockIdx.x * BX + threadIdx.x;
-    const int yi = blockIdx.y * BY + threadIdx.y;
-    const int zi = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    const int k = bx * BX + threadIdx.x;
+    const int yi = by * BY + threadIdx.y;
+    const int zi = bz;
     const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
     if (yi >= NY) return;
     const int x0 = (color + yi + zi + 3 * a.g) & 1;
@@ -89,11 +92,13 @@ __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color
 // Natural interior -> split (both colours).  `nat` uses the padded layout.
 template <typename TS, typename TN>
 __global__ __launch_bounds__(BX *BY) void k_to_split(const TN *nat, int sy, int sz, int g, int NX,
-                                                     int NY, int NXH, TS *red, TS *black)
+                                                     int NY, int NXH, TS *red, TS *black, int3 nb)
 {
-    const int xi = blockIdx.x * BX + threadIdx.x;
-    const int yi = blockIdx.y * BY + threadIdx.y;
-    const int zi = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    const int xi = bx * BX + threadIdx.x;
+    const int yi = by * BY + threadIdx.y;
+    const int zi = bz;
     if (xi >= NX || yi >= NY) return;
     const TN v = nat[(xi + g) + (yi + g) * sy + (zi + g) * sz];
     const int i = (zi * NY + yi) * NXH + (xi >> 1);
@@ -105,12 +110,14 @@ __global__ __launch_bounds__(BX *BY) void k_to_split(const TN *nat, int sy, int 
 template <typename TS, bool ADD>
 __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int sz, int g, int NX,
                                                        int NY, int NXH, const TS *red,
-                                                       const TS *black, const int *done)
+                                                       const TS *black, const int *done, int3 nb)
 {
     if (done && *done) return;
-    const int xi = blockIdx.x * BX + threadIdx.x;
-    const int yi = blockIdx.y * BY + threadIdx.y;
-    const int zi = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    const int xi = bx * BX + threadIdx.x;
+    const int yi = by * BY + threadIdx.y;
+    const int zi = bz;
     if (xi >= NX || yi >= NY) return;
     const int i = (zi * NY + yi) * NXH + (xi >> 1);
     const double v = (double)(parity(xi, yi, zi, g) ? black[i] : red[i]);
@@ -120,38 +127,36 @@ __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int 
 
 __global__ __launch_bounds__(BX *BY) void k_rhs32_split(const double *div, int sy, int sz, int g,
                                                         int NX, int NY, int NXH, double h2,
-                                                        float *red, float *black)
+                                                        float *red, float *black, int3 nb)
 {
-    const int xi = blockIdx.x * BX + threadIdx.x;
-    const int yi = blockIdx.y * BY + threadIdx.y;
-    const int zi = blockIdx.z;
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    const int xi = bx * BX + threadIdx.x;
+    const int yi = by * BY + threadIdx.y;
+    const int zi = bz;
     if (xi >= NX || yi >= NY) return;
     const float v = (float)(-(h2 * div[(xi + g) + (yi + g) * sy + (zi + g) * sz]));
     const int i = (zi * NY + yi) * NXH + (xi >> 1);
     if (parity(xi, yi, zi, g)) black[i] = v; else red[i] = v;
 }
 
-inline dim3 grid_half(int NXH, int NY, int NZ)
+inline int3 nblocks(int nx, int ny, int nz)
 {
-    return dim3((NXH + BX - 1) / BX, (NY + BY - 1) / BY, NZ);
+    return make_int3((nx + BX - 1) / BX, (ny + BY - 1) / BY, nz);
 }
-
-inline dim3 grid_full(int NX, int NY, int NZ)
-{
-    return dim3((NX + BX - 1) / BX, (NY + BY - 1) / BY, NZ);
-}
+inline dim3 grid1(int3 nb) { return dim3((unsigned)(nb.x * nb.y * nb.z)); }
 
 }  // namespace
 
 template <typename T>
 hipError_t launch_split_pass(const SplitArgs<T> &a, int color, int mode, hipStream_t s)
 {
-    const dim3 grid = grid_half(a.NXH, a.NY, a.NZ);
+    const int3 nb = nblocks(a.NXH, a.NY, a.NZ);
     const dim3 blk(BX, BY);
     switch (mode) {
-    case 0: hipLaunchKernelGGL((k_split_pass<T, 0>), grid, blk, 0, s, a, color); break;
-    case 1: hipLaunchKernelGGL((k_split_pass<T, 1>), grid, blk, 0, s, a, color); break;
-    default: hipLaunchKernelGGL((k_split_pass<T, 2>), grid, blk, 0, s, a, color); break;
+    case 0: hipLaunchKernelGGL((k_split_pass<T, 0>), grid1(nb), blk, 0, s, a, color, nb); break;
+    case 1: hipLaunchKernelGGL((k_split_pass<T, 1>), grid1(nb), blk, 0, s, a, color, nb); break;
+    default: hipLaunchKernelGGL((k_split_pass<T, 2>), grid1(nb), blk, 0, s, a, color, nb); break;
     }
     return hipGetLastError();
 }
@@ -160,8 +165,9 @@ template <typename TS, typename TN>
 hipError_t launch_to_split(const TN *nat, int sy, int sz, Geom G, TS *red, TS *black, hipStream_t s)
 {
     const int NXH = (G.NX + 1) / 2;
-    hipLaunchKernelGGL((k_to_split<TS, TN>), grid_full(G.NX, G.NY, G.NZ), dim3(BX, BY), 0, s, nat,
-                       sy, sz, G.g, G.NX, G.NY, NXH, red, black);
+    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    hipLaunchKernelGGL((k_to_split<TS, TN>), grid1(nb), dim3(BX, BY), 0, s, nat, sy, sz, G.g, G.NX,
+                       G.NY, NXH, red, black, nb);
     return hipGetLastError();
 }
 
@@ -170,13 +176,13 @@ hipError_t launch_from_split(double *nat, int sy, int sz, Geom G, const TS *red,
                              bool add, const int *done, hipStream_t s)
 {
     const int NXH = (G.NX + 1) / 2;
-    const dim3 grid = grid_full(G.NX, G.NY, G.NZ);
+    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
     if (add)
-        hipLaunchKernelGGL((k_from_split<TS, true>), grid, dim3(BX, BY), 0, s, nat, sy, sz, G.g,
-                           G.NX, G.NY, NXH, red, black, done);
+        hipLaunchKernelGGL((k_from_split<TS, true>), grid1(nb), dim3(BX, BY), 0, s, nat, sy, sz,
+                           G.g, G.NX, G.NY, NXH, red, black, done, nb);
     else
-        hipLaunchKernelGGL((k_from_split<TS, false>), grid, dim3(BX, BY), 0, s, nat, sy, sz, G.g,
-                           G.NX, G.NY, NXH, red, black, done);
+        hipLaunchKernelGGL((k_from_split<TS, false>), grid1(nb), dim3(BX, BY), 0, s, nat, sy, sz,
+                           G.g, G.NX, G.NY, NXH, red, black, done, nb);
     return hipGetLastError();
 }
 
@@ -184,8 +190,9 @@ hipError_t launch_rhs32_split(const double *div, int sy, int sz, Geom G, double 
                               float *black, hipStream_t s)
 {
     const int NXH = (G.NX + 1) / 2;
-    hipLaunchKernelGGL(k_rhs32_split, grid_full(G.NX, G.NY, G.NZ), dim3(BX, BY), 0, s, div, sy,
-                       sz, G.g, G.NX, G.NY, NXH, h2, red, black);
+    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    hipLaunchKernelGGL(k_rhs32_split, grid1(nb), dim3(BX, BY), 0, s, div, sy, sz, G.g, G.NX, G.NY,
+                       NXH, h2, red, black, nb);
     return hipGetLastError();
 }
 
