@@ -67,7 +67,7 @@ class _CConfig(C.Structure):
         ("Precision", C.c_int32), ("MixedMaxCycles", C.c_int32), ("MixedSweeps", C.c_int32),
         ("MixedFixSweeps", C.c_int32), ("MixedTarget", C.c_double),
         ("TileK", C.c_int32), ("TileRows", C.c_int32), ("TileZ", C.c_int32),
-        ("Reserved1", C.c_int32),
+        ("SweepShape", C.c_int32),
     ]
 
 
@@ -110,6 +110,8 @@ _SIGNATURES = [
     ("af_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("af_mixed_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("af_solver_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]),
+    ("af_sweep_shape", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]),
     ("af_last_error", C.c_char_p, []),
 ]
@@ -164,7 +166,7 @@ class Config:
     TileK: int = 0
     TileRows: int = 0
     TileZ: int = 0
-    Reserved1: int = 0
+    SweepShape: int = 0
 
     def _to_c(self) -> _CConfig:
         c = _CConfig()
@@ -371,6 +373,11 @@ class Engine:
         k, r, z = C.c_int32(), C.c_int32(), C.c_int32()
         _check(load_library().af_solver_tile(self._handle(), C.byref(k), C.byref(r), C.byref(z)))
         return int(k.value), int(r.value), int(z.value)
+
+    def sweep_shape(self) -> tuple[int, int, int]:
+        s, r, z = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(load_library().af_sweep_shape(self._handle(), C.byref(s), C.byref(r), C.byref(z)))
+        return int(s.value), int(r.value), int(z.value)
 
     def mixed_stats(self) -> tuple[float, int]:
         d = C.c_double()

@@ -27,7 +27,6 @@ namespace af {
 namespace {
 
 constexpr int BX = 64;
-constexpr int BY = 4;
 
 __device__ __forceinline__ int idx(const DField &f, int x, int y, int z)
 {
@@ -118,44 +117,78 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, int x, int y,
 }
 
 // One launch per sweep, all three components; threads cover x in [g,g+NX],
-// y in [g,g+NY], z in [g,g+NZ] and compute every component face that exists
-// at their coordinate.
-template <int AX>
-__global__ __launch_bounds__(BX *BY) void k_sweep(SweepArgs a, int3 nb)
+// y in [g,g+NY] and ZR consecutive z in [g,g+NZ], and compute every
+// component face that exists at their coordinates.  All face values are
+// formed before any is stored.  The workgroup is 64 x BYT threads; the
+// engine's tuner picks (BYT, ZR).
+template <int AX, int BYT, int ZR>
+__global__ __launch_bounds__(BX *BYT) void k_sweep(SweepArgs a, int3 nb)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = a.G.g;
     const int x = g + bx * BX + threadIdx.x;
-    const int y = g + by * BY + threadIdx.y;
-    const int z = g + bz;
+    const int y = g + by * BYT + threadIdx.y;
     if (x > g + a.G.NX || y > g + a.G.NY) return;
-    const bool xin = x < g + a.G.NX, yin = y < g + a.G.NY, zin = z < g + a.G.NZ;
-    const bool du = yin && zin, dv = xin && zin, dw = xin && yin;
+    const bool xin = x < g + a.G.NX, yin = y < g + a.G.NY;
     bool bad = false;
-    double vu = 0, vv = 0, vw = 0;
-    if (du) vu = advect_value<AX, 0>(a, x, y, z, bad);
-    if (dv) vv = advect_value<AX, 1>(a, x, y, z, bad);
-    if (dw) vw = advect_value<AX, 2>(a, x, y, z, bad);
-    if (du) a.T0.d[idx(a.T0, x, y, z)] = vu;
-    if (dv) a.T1.d[idx(a.T1, x, y, z)] = vv;
-    if (dw) a.T2.d[idx(a.T2, x, y, z)] = vw;
+    double vu[ZR], vv[ZR], vw[ZR];
+    bool du[ZR], dv[ZR], dw[ZR];
+#pragma unroll
+    for (int r = 0; r < ZR; r++) {
+        const int z = g + bz * ZR + r;
+        const bool zok = z <= g + a.G.NZ, zin = z < g + a.G.NZ;
+        du[r] = yin && zin;
+        dv[r] = xin && zin;
+        dw[r] = xin && yin && zok;
+        vu[r] = vv[r] = vw[r] = 0;
+        if (du[r]) vu[r] = advect_value<AX, 0>(a, x, y, z, bad);
+        if (dv[r]) vv[r] = advect_value<AX, 1>(a, x, y, z, bad);
+        if (dw[r]) vw[r] = advect_value<AX, 2>(a, x, y, z, bad);
+    }
+#pragma unroll
+    for (int r = 0; r < ZR; r++) {
+        const int z = g + bz * ZR + r;
+        if (du[r]) a.T0.d[idx(a.T0, x, y, z)] = vu[r];
+        if (dv[r]) a.T1.d[idx(a.T1, x, y, z)] = vv[r];
+        if (dw[r]) a.T2.d[idx(a.T2, x, y, z)] = vw[r];
+    }
     if (bad) *a.flag = 1;
+}
+
+template <int BYT, int ZRT>
+hipError_t launch_shape(int axis, const SweepArgs &a, hipStream_t s)
+{
+    const int3 nb = make_int3((a.G.NX + 1 + BX - 1) / BX, (a.G.NY + 1 + BYT - 1) / BYT,
+                              (a.G.NZ + 1 + ZRT - 1) / ZRT);
+    const dim3 grid((unsigned)(nb.x * nb.y * nb.z));
+    const dim3 block(BX, BYT);
+    switch (axis) {
+    case 0: hipLaunchKernelGGL((k_sweep<0, BYT, ZRT>), grid, block, 0, s, a, nb); break;
+    case 1: hipLaunchKernelGGL((k_sweep<1, BYT, ZRT>), grid, block, 0, s, a, nb); break;
+    default: hipLaunchKernelGGL((k_sweep<2, BYT, ZRT>), grid, block, 0, s, a, nb); break;
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
 
+int sweep_shape_count() { return 8; }
+
+// Workgroup shapes the tuner chooses from: (rows per block, z per thread).
 hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s)
 {
-    const int3 nb = make_int3((a.G.NX + 1 + BX - 1) / BX, (a.G.NY + 1 + BY - 1) / BY, a.G.NZ + 1);
-    const dim3 grid((unsigned)(nb.x * nb.y * nb.z));
-    const dim3 block(BX, BY);
-    switch (axis) {
-    case 0: hipLaunchKernelGGL(k_sweep<0>, grid, block, 0, s, a, nb); break;
-    case 1: hipLaunchKernelGGL(k_sweep<1>, grid, block, 0, s, a, nb); break;
-    default: hipLaunchKernelGGL(k_sweep<2>, grid, block, 0, s, a, nb); break;
+    switch (a.shape) {
+    case 0: return launch_shape<1, 1>(axis, a, s);
+    case 1: return launch_shape<1, 2>(axis, a, s);
+    case 2: return launch_shape<2, 1>(axis, a, s);
+    case 3: return launch_shape<2, 2>(axis, a, s);
+    case 4: return launch_shape<4, 1>(axis, a, s);
+    case 5: return launch_shape<4, 2>(axis, a, s);
+    case 6: return launch_shape<8, 1>(axis, a, s);
+    case 7: return launch_shape<16, 1>(axis, a, s);
+    default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 }  // namespace af

@@ -130,6 +130,7 @@ struct af_engine {
     // Colour-split FP64 solve buffers: P and Div, red and black, interior only.
     double *ps_red = nullptr, *ps_black = nullptr, *ds_red = nullptr, *ds_black = nullptr;
     Tile tile64;                // red-black schedule (see Tile)
+    int sweep_shape = 4;        // advection workgroup shape (tuned at New)
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
 };
@@ -187,6 +188,7 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
     a.invH = e->invH;
     a.lam = e->cfg.Viscosity * dt * e->invH * e->invH;
     a.flag = e->flag;
+    a.shape = e->sweep_shape;
     return launch_sweep(axis, a, e->stream);
 }
 
@@ -480,6 +482,61 @@ int tune_blocked(af_engine *e, int k, Tile *out)
     return AF_OK;
 }
 
+// Advection workgroup shape: every shape gives the same bits (faces are
+// independent), so the fastest on this grid is kept.  Timed on the engine's
+// own (zeroed) arena, X sweep; cached per grid for the process.
+std::map<std::tuple<int, int, int, int>, int> g_sweep_cache;
+
+int choose_sweep_shape(af_engine *e)
+{
+    const af_config &c = e->cfg;
+    if (c.SweepShape > 0) {
+        if (c.SweepShape > sweep_shape_count())
+            return fail(AF_ERR_CONFIG, "aeroflux: sweep shape %d out of range 1..%d",
+                        (int)c.SweepShape, sweep_shape_count());
+        e->sweep_shape = c.SweepShape - 1;
+        return AF_OK;
+    }
+    const Geom &G = e->G;
+    const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_sweep_cache.find(key);
+        if (it != g_sweep_cache.end()) {
+            e->sweep_shape = it->second;
+            return AF_OK;
+        }
+    }
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a), "create event");
+    HIP_TRY(hipEventCreate(&b), "create event");
+    float best = 1e30f;
+    int bs = e->sweep_shape;
+    hipError_t err = hipSuccess;
+    for (int s = 0; s < sweep_shape_count() && err == hipSuccess; s++) {
+        e->sweep_shape = s;
+        err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
+        if (err == hipSuccess) err = hipEventRecord(a, e->stream);
+        for (int r = 0; r < 3 && err == hipSuccess; r++)
+            err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
+        if (err == hipSuccess) err = hipEventRecord(b, e->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(b);
+        float ms = 0;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+        if (err == hipSuccess && ms < best) {
+            best = ms;
+            bs = s;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (err != hipSuccess) return hip_fail(err, "tune advection workgroup shape");
+    e->sweep_shape = bs;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_sweep_cache[key] = bs;
+    return AF_OK;
+}
+
 int choose_schedule(af_engine *e)
 {
     const af_config &c = e->cfg;
@@ -679,7 +736,9 @@ int af_new(const af_config *cfg_in, af_engine **out)
     }
     if ((err = hipStreamSynchronize(e->stream)) != hipSuccess)
         return bail(err, "initialise engine", AF_ERR_HIP);
-    if (int rc = choose_schedule(e)) {
+    int rc = choose_schedule(e);
+    if (rc == AF_OK) rc = choose_sweep_shape(e);
+    if (rc != AF_OK) {
         std::string msg = g_err;
         release(e);
         delete e;
@@ -962,6 +1021,17 @@ int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchun
     return AF_OK;
 }
 
+int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper)
+{
+    if (!e) return fail(AF_ERR_ARG, "aeroflux: null engine");
+    static const int kRows[] = {1, 1, 2, 2, 4, 4, 8, 16};
+    static const int kZ[] = {1, 2, 1, 2, 1, 2, 1, 1};
+    if (shape) *shape = e->sweep_shape;
+    if (rows) *rows = kRows[e->sweep_shape];
+    if (zper) *zper = kZ[e->sweep_shape];
+    return AF_OK;
+}
+
 int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles)
 {
     STAGE_BEGIN(e);
@@ -1002,12 +1072,14 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
             }
             return launch_rbgs(P, D, e->G, e->h2, AF_COLOR_RED, true, e->stream);
         case 1: return enq_sweep(e, d, AF_AXIS_X, e->cfg.DT * 0.5);
+        case 4: return enq_sweep(e, d, AF_AXIS_Y, e->cfg.DT * 0.5);
+        case 5: return enq_sweep(e, d, AF_AXIS_Z, e->cfg.DT * 0.5);
         case 2: return enq_divergence(e, d);
         case 3: return enq_gradient(e, d);
         default: return hipErrorInvalidValue;
         }
     };
-    if (kernel < 0 || kernel > 3) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
+    if (kernel < 0 || kernel > 5) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
     HIP_TRY(one(), "warm-up launch");
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a), "create event");

@@ -26,7 +26,11 @@ struct SweepArgs {
     Geom G;
     double dt, invH, lam;
     int *flag;             // set to 1 on a NaN departure coordinate
+    int shape = 4;         // workgroup shape, 0 .. sweep_shape_count()-1
 };
+
+// Number of sweep workgroup shapes (rows per block x z per thread).
+int sweep_shape_count();
 
 // Launchers.  All enqueue on `s` and return the hipGetLastError() code.
 hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t s);

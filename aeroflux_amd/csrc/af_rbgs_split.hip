@@ -48,9 +48,14 @@ __device__ __forceinline__ int parity(int xi, int yi, int zi, int g)
     return (xi + yi + zi + 3 * g) & 1;
 }
 
+// Planes per thread: a pass over one plane per thread leaves each wave too
+// little work to keep the CUs full, so every thread updates ZR planes.
+constexpr int ZR = 4;
+
 // MODE 0: normal pass.  MODE 1: first pass of a solve from zero (every
 // neighbour is +0).  MODE 2: second pass of a solve from zero (the cell's own
-// old value, used for mirrored ghosts, is +0).
+// old value, used for mirrored ghosts, is +0).  All ZR results are formed
+// before any is stored, so every load can be issued up front.
 template <typename T, int MODE>
 __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color, int3 nb)
 {
@@ -59,34 +64,44 @@ __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int k = bx * BX + threadIdx.x;
     const int yi = by * BY + threadIdx.y;
-    const int zi = bz;
     const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
     if (yi >= NY) return;
-    const int x0 = (color + yi + zi + 3 * a.g) & 1;
-    const int xi = x0 + 2 * k;
-    if (xi >= NX) return;
-    const int row = zi * NY + yi;
-    const int i = row * NXH + k;
     T *cur = color ? a.black : a.red;
     const T *oth = color ? a.red : a.black;
     const T *rhs = color ? a.rhs_black : a.rhs_red;
-    T sum;
-    if (MODE == 1) {
-        sum = T(0) + T(0) + T(0) + T(0) + T(0) + T(0);
-    } else {
-        const T c = MODE == 2 ? T(0) : ((xi == 0 || xi == NX - 1 || yi == 0 || yi == NY - 1 ||
-                                         zi == 0 || zi == NZ - 1)
-                                            ? cur[i]
-                                            : T(0));
-        const T xm = xi == 0 ? c : oth[row * NXH + ((xi - 1) >> 1)];
-        const T xp = xi == NX - 1 ? c : oth[row * NXH + ((xi + 1) >> 1)];
-        const T ym = yi == 0 ? c : oth[i - NXH];
-        const T yp = yi == NY - 1 ? c : oth[i + NXH];
-        const T zm = zi == 0 ? c : oth[i - NY * NXH];
-        const T zp = zi == NZ - 1 ? c : oth[i + NY * NXH];
-        sum = xm + xp + ym + yp + zm + zp;
+    T out[ZR];
+    int at[ZR];
+#pragma unroll
+    for (int r = 0; r < ZR; r++) {
+        const int zi = bz * ZR + r;
+        const int x0 = (color + yi + zi + 3 * a.g) & 1;
+        const int xi = x0 + 2 * k;
+        at[r] = -1;
+        if (zi >= NZ || xi >= NX) continue;
+        const int row = zi * NY + yi;
+        const int i = row * NXH + k;
+        at[r] = i;
+        T sum;
+        if (MODE == 1) {
+            sum = T(0) + T(0) + T(0) + T(0) + T(0) + T(0);
+        } else {
+            const T c = MODE == 2 ? T(0) : ((xi == 0 || xi == NX - 1 || yi == 0 || yi == NY - 1 ||
+                                             zi == 0 || zi == NZ - 1)
+                                                ? cur[i]
+                                                : T(0));
+            const T xm = xi == 0 ? c : oth[row * NXH + ((xi - 1) >> 1)];
+            const T xp = xi == NX - 1 ? c : oth[row * NXH + ((xi + 1) >> 1)];
+            const T ym = yi == 0 ? c : oth[i - NXH];
+            const T yp = yi == NY - 1 ? c : oth[i + NXH];
+            const T zm = zi == 0 ? c : oth[i - NY * NXH];
+            const T zp = zi == NZ - 1 ? c : oth[i + NY * NXH];
+            sum = xm + xp + ym + yp + zm + zp;
+        }
+        out[r] = relax_s<T>(sum, rhs[i], a.h2);
     }
-    cur[i] = relax_s<T>(sum, rhs[i], a.h2);
+#pragma unroll
+    for (int r = 0; r < ZR; r++)
+        if (at[r] >= 0) cur[at[r]] = out[r];
 }
 
 // Natural interior -> split (both colours).  `nat` uses the padded layout.
@@ -151,7 +166,7 @@ inline dim3 grid1(int3 nb) { return dim3((unsigned)(nb.x * nb.y * nb.z)); }
 template <typename T>
 hipError_t launch_split_pass(const SplitArgs<T> &a, int color, int mode, hipStream_t s)
 {
-    const int3 nb = nblocks(a.NXH, a.NY, a.NZ);
+    const int3 nb = nblocks(a.NXH, a.NY, (a.NZ + ZR - 1) / ZR);
     const dim3 blk(BX, BY);
     switch (mode) {
     case 0: hipLaunchKernelGGL((k_split_pass<T, 0>), grid1(nb), blk, 0, s, a, color, nb); break;

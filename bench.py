@@ -106,10 +106,13 @@ def profile_kernels(af, ic, n: int, reps: int) -> None:
     e.StepN(2)
     t = e.time_kernel(0, reps)
     name, nb = dominant_kernel(e, n ** 3)
+    t_sweep = e.time_kernel(1, reps)
     e.Close()
     print(json.dumps({"profile_kernels": name, "copy_bytes": nbytes, "copies": 3,
                       "kernel_launches": reps + 1, "avg_ms": t,
-                      "algorithmic_bytes_per_launch": nb}), flush=True)
+                      "algorithmic_bytes_per_launch": nb,
+                      "sweep_x_launches": reps + 1, "sweep_x_avg_ms": t_sweep,
+                      "sweep_algorithmic_bytes_per_launch": 48 * n ** 3}), flush=True)
 
 
 def cpu_baseline(cfg: dict, f: dict, seconds: float, workers: int) -> dict:

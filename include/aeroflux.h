@@ -95,7 +95,8 @@ typedef struct af_config {
                                  launch, temporally blocked, arena layout */
     int32_t TileRows;         /* blocked: rows per thread, 1 or 2 (0 -> tuned) */
     int32_t TileZ;            /* blocked: z-planes per workgroup (0 -> tuned) */
-    int32_t Reserved1;
+    int32_t SweepShape;       /* advection workgroup shape: 0 -> tuned, k ->
+                                 shape k-1 (see af_sweep_shape) */
 } af_config;
 
 /* Field, grid.go:12-21 (descriptor after any commitVelocity swaps). */
@@ -193,7 +194,8 @@ int af_project(af_engine *e);                           /* project.go:12-34 */
  * step launches it.  kernel: 0 = one launch of the red-black kernel the
  * solve uses (one colour pass on the colour-split layout by default, one
  * pass on the arena layout for TileK = -1, TileK passes for TileK > 0),
- * 1 = advection sweep (X axis), 2 = divergence, 3 = gradient subtract.
+ * 1 = advection sweep (X axis), 2 = divergence, 3 = gradient subtract,
+ * 4 = advection sweep (Y axis), 5 = advection sweep (Z axis).
  * No reference
  * counterpart (measurement only); it overwrites P and the Tmp fields. */
 int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
@@ -207,6 +209,12 @@ int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles);
  * passes per launch (-1: one pass per launch), rows per thread, z-chunk.
  * No reference counterpart. */
 int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchunk);
+
+/* Advection sweep workgroup shape in use: index, rows per workgroup and z
+ * positions per thread (64 lanes along x).  Chosen by timing every shape on
+ * the engine's grid at New unless SweepShape fixes it.  No reference
+ * counterpart. */
+int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *af_last_error(void);

@@ -408,3 +408,20 @@ def test_mixed_precision_is_reproducible(af):
         out, d, _ = run_engine(af, cfg, f, 3)
         outs.append([out[k].tobytes() for k in "PUVW"])
     assert outs[0] == outs[1]
+
+
+@pytest.mark.parametrize("shape", range(1, 9))
+def test_step_bitwise_across_sweep_shapes(af, oracle_lib, shape):
+    """Every advection workgroup shape gives the reference's bits."""
+    cfg = dict(NX=20, NY=18, NZ=14, H=1 / 20, DT=0.01, Viscosity=1e-4, GaussSeidelIters=6)
+    f = ic.taylor_green(20, 18, 14, 1, 1 / 20, seed=21, amp=3.0, noise=0.05)
+    ge, oe = make_pair(af, oracle_lib, f, **cfg)
+    ge.Close()
+    ge = af.New(af.Config(**cfg, SweepShape=shape))
+    assert ge.sweep_shape()[0] == shape - 1
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        ge.Mesh.SetData(ge.Mesh.field(fid), f[name])
+    for s in range(2):
+        ge.Step(); assert oe.step() == 0
+    assert_bitwise(ge, oe, range(8), f"sweep shape {shape}")
+    ge.Close()
