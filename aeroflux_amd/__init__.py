@@ -34,6 +34,7 @@ __all__ = [
     "FIELD_P", "FIELD_U", "FIELD_V", "FIELD_W", "FIELD_DIV", "FIELD_TMP0",
     "FIELD_TMP1", "FIELD_TMP2", "AXIS_X", "AXIS_Y", "AXIS_Z", "COLOR_RED",
     "COLOR_BLACK", "PRECISION_FP64", "PRECISION_MIXED", "lib_path", "load_library",
+    "Batch", "BatchConfig", "SlotBytes",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -76,6 +77,24 @@ class _CField(C.Structure):
                 ("SY", C.c_int64), ("SZ", C.c_int64), ("Count", C.c_int64)]
 
 
+class _CBatchConfig(C.Structure):
+    _fields_ = [
+        ("Precision", C.c_int32), ("MaxSlots", C.c_int32), ("MemoryBudget", C.c_int64),
+        ("GaussSeidelIters", C.c_int64), ("MixedMaxCycles", C.c_int32),
+        ("MixedSweeps", C.c_int32), ("MixedFixSweeps", C.c_int32), ("MixedChecked", C.c_int32),
+        ("SweepShape", C.c_int32), ("Workgroups", C.c_int32),
+    ]
+
+
+class _CBatchInfo(C.Structure):
+    _fields_ = [
+        ("Slots", C.c_int32), ("SweepShape", C.c_int32), ("SweepWorkgroups", C.c_int32),
+        ("PassWorkgroups", C.c_int32), ("OtherWorkgroups", C.c_int32), ("Tuned", C.c_int32),
+        ("Cells", C.c_int64), ("BytesInUse", C.c_int64), ("BytesLargestFree", C.c_int64),
+        ("MemoryBudget", C.c_int64),
+    ]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported symbol in include/aeroflux.h.
@@ -113,6 +132,25 @@ _SIGNATURES = [
                                  C.POINTER(C.c_int32)]),
     ("af_sweep_shape", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]),
+    ("af_batch_new", C.c_int, [C.POINTER(_CBatchConfig), C.POINTER(C.c_void_p)]),
+    ("af_batch_close", C.c_int, [C.c_void_p]),
+    ("af_batch_free", None, [C.c_void_p]),
+    ("af_batch_get_info", C.c_int, [C.c_void_p, C.POINTER(_CBatchInfo)]),
+    ("af_batch_slot_bytes", C.c_int64, [C.POINTER(_CConfig), C.c_int32]),
+    ("af_batch_add", C.c_int, [C.c_void_p, C.POINTER(_CConfig), C.POINTER(C.c_int32)]),
+    ("af_batch_remove", C.c_int, [C.c_void_p, C.c_int32]),
+    ("af_batch_step_n", C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int32), C.c_int32,
+                                  C.POINTER(C.c_int32)]),
+    ("af_batch_field_desc", C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.POINTER(_CField)]),
+    ("af_batch_field_read", C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.c_void_p, C.c_int64]),
+    ("af_batch_field_write", C.c_int, [C.c_void_p, C.c_int32, C.c_int, C.c_void_p, C.c_int64]),
+    ("af_batch_apply_velocity_bc", C.c_int, [C.c_void_p, C.c_int32]),
+    ("af_batch_max_divergence", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    ("af_batch_kinetic_energy", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    ("af_batch_mixed_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int32)]),
+    ("af_batch_tune", C.c_int, [C.c_void_p]),
+    ("af_batch_time_stage", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("af_last_error", C.c_char_p, []),
 ]
 
@@ -397,3 +435,170 @@ def New(cfg: Config) -> Engine:
     ncfg = Config()
     ncfg._from_c(out)
     return Engine(h.value, ncfg)
+
+
+# ------------------------------------------------------------------ batching
+
+
+@dataclass
+class BatchConfig:
+    """Batch of engines stepped together (no reference counterpart; each slot
+    steps as Step, aeroflux.go:348-409, does).  Zero fields take defaults:
+    1024 slots, a 1 GiB pool, the Config.Normalize solve schedule, tuned
+    sweep shape and workgroup counts."""
+    Precision: int = PRECISION_FP64
+    MaxSlots: int = 0
+    MemoryBudget: int = 0
+    GaussSeidelIters: int = 0
+    MixedMaxCycles: int = 0
+    MixedSweeps: int = 0
+    MixedFixSweeps: int = 0
+    MixedChecked: int = 0
+    SweepShape: int = 0
+    Workgroups: int = 0
+
+    def _to_c(self) -> _CBatchConfig:
+        c = _CBatchConfig()
+        for f in dc_fields(self):
+            setattr(c, f.name, int(getattr(self, f.name)))
+        return c
+
+
+def SlotBytes(cfg: Config, precision: int) -> int:
+    """HBM bytes one batch slot of ``cfg`` takes (0 if cfg is invalid)."""
+    c = cfg._to_c()
+    return int(load_library().af_batch_slot_bytes(C.byref(c), precision))
+
+
+class Batch:
+    """Many engines of uneven sizes, resident in one HBM pool and stepped
+    together: one persistent launch per stage over every slot's tiles, the
+    step replayed as one hipGraph.  Each slot's results are bit-identical to
+    the same engine stepped alone."""
+
+    def __init__(self, cfg: BatchConfig | None = None):
+        lib = load_library()
+        c = (cfg or BatchConfig())._to_c()
+        h = C.c_void_p()
+        _check(lib.af_batch_new(C.byref(c), C.byref(h)))
+        self._h = h.value
+        self._cfgs: dict[int, Config] = {}
+
+    def _handle(self) -> int:
+        if self._h is None:
+            raise AeroFluxError(AF_ERR_CLOSED, "aeroflux: batch is closed")
+        return self._h
+
+    def Add(self, cfg: Config) -> int:
+        """Adds an engine (New for one slot); returns its slot id."""
+        lib = load_library()
+        c = cfg._to_c()
+        _check(lib.af_config_normalize(C.byref(c)))
+        slot = C.c_int32()
+        _check(lib.af_batch_add(self._handle(), C.byref(c), C.byref(slot)))
+        n = Config()
+        n._from_c(c)
+        self._cfgs[int(slot.value)] = n
+        return int(slot.value)
+
+    def Remove(self, slot: int) -> None:
+        _check(load_library().af_batch_remove(self._handle(), slot))
+        self._cfgs.pop(slot, None)
+
+    def Cfg(self, slot: int) -> Config:
+        return self._cfgs[slot]
+
+    def StepN(self, n: int) -> list[int]:
+        """n batched steps; raises AeroFluxError (AF_ERR_FAULT) naming the
+        faulted slots, as Step does for one engine."""
+        cap = max(1, len(self._cfgs))
+        out = (C.c_int32 * cap)()
+        nf = C.c_int32()
+        rc = load_library().af_batch_step_n(self._handle(), n, out, cap, C.byref(nf))
+        if rc == AF_ERR_FAULT:
+            err = AeroFluxError(rc, load_library().af_last_error().decode())
+            err.slots = [int(out[i]) for i in range(min(nf.value, cap))]
+            raise err
+        _check(rc)
+        return []
+
+    def Step(self) -> None:
+        self.StepN(1)
+
+    def field(self, slot: int, fid: int) -> Field:
+        f = _CField()
+        _check(load_library().af_batch_field_desc(self._handle(), slot, fid, C.byref(f)))
+        return Field(fid, f.Off, f.NX, f.NY, f.NZ, f.SY, f.SZ, f.Count)
+
+    def Data(self, slot: int, fid: int) -> np.ndarray:
+        f = self.field(slot, fid)
+        out = np.empty(f.shape, dtype=np.float64)
+        _check(load_library().af_batch_field_read(self._handle(), slot, fid,
+                                                  out.ctypes.data_as(C.c_void_p), f.Count))
+        return out
+
+    def SetData(self, slot: int, fid: int, data: np.ndarray) -> None:
+        f = self.field(slot, fid)
+        arr = np.ascontiguousarray(data, dtype=np.float64).reshape(-1)
+        if arr.size != f.Count:
+            raise AeroFluxError(AF_ERR_ARG, f"aeroflux: field has {f.Count} elements, "
+                                            f"got {arr.size}")
+        _check(load_library().af_batch_field_write(self._handle(), slot, fid,
+                                                   arr.ctypes.data_as(C.c_void_p), f.Count))
+
+    def apply_velocity_bc(self, slot: int) -> None:  # stencil.go:246-315
+        _check(load_library().af_batch_apply_velocity_bc(self._handle(), slot))
+
+    def _scalar(self, fn, slot: int) -> float:
+        out = C.c_double()
+        _check(fn(self._handle(), slot, C.byref(out)))
+        return float(out.value)
+
+    def MaxDivergence(self, slot: int) -> float:  # project.go:210-232
+        return self._scalar(load_library().af_batch_max_divergence, slot)
+
+    def KineticEnergy(self, slot: int) -> float:  # project.go:238-258
+        return self._scalar(load_library().af_batch_kinetic_energy, slot)
+
+    def mixed_stats(self, slot: int) -> tuple[float, int]:
+        d = C.c_double()
+        n = C.c_int32()
+        _check(load_library().af_batch_mixed_stats(self._handle(), slot, C.byref(d),
+                                                   C.byref(n)))
+        return float(d.value), int(n.value)
+
+    def Tune(self) -> None:
+        _check(load_library().af_batch_tune(self._handle()))
+
+    def time_stage(self, stage: int, reps: int) -> float:
+        out = C.c_double()
+        _check(load_library().af_batch_time_stage(self._handle(), stage, reps, C.byref(out)))
+        return float(out.value)
+
+    def info(self) -> dict:
+        i = _CBatchInfo()
+        _check(load_library().af_batch_get_info(self._handle(), C.byref(i)))
+        return {name: getattr(i, name) for name, _ in _CBatchInfo._fields_}
+
+    def Close(self) -> None:
+        if self._h is not None:
+            lib = load_library()
+            rc = lib.af_batch_close(self._h)
+            lib.af_batch_free(self._h)
+            self._h = None
+            self._cfgs.clear()
+            _check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.Close()
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                load_library().af_batch_free(self._h)
+                self._h = None
+        except Exception:
+            pass

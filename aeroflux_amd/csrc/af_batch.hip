@@ -1,0 +1,1214 @@
+// af_batch.hip -- many engines of uneven sizes stepped together on one GPU.
+//
+// The reference steps one Engine per Step call (aeroflux.go:348-409).  Small
+// grids leave most of an MI355X idle and pay a launch per stage each, so a
+// batch keeps many engines ("slots") resident in one HBM pool and steps them
+// together:
+//   * every slot is New's arena (grid.go:79-134, same offsets) plus its
+//     colour-split solve buffers, carved out of one pool allocation;
+//   * each stage of the Strang palindrome is ONE persistent launch over a
+//     work-list of tiles of every resident slot; the work-list is a prefix
+//     table per stage kind, so membership changes rewrite a few kilobytes
+//     and never the launch sequence;
+//   * the whole step is captured once as a hipGraph and replayed;
+//   * workgroups take contiguous ranges of the work-list per XCD (workgroup
+//     b runs on XCD b % 8), so each XCD's L2 sees neighbouring tiles;
+//   * the per-thread arithmetic is af_bodies.h, shared with the single
+//     engine, so a slot's results are bit-identical to af_step's.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "../../include/aeroflux.h"
+#include "af_bodies.h"
+#include "af_host.h"
+#include "af_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace af {
+namespace {
+
+constexpr int BX = body::BX;
+constexpr int BY = 4;       // rows per workgroup of the 64x4 stages
+constexpr int PZ = 4;       // z-planes per thread of a colour pass
+constexpr int CHUNK = 256;  // cells per item of the shell stages
+constexpr int XCDS = 8;
+
+// Stage kinds: each has its own tile decomposition and prefix table.
+enum Kind {
+    K_VBC,    // velocity-BC shell cells, CHUNK per item
+    K_SWEEP,  // advection tiles (64 x rows x z-per-thread) of the union box
+    K_CELL,   // 64 x 4 tiles of one interior z-plane
+    K_GRAD,   // as K_CELL over NZ + 1 planes
+    K_PASS,   // colour-pass tiles of the split layout (64 x 4 x PZ)
+    K_PBC,    // pressure ghost-shell cells, CHUNK per item
+    K_COL,    // 64 x 4 interior columns (z loop inside)
+    K_ENG,    // one item per slot
+    K_COUNT
+};
+
+constexpr int kShapeRows[8] = {1, 1, 2, 2, 4, 4, 8, 16};
+constexpr int kShapeZ[8] = {1, 2, 1, 2, 1, 2, 1, 1};
+
+// Device descriptor of one slot.
+struct BSlot {
+    DField f[AF_FIELD_COUNT];  // P, U, V, W, Div, Tmp0, Tmp1, Tmp2 (unswapped)
+    Geom G;
+    double dt_half, invH, lam_half, h2, div_scale, grad_scale, res_to_div, target;
+    void *s_red, *s_black;  // FP64: P split; mixed: FP32 correction e
+    void *d_red, *d_black;  // FP64: Div split; mixed: FP32 right-hand side / residual
+    double *partials;       // residual and reduction partials
+    double *out;            // reduction result
+};
+
+// Kernel argument shared by every stage.
+struct BCtl {
+    const BSlot *slots;
+    const int *tab;  // nact, stride, act[stride], first[K_COUNT][stride]
+    int *flags, *done, *cycles;
+    double *last;
+};
+
+inline int64_t kind_items(int kind, const Geom &G, const DField &P, int shape)
+{
+    switch (kind) {
+    case K_VBC: {
+        int64_t cu, cv, cw;
+        body::velocity_bc_counts(G, cu, cv, cw);
+        return (cu + cv + cw + CHUNK - 1) / CHUNK;
+    }
+    case K_SWEEP: {
+        const int r = kShapeRows[shape], z = kShapeZ[shape];
+        return (int64_t)((G.NX + BX) / BX) * ((G.NY + r) / r) * ((G.NZ + z) / z);
+    }
+    case K_CELL: return (int64_t)((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY) * G.NZ;
+    case K_GRAD: return (int64_t)((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY) * (G.NZ + 1);
+    case K_PASS: {
+        const int nxh = (G.NX + 1) / 2;
+        return (int64_t)((nxh + BX - 1) / BX) * ((G.NY + BY - 1) / BY) * ((G.NZ + PZ - 1) / PZ);
+    }
+    case K_PBC:
+        return (body::pressure_shell_count(P.nx, P.ny, P.nz, G.g) + CHUNK - 1) / CHUNK;
+    case K_COL: return (int64_t)((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY);
+    default: return 1;
+    }
+}
+
+// Persistent loop over the work-list of stage KIND.  XCD x (= blockIdx.x %
+// 8) owns the x-th eighth of the items, in order; its workgroups stride
+// through that range.  f(slot, tile) runs with a workgroup-uniform slot and
+// tile, so bodies may synchronise the workgroup.  gridDim.x % 8 == 0.
+template <int KIND, typename F>
+__device__ __forceinline__ void for_items(const BCtl &c, F f)
+{
+    const int *tab = c.tab;
+    const int nact = tab[0], stride = tab[1];
+    const int *act = tab + 2;
+    const int *first = tab + 2 + stride * (1 + KIND);
+    const int total = first[nact];
+    const int lanes = (int)(gridDim.x / XCDS);
+    const int xcd = (int)(blockIdx.x % XCDS);
+    const int lo = (int)(((int64_t)total * xcd) / XCDS);
+    const int hi = (int)(((int64_t)total * (xcd + 1)) / XCDS);
+    int i = lo + (int)(blockIdx.x / XCDS);
+    if (i >= hi) return;
+    int a = 0, r = nact - 1;  // largest a with first[a] <= i
+    while (a < r) {
+        const int m = (a + r + 1) >> 1;
+        if (first[m] <= i) a = m; else r = m - 1;
+    }
+    for (; i < hi; i += lanes) {
+        while (first[a + 1] <= i) a++;
+        f(act[a], i - first[a]);
+    }
+}
+
+__device__ __forceinline__ const DField &velf(const BSlot &e, int sw, int c)
+{
+    return e.f[(sw ? AF_FIELD_TMP0 : AF_FIELD_U) + c];
+}
+__device__ __forceinline__ const DField &tmpf(const BSlot &e, int sw, int c)
+{
+    return e.f[(sw ? AF_FIELD_U : AF_FIELD_TMP0) + c];
+}
+
+// applyVelocityBC, stencil.go:246-315.
+__global__ __launch_bounds__(256) void kb_vbc(BCtl c, int sw)
+{
+    for_items<K_VBC>(c, [&](int s, int t) {
+        const BSlot &e = c.slots[s];
+        int64_t cu, cv, cw;
+        body::velocity_bc_counts(e.G, cu, cv, cw);
+        body::velocity_bc_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2), e.G,
+                               (int64_t)t * CHUNK + threadIdx.x, cu, cv, cw);
+    });
+}
+
+// advectDiffuseAxis, stencil.go:21-119.
+template <int AX, int BYT, int ZR>
+__global__ __launch_bounds__(BX *BYT) void kb_sweep(BCtl c, int sw)
+{
+    for_items<K_SWEEP>(c, [&](int s, int t) {
+        const BSlot &e = c.slots[s];
+        SweepArgs a;
+        a.U = velf(e, sw, 0); a.V = velf(e, sw, 1); a.W = velf(e, sw, 2);
+        a.T0 = tmpf(e, sw, 0); a.T1 = tmpf(e, sw, 1); a.T2 = tmpf(e, sw, 2);
+        a.G = e.G;
+        a.dt = e.dt_half;
+        a.invH = e.invH;
+        a.lam = e.lam_half;
+        a.flag = c.flags + s;
+        const int nbx = (e.G.NX + BX) / BX, nby = (e.G.NY + BYT) / BYT;
+        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        const int g = e.G.g;
+        body::sweep_column<AX, ZR>(a, g + bx * BX + threadIdx.x, g + by * BYT + threadIdx.y,
+                                   g + bz * ZR);
+    });
+}
+
+// computeDivergence, project.go:42-62, also writing the solve's split
+// right-hand side: the FP64 divergence itself, or the mixed solve's FP32
+// -(H^2 div).  Same values as a separate split pass over Div.
+template <bool MIXED>
+__global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
+{
+    for_items<K_CELL>(c, [&](int s, int t) {
+        const BSlot &e = c.slots[s];
+        const Geom &G = e.G;
+        const int nbx = (G.NX + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
+        const int bx = t % nbx, r = t / nbx, by = r % nby, zi = r / nby;
+        const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y;
+        double v;
+        if (!body::divergence_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2),
+                                   e.f[AF_FIELD_DIV], G, e.div_scale, xi + G.g, yi + G.g,
+                                   zi + G.g, v))
+            return;
+        const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
+        const bool black = body::parity(xi, yi, zi, G.g);
+        if (MIXED)
+            static_cast<float *>(black ? e.d_black : e.d_red)[i] = (float)(-(e.h2 * v));
+        else
+            static_cast<double *>(black ? e.d_black : e.d_red)[i] = v;
+    });
+}
+
+// One red-black colour pass on the split layout (gaussSeidelSweep,
+// project.go:103-130).  gated: skip slots whose mixed solve is done.
+template <typename T, int MODE>
+__global__ __launch_bounds__(BX *BY) void kb_pass(BCtl c, int color, int gated)
+{
+    for_items<K_PASS>(c, [&](int s, int t) {
+        if (gated && c.done[s]) return;
+        const BSlot &e = c.slots[s];
+        const Geom &G = e.G;
+        SplitArgs<T> a;
+        a.red = static_cast<T *>(e.s_red);
+        a.black = static_cast<T *>(e.s_black);
+        a.rhs_red = static_cast<const T *>(e.d_red);
+        a.rhs_black = static_cast<const T *>(e.d_black);
+        a.NX = G.NX; a.NY = G.NY; a.NZ = G.NZ; a.NXH = (G.NX + 1) / 2; a.g = G.g;
+        a.h2 = e.h2;
+        const int nbx = (a.NXH + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
+        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        body::split_pass_column<T, MODE, PZ>(a, color, bx * BX + threadIdx.x,
+                                             by * BY + threadIdx.y, bz * PZ);
+    });
+}
+
+// Split -> P interior (ADD: P += split).
+template <typename TS, bool ADD>
+__global__ __launch_bounds__(BX *BY) void kb_from_split(BCtl c, int gated)
+{
+    for_items<K_CELL>(c, [&](int s, int t) {
+        if (gated && c.done[s]) return;
+        const BSlot &e = c.slots[s];
+        const Geom &G = e.G;
+        const DField &P = e.f[AF_FIELD_P];
+        const int nbx = (G.NX + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
+        const int bx = t % nbx, r = t / nbx, by = r % nby, zi = r / nby;
+        body::from_split_cell<TS, ADD>(P.d, P.sy, P.sz, G.g, G.NX, G.NY, (G.NX + 1) / 2,
+                                       static_cast<const TS *>(e.s_red),
+                                       static_cast<const TS *>(e.s_black), bx * BX + threadIdx.x,
+                                       by * BY + threadIdx.y, zi);
+    });
+}
+
+// Final applyPressureBC of the solve, project.go:98 (outer layers zeroed for
+// Ghost > 1, as after the solve's zeroing).
+__global__ __launch_bounds__(256) void kb_pbc(BCtl c)
+{
+    for_items<K_PBC>(c, [&](int s, int t) {
+        const BSlot &e = c.slots[s];
+        const DField &P = e.f[AF_FIELD_P];
+        const int64_t k = (int64_t)t * CHUNK + threadIdx.x;
+        if (k >= body::pressure_shell_count(P.nx, P.ny, P.nz, e.G.g)) return;
+        body::pressure_bc_cell(P, e.G, k, e.G.g > 1);
+    });
+}
+
+// subtractGradient, project.go:172-205.
+__global__ __launch_bounds__(BX *BY) void kb_grad(BCtl c, int sw)
+{
+    for_items<K_GRAD>(c, [&](int s, int t) {
+        const BSlot &e = c.slots[s];
+        const Geom &G = e.G;
+        const int nbx = (G.NX + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
+        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        body::gradient_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2), e.f[AF_FIELD_P], G,
+                            e.grad_scale, G.g + bx * BX + threadIdx.x, G.g + by * BY + threadIdx.y,
+                            G.g + bz);
+    });
+}
+
+// Mixed solve: FP64 residual with FP32 split output and per-column partials.
+__global__ __launch_bounds__(BX *BY) void kb_resid(BCtl c, int gated)
+{
+    __shared__ double sh[BX * BY];
+    for_items<K_COL>(c, [&](int s, int t) {
+        if (gated && c.done[s]) return;
+        const BSlot &e = c.slots[s];
+        const Geom &G = e.G;
+        const int nbx = (G.NX + BX - 1) / BX;
+        const int bx = t % nbx, by = t / nbx;
+        const double m = body::residual_column(
+            e.f[AF_FIELD_P], e.f[AF_FIELD_DIV].d, static_cast<float *>(e.d_red),
+            static_cast<float *>(e.d_black), G, e.h2, bx * BX + threadIdx.x, by * BY + threadIdx.y);
+        const double b = body::block_max256(m, sh);
+        if (threadIdx.x == 0 && threadIdx.y == 0) e.partials[t] = b;
+    });
+}
+
+// Mixed solve: fold a slot's partials (fixed order), record max|div| and
+// raise the slot's done flag once its target is met.
+__global__ __launch_bounds__(256) void kb_res_final(BCtl c, int gated)
+{
+    __shared__ double sh[256];
+    for_items<K_ENG>(c, [&](int s, int) {
+        if (gated && c.done[s]) return;
+        const BSlot &e = c.slots[s];
+        const int np = ((e.G.NX + BX - 1) / BX) * ((e.G.NY + BY - 1) / BY);
+        double v = 0;
+        for (int k = threadIdx.x; k < np; k += 256) v = e.partials[k] > v ? e.partials[k] : v;
+        const double m = body::block_max256(v, sh);
+        if (threadIdx.x == 0) {
+            const double d = e.res_to_div * m;
+            c.last[s] = d;
+            c.cycles[s] = c.cycles[s] + 1;
+            if (d <= e.target) c.done[s] = 1;
+        }
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_mixed_reset(BCtl c, int checked)
+{
+    for_items<K_ENG>(c, [&](int s, int) {
+        if (threadIdx.x == 0) {
+            c.done[s] = 0;
+            c.cycles[s] = checked ? 0 : 1;
+        }
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_fill_int(int *p, int n, int v)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------ host side
+
+int64_t round256(int64_t n) { return (n + 255) & ~(int64_t)255; }
+
+struct SlotLayout {
+    af_field f[AF_FIELD_COUNT];
+    int64_t split_off[4];
+    int64_t partials_off, out_off, total;
+};
+
+bool slot_layout(const af_config &c, int precision, SlotLayout *L)
+{
+    const int64_t arena = mac_layout(L->f, c.NX, c.NY, c.NZ, c.Ghost);
+    int64_t off = round256(arena);
+    const int64_t elem = precision == AF_PRECISION_MIXED ? 4 : 8;
+    const int64_t n = c.NZ * c.NY * ((c.NX + 1) / 2) * elem;
+    for (int k = 0; k < 4; k++) {
+        L->split_off[k] = off;
+        off += round256(n);
+    }
+    Geom G{(int)c.NX, (int)c.NY, (int)c.NZ, (int)c.Ghost};
+    L->partials_off = off;
+    off += round256((int64_t)reduce_partials_needed(G) * 8);
+    L->out_off = off;
+    off += 256;
+    L->total = off;
+    for (int i = 0; i < AF_FIELD_COUNT; i++)
+        if (L->f[i].Count >= (int64_t)1 << 31) return false;
+    return true;
+}
+
+// First-fit pool over one device allocation.
+struct Pool {
+    char *base = nullptr;
+    int64_t cap = 0, used = 0;
+    std::map<int64_t, int64_t> free_;  // offset -> length
+
+    bool alloc(int64_t n, int64_t *off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            *off = it->first;
+            const int64_t rest = it->second - n;
+            const int64_t at = it->first + n;
+            free_.erase(it);
+            if (rest > 0) free_[at] = rest;
+            used += n;
+            return true;
+        }
+        return false;
+    }
+    void release(int64_t off, int64_t n)
+    {
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    int64_t largest() const
+    {
+        int64_t m = 0;
+        for (const auto &kv : free_) m = kv.second > m ? kv.second : m;
+        return m;
+    }
+};
+
+struct HSlot {
+    bool used = false;
+    af_config cfg{};
+    SlotLayout L{};
+    int64_t off = 0;
+};
+
+}  // namespace
+}  // namespace af
+
+using namespace af;
+
+struct af_batch {
+    af_batch_config cfg{};
+    int device = 0;
+    bool closed = true;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    std::vector<HSlot> slots;
+    BSlot *d_slots = nullptr, *h_slots = nullptr;  // h_slots pinned
+    int *d_tab = nullptr, *h_tab = nullptr;        // h_tab pinned
+    int64_t tab_ints = 0;
+    int *d_ctl = nullptr;    // flags | done | cycles, MaxSlots each
+    double *d_last = nullptr;
+    int *h_ints = nullptr;   // pinned, MaxSlots
+    double *h_dbl = nullptr; // pinned, MaxSlots
+    bool tab_dirty = true;
+    int tab_shape = -1;
+    hipGraphExec_t graph = nullptr;
+    int shape = 4, nwg_sweep = 2048, nwg_pass = 2048, nwg_other = 2048;
+    bool tuned = false;
+    int nact = 0;
+    int64_t cells = 0;
+};
+
+namespace {
+
+int check_open(const af_batch *b)
+{
+    if (!b) return fail(AF_ERR_ARG, "aeroflux: null batch");
+    if (b->closed) return fail(AF_ERR_CLOSED, "aeroflux: batch is closed");
+    return AF_OK;
+}
+
+int check_slot(const af_batch *b, int32_t slot)
+{
+    if (int rc = check_open(b)) return rc;
+    if (slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].used)
+        return fail(AF_ERR_ARG, "aeroflux: no engine in batch slot %d", (int)slot);
+    return AF_OK;
+}
+
+BCtl ctl(const af_batch *b)
+{
+    BCtl c;
+    c.slots = b->d_slots;
+    c.tab = b->d_tab;
+    const int m = b->cfg.MaxSlots;
+    c.flags = b->d_ctl;
+    c.done = b->d_ctl + m;
+    c.cycles = b->d_ctl + 2 * m;
+    c.last = b->d_last;
+    return c;
+}
+
+DField slot_field(const af_batch *b, const HSlot &h, int fid)
+{
+    const af_field &f = h.L.f[fid];
+    DField d;
+    d.d = reinterpret_cast<double *>(b->pool.base + h.off + f.Off);
+    d.sy = (int)f.SY; d.sz = (int)f.SZ;
+    d.nx = (int)f.NX; d.ny = (int)f.NY; d.nz = (int)f.NZ;
+    return d;
+}
+
+Geom slot_geom(const HSlot &h)
+{
+    return Geom{(int)h.cfg.NX, (int)h.cfg.NY, (int)h.cfg.NZ, (int)h.cfg.Ghost};
+}
+
+// Rebuilds the host work-list table for the current slots and sweep shape.
+int build_table(af_batch *b, int shape)
+{
+    const int stride = b->cfg.MaxSlots + 1;
+    int *t = b->h_tab;
+    int *act = t + 2;
+    int n = 0;
+    for (int s = 0; s < (int)b->slots.size(); s++)
+        if (b->slots[s].used) act[n++] = s;
+    t[0] = n;
+    t[1] = stride;
+    for (int k = 0; k < K_COUNT; k++) {
+        int *first = t + 2 + (int64_t)stride * (1 + k);
+        int64_t acc = 0;
+        first[0] = 0;
+        for (int a = 0; a < n; a++) {
+            const HSlot &h = b->slots[act[a]];
+            acc += kind_items(k, slot_geom(h), slot_field(b, h, AF_FIELD_P), shape);
+            if (acc > INT_MAX)
+                return fail(AF_ERR_CONFIG, "aeroflux: batch work-list exceeds 2^31 tiles");
+            first[a + 1] = (int)acc;
+        }
+    }
+    b->nact = n;
+    return AF_OK;
+}
+
+// Uploads the table through the pinned staging copy, after any earlier
+// copy from it has run.
+int upload_table(af_batch *b, int shape)
+{
+    if (!b->tab_dirty && b->tab_shape == shape) return AF_OK;
+    HIP_TRY(hipStreamSynchronize(b->stream), "upload batch work-list");
+    if (int rc = build_table(b, shape)) return rc;
+    HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, (size_t)b->tab_ints * sizeof(int),
+                           hipMemcpyHostToDevice, b->stream),
+            "upload batch work-list");
+    b->tab_dirty = false;
+    b->tab_shape = shape;
+    return AF_OK;
+}
+
+int sync(af_batch *b, const char *what)
+{
+    HIP_TRY(hipStreamSynchronize(b->stream), what);
+    return AF_OK;
+}
+
+// ---- stage launchers ----
+
+template <int BYT, int ZR>
+hipError_t sweep_shape_launch(const af_batch *b, int nwg, int axis, int sw)
+{
+    const BCtl c = ctl(b);
+    const dim3 grid(nwg), blk(BX, BYT);
+    switch (axis) {
+    case 0: hipLaunchKernelGGL((kb_sweep<0, BYT, ZR>), grid, blk, 0, b->stream, c, sw); break;
+    case 1: hipLaunchKernelGGL((kb_sweep<1, BYT, ZR>), grid, blk, 0, b->stream, c, sw); break;
+    default: hipLaunchKernelGGL((kb_sweep<2, BYT, ZR>), grid, blk, 0, b->stream, c, sw); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t enq_sweep(const af_batch *b, int shape, int nwg, int axis, int sw)
+{
+    switch (shape) {
+    case 0: return sweep_shape_launch<1, 1>(b, nwg, axis, sw);
+    case 1: return sweep_shape_launch<1, 2>(b, nwg, axis, sw);
+    case 2: return sweep_shape_launch<2, 1>(b, nwg, axis, sw);
+    case 3: return sweep_shape_launch<2, 2>(b, nwg, axis, sw);
+    case 4: return sweep_shape_launch<4, 1>(b, nwg, axis, sw);
+    case 5: return sweep_shape_launch<4, 2>(b, nwg, axis, sw);
+    case 6: return sweep_shape_launch<8, 1>(b, nwg, axis, sw);
+    default: return sweep_shape_launch<16, 1>(b, nwg, axis, sw);
+    }
+}
+
+template <typename T>
+hipError_t enq_pass(const af_batch *b, int nwg, int color, int mode, int gated)
+{
+    const BCtl c = ctl(b);
+    const dim3 grid(nwg), blk(BX, BY);
+    switch (mode) {
+    case 0: hipLaunchKernelGGL((kb_pass<T, 0>), grid, blk, 0, b->stream, c, color, gated); break;
+    case 1: hipLaunchKernelGGL((kb_pass<T, 1>), grid, blk, 0, b->stream, c, color, gated); break;
+    default: hipLaunchKernelGGL((kb_pass<T, 2>), grid, blk, 0, b->stream, c, color, gated); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t enq_vbc(const af_batch *b, int sw)
+{
+    hipLaunchKernelGGL(kb_vbc, dim3(b->nwg_other), dim3(256), 0, b->stream, ctl(b), sw);
+    return hipGetLastError();
+}
+
+hipError_t enq_div(const af_batch *b, int sw)
+{
+    const dim3 grid(b->nwg_other), blk(BX, BY);
+    if (b->cfg.Precision == AF_PRECISION_MIXED)
+        hipLaunchKernelGGL(kb_div<true>, grid, blk, 0, b->stream, ctl(b), sw);
+    else
+        hipLaunchKernelGGL(kb_div<false>, grid, blk, 0, b->stream, ctl(b), sw);
+    return hipGetLastError();
+}
+
+hipError_t enq_grad(const af_batch *b, int sw)
+{
+    hipLaunchKernelGGL(kb_grad, dim3(b->nwg_other), dim3(BX, BY), 0, b->stream, ctl(b), sw);
+    return hipGetLastError();
+}
+
+// solvePressureRedBlack, project.go:73-100, on every slot: FP64 colour-split
+// passes (the first from P == 0), or the mixed-precision defect correction.
+hipError_t enq_solve(const af_batch *b)
+{
+    const BCtl c = ctl(b);
+    const dim3 gp(b->nwg_other), blk(BX, BY);
+    hipError_t err = hipSuccess;
+    auto mode = [](int p) { return p == 0 ? 1 : (p == 1 ? 2 : 0); };
+    if (b->cfg.Precision != AF_PRECISION_MIXED) {
+        const int passes = (int)(2 * b->cfg.GaussSeidelIters);
+        for (int p = 0; p < passes && err == hipSuccess; p++)
+            err = enq_pass<double>(b, b->nwg_pass, p & 1, mode(p), 0);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((kb_from_split<double, false>), gp, blk, 0, b->stream, c, 0);
+    } else {
+        const bool checked = b->cfg.MixedChecked != 0;
+        hipLaunchKernelGGL(kb_mixed_reset, gp, dim3(256), 0, b->stream, c, checked ? 1 : 0);
+        for (int p = 0; p < 2 * b->cfg.MixedSweeps && err == hipSuccess; p++)
+            err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 0);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, c, 0);
+        if (checked) {
+            hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 0);
+            hipLaunchKernelGGL(kb_res_final, gp, dim3(256), 0, b->stream, c, 0);
+            for (int cyc = 1; cyc < b->cfg.MixedMaxCycles; cyc++) {
+                for (int p = 0; p < 2 * b->cfg.MixedFixSweeps && err == hipSuccess; p++)
+                    err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 1);
+                if (err != hipSuccess) return err;
+                hipLaunchKernelGGL((kb_from_split<float, true>), gp, blk, 0, b->stream, c, 1);
+                hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 1);
+                hipLaunchKernelGGL(kb_res_final, gp, dim3(256), 0, b->stream, c, 1);
+            }
+        }
+    }
+    if ((err = hipGetLastError()) != hipSuccess) return err;
+    hipLaunchKernelGGL(kb_pbc, gp, dim3(256), 0, b->stream, c);
+    return hipGetLastError();
+}
+
+// project, project.go:12-34.
+hipError_t enq_project(const af_batch *b, int sw)
+{
+    hipError_t err = enq_vbc(b, sw);
+    if (err == hipSuccess) err = enq_div(b, sw);
+    if (err == hipSuccess) err = enq_solve(b);
+    if (err == hipSuccess) err = enq_grad(b, sw);
+    if (err == hipSuccess) err = enq_vbc(b, sw);
+    return err;
+}
+
+// Step, aeroflux.go:348-409, for every slot.  sw tracks the commitVelocity
+// swaps (0: velocity in U/V/W); the six swaps return it to 0.
+hipError_t enq_step(const af_batch *b)
+{
+    static const int fwd[3] = {AF_AXIS_X, AF_AXIS_Y, AF_AXIS_Z};
+    static const int rev[3] = {AF_AXIS_Z, AF_AXIS_Y, AF_AXIS_X};
+    hipError_t err = hipSuccess;
+    int sw = 0;
+    for (int k = 0; k < 3 && err == hipSuccess; k++) {
+        err = enq_vbc(b, sw);
+        if (err == hipSuccess) err = enq_sweep(b, b->shape, b->nwg_sweep, fwd[k], sw);
+        sw ^= 1;
+    }
+    if (err == hipSuccess) err = enq_project(b, sw);
+    for (int k = 0; k < 3 && err == hipSuccess; k++) {
+        err = enq_vbc(b, sw);
+        if (err == hipSuccess) err = enq_sweep(b, b->shape, b->nwg_sweep, rev[k], sw);
+        sw ^= 1;
+    }
+    if (err == hipSuccess) err = enq_project(b, sw);
+    return err;
+}
+
+void drop_graph(af_batch *b)
+{
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+}
+
+int get_graph(af_batch *b)
+{
+    if (b->graph) return AF_OK;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch step capture");
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
+            "begin batch step capture");
+    hipError_t err = enq_step(b);
+    hipError_t err2 = hipStreamEndCapture(b->stream, &g);
+    if (err != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(err, "capture batch step");
+    }
+    if (err2 != hipSuccess) return hip_fail(err2, "end batch step capture");
+    hipGraphExec_t x = nullptr;
+    hipError_t err3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (err3 != hipSuccess) return hip_fail(err3, "instantiate batch step graph");
+    b->graph = x;
+    return AF_OK;
+}
+
+// Times `reps` launches of `one` (after one warm-up) with events.
+template <typename F>
+int time_launches(af_batch *b, int reps, F one, double *ms_out)
+{
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0), "create event");
+    HIP_TRY(hipEventCreate(&e1), "create event");
+    hipError_t err = one();
+    if (err == hipSuccess) err = hipEventRecord(e0, b->stream);
+    for (int r = 0; r < reps && err == hipSuccess; r++) err = one();
+    if (err == hipSuccess) err = hipEventRecord(e1, b->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return hip_fail(err, "time batched stage");
+    *ms_out = (double)ms / reps;
+    return AF_OK;
+}
+
+hipError_t enq_timed_pass(const af_batch *b, int nwg)
+{
+    if (b->cfg.Precision == AF_PRECISION_MIXED) return enq_pass<float>(b, nwg, 0, 0, 0);
+    return enq_pass<double>(b, nwg, 0, 0, 0);
+}
+
+// Autotuner: advection workgroup shape, then persistent workgroup counts for
+// the sweeps and the colour passes, each timed on batched launches over the
+// resident slots.  The X sweep writes to the Tmp fields what the next step's
+// first sweep writes; colour passes touch only solve buffers that every
+// solve rewrites before reading.  Results therefore do not depend on tuning.
+int tune(af_batch *b)
+{
+    if (b->nact == 0) return AF_OK;
+    static const int kWg[] = {1024, 2048, 4096, 8192};
+    const int reps = 3;
+    int best_shape = b->cfg.SweepShape > 0 ? b->cfg.SweepShape - 1 : b->shape;
+    const int wg0 = b->cfg.Workgroups > 0 ? b->cfg.Workgroups : 2048;
+    if (b->cfg.SweepShape <= 0) {
+        double best = 1e30;
+        for (int s = 0; s < 8; s++) {
+            if (int rc = upload_table(b, s)) return rc;
+            double ms;
+            if (int rc = time_launches(b, reps, [&] { return enq_sweep(b, s, wg0, 0, 0); }, &ms))
+                return rc;
+            if (ms < best) {
+                best = ms;
+                best_shape = s;
+            }
+        }
+    }
+    b->shape = best_shape;
+    if (int rc = upload_table(b, b->shape)) return rc;
+    b->nwg_sweep = b->nwg_pass = b->nwg_other = wg0;
+    if (b->cfg.Workgroups <= 0) {
+        double bs = 1e30, bp = 1e30;
+        for (int w : kWg) {
+            double ms;
+            if (int rc = time_launches(b, reps, [&] { return enq_sweep(b, b->shape, w, 0, 0); },
+                                       &ms))
+                return rc;
+            if (ms < bs) {
+                bs = ms;
+                b->nwg_sweep = w;
+            }
+            if (int rc = time_launches(b, reps, [&] { return enq_timed_pass(b, w); }, &ms))
+                return rc;
+            if (ms < bp) {
+                bp = ms;
+                b->nwg_pass = w;
+            }
+        }
+        b->nwg_other = b->nwg_pass;
+    }
+    drop_graph(b);
+    b->tuned = true;
+    return sync(b, "tune batch");
+}
+
+void release(af_batch *b)
+{
+    if (b->closed) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    if (b->pool.base) (void)hipFree(b->pool.base);
+    if (b->d_slots) (void)hipFree(b->d_slots);
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->d_ctl) (void)hipFree(b->d_ctl);
+    if (b->d_last) (void)hipFree(b->d_last);
+    if (b->h_slots) (void)hipHostFree(b->h_slots);
+    if (b->h_tab) (void)hipHostFree(b->h_tab);
+    if (b->h_ints) (void)hipHostFree(b->h_ints);
+    if (b->h_dbl) (void)hipHostFree(b->h_dbl);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->pool = Pool();
+    b->d_slots = b->h_slots = nullptr;
+    b->d_tab = b->h_tab = nullptr;
+    b->d_ctl = nullptr;
+    b->d_last = nullptr;
+    b->h_ints = nullptr;
+    b->h_dbl = nullptr;
+    b->stream = nullptr;
+    b->slots.clear();
+    b->nact = 0;
+    b->cells = 0;
+    b->closed = true;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int64_t af_batch_slot_bytes(const af_config *cfg, int32_t precision)
+{
+    if (!cfg) return 0;
+    af_config c = *cfg;
+    if (af_config_normalize(&c) != AF_OK) return 0;
+    SlotLayout L;
+    if (!slot_layout(c, precision, &L)) return 0;
+    return L.total;
+}
+
+int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
+{
+    if (!cfg_in || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *out = nullptr;
+    af_batch_config c = *cfg_in;
+    if (c.Precision != AF_PRECISION_FP64 && c.Precision != AF_PRECISION_MIXED)
+        return fail(AF_ERR_CONFIG, "aeroflux: unknown precision mode %d", (int)c.Precision);
+    if (c.MaxSlots == 0) c.MaxSlots = 1024;
+    if (c.MemoryBudget == 0) c.MemoryBudget = (int64_t)1 << 30;
+    if (c.GaussSeidelIters == 0) c.GaussSeidelIters = 24;
+    if (c.MixedMaxCycles == 0) c.MixedMaxCycles = 4;
+    if (c.MixedSweeps == 0) c.MixedSweeps = (int32_t)c.GaussSeidelIters;
+    if (c.MixedFixSweeps == 0) c.MixedFixSweeps = 2;
+    if (c.MaxSlots < 1 || c.MaxSlots > (1 << 20) || c.MemoryBudget < 0 ||
+        c.GaussSeidelIters < 1 || c.MixedMaxCycles < 1 || c.MixedSweeps < 1 ||
+        c.MixedFixSweeps < 1)
+        return fail(AF_ERR_CONFIG, "aeroflux: bad batch capacity, budget or solve schedule");
+    if (c.SweepShape < 0 || c.SweepShape > 8)
+        return fail(AF_ERR_CONFIG, "aeroflux: sweep shape %d out of range 1..8",
+                    (int)c.SweepShape);
+    if (c.Workgroups < 0 || c.Workgroups % XCDS != 0)
+        return fail(AF_ERR_CONFIG, "aeroflux: persistent workgroups must be a multiple of 8");
+
+    af_batch *b = new (std::nothrow) af_batch();
+    if (!b) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
+    b->cfg = c;
+    b->closed = false;
+    auto bail = [&](hipError_t er, const char *what, int code) {
+        int rc = code == AF_ERR_OOM ? fail(AF_ERR_OOM, "aeroflux: %s: %s", what,
+                                           hipGetErrorString(er))
+                                    : hip_fail(er, what);
+        release(b);
+        delete b;
+        return rc;
+    };
+    hipError_t err = hipGetDevice(&b->device);
+    if (err != hipSuccess) return bail(err, "no HIP device", AF_ERR_HIP);
+    const int m = c.MaxSlots;
+    b->slots.resize(m);
+    b->tab_ints = 2 + (int64_t)(m + 1) * (1 + K_COUNT);
+    if (c.SweepShape > 0) b->shape = c.SweepShape - 1;
+    if (c.Workgroups > 0) b->nwg_sweep = b->nwg_pass = b->nwg_other = c.Workgroups;
+    if ((err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(err, "create stream", AF_ERR_HIP);
+    if (c.MemoryBudget > 0) {
+        if ((err = hipMalloc(&b->pool.base, (size_t)c.MemoryBudget)) != hipSuccess)
+            return bail(err, "allocate batch pool", AF_ERR_OOM);
+        b->pool.cap = c.MemoryBudget;
+        b->pool.free_[0] = c.MemoryBudget;
+    }
+    if ((err = hipMalloc(&b->d_slots, sizeof(BSlot) * (size_t)m)) != hipSuccess ||
+        (err = hipMalloc(&b->d_tab, sizeof(int) * (size_t)b->tab_ints)) != hipSuccess ||
+        (err = hipMalloc(&b->d_ctl, sizeof(int) * 3 * (size_t)m)) != hipSuccess ||
+        (err = hipMalloc(&b->d_last, sizeof(double) * (size_t)m)) != hipSuccess)
+        return bail(err, "allocate batch tables", AF_ERR_OOM);
+    if ((err = hipHostMalloc(&b->h_slots, sizeof(BSlot) * (size_t)m, hipHostMallocDefault)) !=
+            hipSuccess ||
+        (err = hipHostMalloc(&b->h_tab, sizeof(int) * (size_t)b->tab_ints,
+                             hipHostMallocDefault)) != hipSuccess ||
+        (err = hipHostMalloc(&b->h_ints, sizeof(int) * (size_t)m, hipHostMallocDefault)) !=
+            hipSuccess ||
+        (err = hipHostMalloc(&b->h_dbl, sizeof(double) * (size_t)m, hipHostMallocDefault)) !=
+            hipSuccess)
+        return bail(err, "allocate pinned batch tables", AF_ERR_OOM);
+    const int nctl = 3 * m;
+    hipLaunchKernelGGL(kb_fill_int, dim3((nctl + 255) / 256), dim3(256), 0, b->stream, b->d_ctl,
+                       nctl, 0);
+    if ((err = hipGetLastError()) != hipSuccess) return bail(err, "clear batch flags", AF_ERR_HIP);
+    if ((err = launch_zero(b->d_last, m, b->stream)) != hipSuccess)
+        return bail(err, "clear batch stats", AF_ERR_HIP);
+    if (int rc = build_table(b, b->shape)) {
+        release(b);
+        delete b;
+        return rc;
+    }
+    b->tab_dirty = true;
+    if ((err = hipStreamSynchronize(b->stream)) != hipSuccess)
+        return bail(err, "initialise batch", AF_ERR_HIP);
+    *out = b;
+    return AF_OK;
+}
+
+int af_batch_close(af_batch *b)
+{
+    if (!b) return fail(AF_ERR_ARG, "aeroflux: null batch");
+    release(b);
+    return AF_OK;
+}
+
+void af_batch_free(af_batch *b)
+{
+    if (!b) return;
+    release(b);
+    delete b;
+}
+
+int af_batch_get_info(const af_batch *b, af_batch_info *out)
+{
+    if (!b || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    af_batch_info i{};
+    for (const HSlot &h : b->slots) i.Slots += h.used ? 1 : 0;
+    i.SweepShape = b->shape;
+    i.SweepWorkgroups = b->nwg_sweep;
+    i.PassWorkgroups = b->nwg_pass;
+    i.OtherWorkgroups = b->nwg_other;
+    i.Tuned = b->tuned ? 1 : 0;
+    i.Cells = b->cells;
+    i.BytesInUse = b->pool.used;
+    i.BytesLargestFree = b->pool.largest();
+    i.MemoryBudget = b->cfg.MemoryBudget;
+    *out = i;
+    return AF_OK;
+}
+
+int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
+{
+    if (int rc = check_open(b)) return rc;
+    if (!cfg_in || !slot_out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    af_config c = *cfg_in;
+    if (int rc = af_config_normalize(&c)) return rc;
+    if (c.UseSparseSolver)
+        return fail(AF_ERR_UNSUPPORTED,
+                    "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
+                    "Red-Black iterative solver is used instead");
+    const af_batch_config &bc = b->cfg;
+    if (c.Precision != bc.Precision)
+        return fail(AF_ERR_CONFIG, "aeroflux: engine precision %d differs from the batch's %d",
+                    (int)c.Precision, (int)bc.Precision);
+    if (c.GaussSeidelIters != bc.GaussSeidelIters)
+        return fail(AF_ERR_CONFIG,
+                    "aeroflux: engine runs %ld sweep pairs, the batch's schedule %ld",
+                    (long)c.GaussSeidelIters, (long)bc.GaussSeidelIters);
+    if (c.Precision == AF_PRECISION_MIXED) {
+        if (c.MixedSweeps != bc.MixedSweeps || c.MixedFixSweeps != bc.MixedFixSweeps ||
+            c.MixedMaxCycles != bc.MixedMaxCycles)
+            return fail(AF_ERR_CONFIG,
+                        "aeroflux: engine mixed-precision schedule differs from the batch's");
+        if (bc.MixedChecked ? !(c.MixedTarget > 0) : c.MixedTarget != 0)
+            return fail(AF_ERR_CONFIG,
+                        bc.MixedChecked
+                            ? "aeroflux: a checked mixed-precision batch needs MixedTarget > 0"
+                            : "aeroflux: a one-cycle mixed-precision batch needs MixedTarget == 0");
+    }
+    SlotLayout L;
+    if (!slot_layout(c, c.Precision, &L))
+        return fail(AF_ERR_CONFIG, "aeroflux: a %ldx%ldx%ld grid exceeds 2^31 elements per field",
+                    (long)c.NX, (long)c.NY, (long)c.NZ);
+    int slot = -1;
+    for (int s = 0; s < (int)b->slots.size(); s++)
+        if (!b->slots[s].used) {
+            slot = s;
+            break;
+        }
+    if (slot < 0) return fail(AF_ERR_OOM, "aeroflux: batch has no free slot");
+    int64_t off;
+    if (!b->pool.alloc(L.total, &off))
+        return fail(AF_ERR_OOM,
+                    "aeroflux: batch pool cannot place %ld bytes (%ld of %ld in use, largest "
+                    "free block %ld)",
+                    (long)L.total, (long)b->pool.used, (long)b->pool.cap,
+                    (long)b->pool.largest());
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    HSlot &h = b->slots[slot];
+    h.used = true;
+    h.cfg = c;
+    h.L = L;
+    h.off = off;
+    char *base = b->pool.base + off;
+    BSlot &d = b->h_slots[slot];
+    for (int i = 0; i < AF_FIELD_COUNT; i++) d.f[i] = slot_field(b, h, i);
+    d.G = slot_geom(h);
+    // The same expressions, in the same order, as the single engine.
+    d.dt_half = c.DT * 0.5;
+    d.invH = 1.0 / c.H;
+    d.lam_half = c.Viscosity * d.dt_half * d.invH * d.invH;
+    d.h2 = c.H * c.H;
+    d.div_scale = c.Density / (c.DT * c.H);
+    d.grad_scale = c.DT / (c.Density * c.H);
+    d.res_to_div = c.DT / (c.Density * d.h2);
+    d.target = c.MixedTarget;
+    d.s_red = base + L.split_off[0];
+    d.s_black = base + L.split_off[1];
+    d.d_red = base + L.split_off[2];
+    d.d_black = base + L.split_off[3];
+    d.partials = reinterpret_cast<double *>(base + L.partials_off);
+    d.out = reinterpret_cast<double *>(base + L.out_off);
+    b->cells += c.NX * c.NY * c.NZ;
+    b->tab_dirty = true;
+    auto undo = [&](hipError_t er, const char *what) {
+        h.used = false;
+        b->pool.release(off, L.total);
+        b->cells -= c.NX * c.NY * c.NZ;
+        return hip_fail(er, what);
+    };
+    hipError_t err = launch_zero(reinterpret_cast<double *>(base), L.total / 8, b->stream);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(b->d_slots + slot, b->h_slots + slot, sizeof(BSlot),
+                             hipMemcpyHostToDevice, b->stream);
+    if (err == hipSuccess) {
+        int *ctl3[3] = {b->d_ctl + slot, b->d_ctl + b->cfg.MaxSlots + slot,
+                        b->d_ctl + 2 * b->cfg.MaxSlots + slot};
+        for (int *p : ctl3) {
+            hipLaunchKernelGGL(kb_fill_int, dim3(1), dim3(256), 0, b->stream, p, 1, 0);
+        }
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = launch_zero(b->d_last + slot, 1, b->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
+    if (err != hipSuccess) return undo(err, "add engine to batch");
+    *slot_out = slot;
+    return AF_OK;
+}
+
+int af_batch_remove(af_batch *b, int32_t slot)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    HIP_TRY(hipStreamSynchronize(b->stream), "remove engine from batch");
+    HSlot &h = b->slots[slot];
+    b->pool.release(h.off, h.L.total);
+    b->cells -= h.cfg.NX * h.cfg.NY * h.cfg.NZ;
+    h.used = false;
+    b->tab_dirty = true;
+    return AF_OK;
+}
+
+int af_batch_tune(af_batch *b)
+{
+    if (int rc = check_open(b)) return rc;
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    if (int rc = upload_table(b, b->shape)) return rc;
+    return tune(b);
+}
+
+int af_batch_step_n(af_batch *b, int64_t n, int32_t *faulted, int32_t cap, int32_t *nfaulted)
+{
+    if (int rc = check_open(b)) return rc;
+    if (n < 0) return fail(AF_ERR_ARG, "aeroflux: negative step count");
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    if (nfaulted) *nfaulted = 0;
+    if (int rc = upload_table(b, b->shape)) return rc;
+    if (!b->tuned && b->nact > 0) {
+        if (int rc = tune(b)) return rc;
+        if (int rc = upload_table(b, b->shape)) return rc;
+    }
+    if (int rc = get_graph(b)) return rc;
+    const int m = b->cfg.MaxSlots;
+    hipLaunchKernelGGL(kb_fill_int, dim3((m + 255) / 256), dim3(256), 0, b->stream, b->d_ctl, m,
+                       0);
+    HIP_TRY(hipGetLastError(), "clear fault flags");
+    for (int64_t k = 0; k < n; k++) HIP_TRY(hipGraphLaunch(b->graph, b->stream), "launch batch step");
+    HIP_TRY(hipMemcpyAsync(b->h_ints, b->d_ctl, sizeof(int) * (size_t)m, hipMemcpyDeviceToHost,
+                           b->stream),
+            "read fault flags");
+    if (int rc = sync(b, "batch step")) return rc;
+    int nf = 0;
+    for (int s = 0; s < m; s++) {
+        if (!b->slots[s].used || !b->h_ints[s]) continue;
+        if (faulted && nf < cap) faulted[nf] = s;
+        nf++;
+    }
+    if (nfaulted) *nfaulted = nf;
+    if (nf)
+        return fail(AF_ERR_FAULT,
+                    "aeroflux: device fault contained: non-finite departure coordinate in an "
+                    "advection sweep (%d batch slots)",
+                    nf);
+    return AF_OK;
+}
+
+int af_batch_field_desc(const af_batch *b, int32_t slot, int id, af_field *out)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    if (!out || id < 0 || id >= AF_FIELD_COUNT)
+        return fail(AF_ERR_ARG, "aeroflux: bad field descriptor arguments");
+    *out = b->slots[slot].L.f[id];
+    return AF_OK;
+}
+
+static int batch_copy(af_batch *b, int32_t slot, int id, void *host, int64_t count, bool write)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    if (id < 0 || id >= AF_FIELD_COUNT || !host)
+        return fail(AF_ERR_ARG, "aeroflux: bad field copy arguments");
+    const HSlot &h = b->slots[slot];
+    const af_field &f = h.L.f[id];
+    if (count != f.Count)
+        return fail(AF_ERR_ARG, "aeroflux: field %d has %ld elements, got %ld", id, (long)f.Count,
+                    (long)count);
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    char *dev = b->pool.base + h.off + f.Off;
+    if (write)
+        HIP_TRY(hipMemcpyAsync(dev, host, (size_t)count * 8, hipMemcpyHostToDevice, b->stream),
+                "write field");
+    else
+        HIP_TRY(hipMemcpyAsync(host, dev, (size_t)count * 8, hipMemcpyDeviceToHost, b->stream),
+                "read field");
+    return sync(b, "field copy");
+}
+
+int af_batch_field_read(af_batch *b, int32_t slot, int id, double *host, int64_t count)
+{
+    return batch_copy(b, slot, id, host, count, false);
+}
+
+int af_batch_field_write(af_batch *b, int32_t slot, int id, const double *host, int64_t count)
+{
+    return batch_copy(b, slot, id, const_cast<double *>(host), count, true);
+}
+
+int af_batch_apply_velocity_bc(af_batch *b, int32_t slot)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    const HSlot &h = b->slots[slot];
+    HIP_TRY(launch_velocity_bc(slot_field(b, h, AF_FIELD_U), slot_field(b, h, AF_FIELD_V),
+                               slot_field(b, h, AF_FIELD_W), slot_geom(h), b->stream),
+            "velocity bc");
+    return sync(b, "velocity bc");
+}
+
+static int slot_reduce(af_batch *b, int32_t slot, bool ke, double *out)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    if (!out) return fail(AF_ERR_ARG, "aeroflux: null output");
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    const HSlot &h = b->slots[slot];
+    const BSlot &d = b->h_slots[slot];
+    const DField U = slot_field(b, h, AF_FIELD_U), V = slot_field(b, h, AF_FIELD_V),
+                 W = slot_field(b, h, AF_FIELD_W);
+    if (ke)
+        HIP_TRY(launch_kinetic_energy(U, V, W, d.G, d.partials, d.out, b->stream),
+                "kinetic energy");
+    else
+        HIP_TRY(launch_max_divergence(U, V, W, d.G, d.invH, d.partials, d.out, b->stream),
+                "max divergence");
+    HIP_TRY(hipMemcpyAsync(b->h_dbl, d.out, sizeof(double), hipMemcpyDeviceToHost, b->stream),
+            "copy reduction");
+    if (int rc = sync(b, "reduction")) return rc;
+    double v = b->h_dbl[0];
+    if (ke) {
+        const double vol = h.cfg.H * h.cfg.H * h.cfg.H;
+        v = 0.5 * v * vol;  // project.go:257
+    }
+    *out = v;
+    return AF_OK;
+}
+
+int af_batch_max_divergence(af_batch *b, int32_t slot, double *out)
+{
+    return slot_reduce(b, slot, false, out);
+}
+
+int af_batch_kinetic_energy(af_batch *b, int32_t slot, double *out)
+{
+    return slot_reduce(b, slot, true, out);
+}
+
+int af_batch_mixed_stats(af_batch *b, int32_t slot, double *last_div, int32_t *cycles)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    if (b->cfg.Precision != AF_PRECISION_MIXED)
+        return fail(AF_ERR_ARG, "aeroflux: batch is not in mixed precision");
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    HIP_TRY(hipMemcpyAsync(b->h_dbl, b->d_last + slot, sizeof(double), hipMemcpyDeviceToHost,
+                           b->stream),
+            "read mixed stats");
+    HIP_TRY(hipMemcpyAsync(b->h_ints, b->d_ctl + 2 * b->cfg.MaxSlots + slot, sizeof(int),
+                           hipMemcpyDeviceToHost, b->stream),
+            "read mixed stats");
+    if (int rc = sync(b, "mixed stats")) return rc;
+    if (last_div) *last_div = b->h_dbl[0];
+    if (cycles) *cycles = b->h_ints[0];
+    return AF_OK;
+}
+
+int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
+{
+    if (int rc = check_open(b)) return rc;
+    if (reps < 1 || !avg_ms || stage < 0 || stage > 3)
+        return fail(AF_ERR_ARG, "aeroflux: bad timing arguments");
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    if (int rc = upload_table(b, b->shape)) return rc;
+    return time_launches(
+        b, reps,
+        [&]() -> hipError_t {
+            switch (stage) {
+            case 0: return enq_timed_pass(b, b->nwg_pass);
+            case 1: return enq_sweep(b, b->shape, b->nwg_sweep, 0, 0);
+            case 2: return enq_div(b, 0);
+            default: return enq_grad(b, 0);
+            }
+        },
+        avg_ms);
+}
+
+}  // extern "C"

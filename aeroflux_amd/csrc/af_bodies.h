@@ -1,0 +1,534 @@
+// af_bodies.h -- per-thread bodies of the step's stencil kernels.
+//
+// Each body computes what one thread of one workgroup tile computes, given
+// the tile's coordinates.  The single-engine kernels (af_kernels.hip,
+// af_advect.hip, af_rbgs_split.hip) and the batched persistent kernels
+// (af_batch.hip) both call them, so a batched engine runs exactly the
+// arithmetic of a single one.  Every body reproduces the reference's IEEE
+// binary64 operation order; contraction is off for every file that includes
+// this header.
+#pragma once
+
+#include "af_internal.h"
+#include "af_rbgs_split.h"
+
+#pragma clang fp contract(off)
+
+namespace af {
+namespace body {
+
+constexpr int BX = 64;  // lanes along x
+
+__device__ __forceinline__ int idx(const DField &f, int x, int y, int z)
+{
+    return x + y * f.sy + z * f.sz;  // Field.Index, grid.go:24-26
+}
+
+// ------------------------------------------------------------ velocity BC
+//
+// applyVelocityBC (stencil.go:246-315) as one gather.  For a component whose
+// normal axis is n, the reference writes exactly the boundary cells of the
+// box  c_n in [g, g+N_n],  c_a in [g-1, g+N_a] (a != n):
+//   * wall faces c_n in {g, g+N_n} with interior tangential coordinates get 0;
+//   * every other written cell gets -(value of the cell one step inward along
+//     one ghost axis), applied axis by axis.
+// Unrolling that chain, the final value of any written cell is the value of
+// its "base" cell (ghost coordinates clamped inward) negated once per ghost
+// coordinate, where the base is +0 if it lies on a wall face and the
+// untouched interior value otherwise.  Negation is exact, so this equals the
+// serial loops bit for bit, signed zeros included (one reflection of a wall
+// zero gives -0, two give +0).  The base cells are never written by this
+// kernel, so all writes are independent.
+
+// Boundary of an Lx*Ly*Lz box, enumerated once per cell:
+//   family Z: z in {0, Lz-1}, all x, y          (2*Lx*Ly cells)
+//   family Y: y in {0, Ly-1}, 0 < z < Lz-1      (2*Lx*(Lz-2))
+//   family X: x in {0, Lx-1}, 0 < y,z < L-1     (2*(Ly-2)*(Lz-2))
+__host__ __device__ inline int64_t shell_count(int Lx, int Ly, int Lz)
+{
+    return 2LL * Lx * Ly + 2LL * Lx * (Lz - 2) + 2LL * (Ly - 2) * (Lz - 2);
+}
+
+__device__ __forceinline__ void shell_decode(int64_t k, int Lx, int Ly, int Lz, int &x, int &y,
+                                             int &z)
+{
+    const int64_t nz = 2LL * Lx * Ly;
+    if (k < nz) {
+        x = (int)(k % Lx);
+        const int64_t r = k / Lx;
+        y = (int)(r % Ly);
+        z = (r / Ly) == 0 ? 0 : Lz - 1;
+        return;
+    }
+    k -= nz;
+    const int64_t ny = 2LL * Lx * (Lz - 2);
+    if (k < ny) {
+        x = (int)(k % Lx);
+        const int64_t r = k / Lx;
+        z = 1 + (int)(r % (Lz - 2));
+        y = (r / (Lz - 2)) == 0 ? 0 : Ly - 1;
+        return;
+    }
+    k -= ny;
+    y = 1 + (int)(k % (Ly - 2));
+    const int64_t r = k / (Ly - 2);
+    z = 1 + (int)(r % (Lz - 2));
+    x = (r / (Lz - 2)) == 0 ? 0 : Lx - 1;
+}
+
+// Velocity-BC shell sizes of U, V, W (counts of cells written).
+__host__ __device__ inline void velocity_bc_counts(const Geom &G, int64_t &cu, int64_t &cv,
+                                                   int64_t &cw)
+{
+    cu = shell_count(G.NX + 1, G.NY + 2, G.NZ + 2);
+    cv = shell_count(G.NX + 2, G.NY + 1, G.NZ + 2);
+    cw = shell_count(G.NX + 2, G.NY + 2, G.NZ + 1);
+}
+
+// Writes the final BC value of shell cell k (k < cu+cv+cw).
+__device__ __forceinline__ void velocity_bc_cell(const DField &U, const DField &V, const DField &W,
+                                                 const Geom &G, int64_t k, int64_t cu, int64_t cv,
+                                                 int64_t cw)
+{
+    const int g = G.g;
+    int n;
+    DField f;
+    int Lx, Ly, Lz, ox, oy, oz;  // box extents and origin
+    if (k < cu) {
+        n = 0; f = U;
+        Lx = G.NX + 1; Ly = G.NY + 2; Lz = G.NZ + 2; ox = g; oy = g - 1; oz = g - 1;
+    } else if (k < cu + cv) {
+        k -= cu; n = 1; f = V;
+        Lx = G.NX + 2; Ly = G.NY + 1; Lz = G.NZ + 2; ox = g - 1; oy = g; oz = g - 1;
+    } else if (k < cu + cv + cw) {
+        k -= cu + cv; n = 2; f = W;
+        Lx = G.NX + 2; Ly = G.NY + 2; Lz = G.NZ + 1; ox = g - 1; oy = g - 1; oz = g;
+    } else {
+        return;
+    }
+    int bx, by, bz;
+    shell_decode(k, Lx, Ly, Lz, bx, by, bz);
+    int c[3] = {ox + bx, oy + by, oz + bz};
+    const int x = c[0], y = c[1], z = c[2];
+    const int N[3] = {G.NX, G.NY, G.NZ};
+    const bool wall = (c[n] == g) || (c[n] == g + N[n]);
+    int flips = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        if (a == n) continue;
+        if (c[a] == g - 1) {
+            c[a] = g;
+            flips++;
+        } else if (c[a] == g + N[a]) {
+            c[a] = g + N[a] - 1;
+            flips++;
+        }
+    }
+    double v = wall ? 0.0 : f.d[idx(f, c[0], c[1], c[2])];
+    if (flips & 1) v = -v;  // two exact negations are the identity
+    f.d[idx(f, x, y, z)] = v;
+}
+
+// -------------------------------------------------------------- advection
+//
+// advectComponent (stencil.go:44-119), written so that each face's loads can
+// be issued together: no branches inside a face (the departure clamp is a
+// select; a NaN coordinate only sets `bad`), and a thread forms every face
+// value before storing any.  Two exact specialisations of sampleTrilinear
+// (stencil.go:169-225):
+//   * only the velocity component along the sweep axis enters the
+//     backtrace (stencil.go:89-96), so faceVelocity's other two averages,
+//     which have no other use, are not formed;
+//   * on the two axes the backtrace does not move, the coordinate is the
+//     face's own index, which lies in [0, extent-2] for every face a sweep
+//     writes (x in [g, g+NX] of the NX+2g+1-wide U field, and likewise), so
+//     the reference's clamp and x0 >= extent-1 adjustment never fire there:
+//     the tap base is the face index and the weight is exactly 0.  All eight
+//     taps are still loaded and combined in the reference's order (0*Inf
+//     and signed zeros depend on the zero-weight taps).
+
+// The velocity component along sweep axis AX at a face of component C, as
+// faceVelocity (stencil.go:128-155) forms it (same four-point order).
+template <int AX, int C>
+__device__ __forceinline__ double axis_velocity(const SweepArgs &a, int x, int y, int z)
+{
+    if (AX == C) {
+        const DField &f = C == 0 ? a.U : (C == 1 ? a.V : a.W);
+        return f.d[idx(f, x, y, z)];
+    }
+    if (AX == 0) {
+        const double *u = a.U.d;
+        if (C == 1)
+            return 0.25 * (u[idx(a.U, x, y - 1, z)] + u[idx(a.U, x + 1, y - 1, z)] +
+                           u[idx(a.U, x, y, z)] + u[idx(a.U, x + 1, y, z)]);
+        return 0.25 * (u[idx(a.U, x, y, z - 1)] + u[idx(a.U, x + 1, y, z - 1)] +
+                       u[idx(a.U, x, y, z)] + u[idx(a.U, x + 1, y, z)]);
+    }
+    if (AX == 1) {
+        const double *v = a.V.d;
+        if (C == 0)
+            return 0.25 * (v[idx(a.V, x - 1, y, z)] + v[idx(a.V, x, y, z)] +
+                           v[idx(a.V, x - 1, y + 1, z)] + v[idx(a.V, x, y + 1, z)]);
+        return 0.25 * (v[idx(a.V, x, y, z - 1)] + v[idx(a.V, x, y + 1, z - 1)] +
+                       v[idx(a.V, x, y, z)] + v[idx(a.V, x, y + 1, z)]);
+    }
+    const double *w = a.W.d;
+    if (C == 0)
+        return 0.25 * (w[idx(a.W, x - 1, y, z)] + w[idx(a.W, x, y, z)] +
+                       w[idx(a.W, x - 1, y, z + 1)] + w[idx(a.W, x, y, z + 1)]);
+    return 0.25 * (w[idx(a.W, x, y - 1, z)] + w[idx(a.W, x, y, z)] +
+                   w[idx(a.W, x, y - 1, z + 1)] + w[idx(a.W, x, y, z + 1)]);
+}
+
+// advectComponent's value at one face (stencil.go:83-115).  `bad` is set
+// when the departure coordinate is NaN -- the reference's bounds-panic
+// trigger; the value is then NaN and no load leaves the field.
+template <int AX, int C>
+__device__ __forceinline__ double advect_value(const SweepArgs &a, int x, int y, int z, bool &bad)
+{
+    const DField &s = C == 0 ? a.U : (C == 1 ? a.V : a.W);
+    const double v = axis_velocity<AX, C>(a, x, y, z);
+    const int coord = AX == 0 ? x : (AX == 1 ? y : z);
+    const int ext = AX == 0 ? s.nx : (AX == 1 ? s.ny : s.nz);
+    double p = (double)coord;
+    p -= a.dt * v * a.invH;
+    const double pmax = (double)(ext - 1);
+    p = p < 0 ? 0.0 : (p > pmax ? pmax : p);  // NaN fails both tests
+    const bool nan = p != p;
+    bad = bad || nan;
+    int i0 = nan ? 0 : (int)p;
+    i0 = i0 >= ext - 1 ? ext - 2 : i0;
+    const double t = p - (double)i0;
+
+    const int x0 = AX == 0 ? i0 : x;
+    const int y0 = AX == 1 ? i0 : y;
+    const int z0 = AX == 2 ? i0 : z;
+    const double tx = AX == 0 ? t : 0.0;
+    const double ty = AX == 1 ? t : 0.0;
+    const double tz = AX == 2 ? t : 0.0;
+    const double *d = s.d;
+    const int i000 = idx(s, x0, y0, z0);
+    const int i100 = i000 + 1;
+    const int i010 = i000 + s.sy;
+    const int i110 = i010 + 1;
+    const int i001 = i000 + s.sz;
+    const int i101 = i001 + 1;
+    const int i011 = i001 + s.sy;
+    const int i111 = i011 + 1;
+    const double c00 = d[i000] * (1 - tx) + d[i100] * tx;
+    const double c10 = d[i010] * (1 - tx) + d[i110] * tx;
+    const double c01 = d[i001] * (1 - tx) + d[i101] * tx;
+    const double c11 = d[i011] * (1 - tx) + d[i111] * tx;
+    const double c0 = c00 * (1 - ty) + c10 * ty;
+    const double c1 = c01 * (1 - ty) + c11 * ty;
+    const double adv = c0 * (1 - tz) + c1 * tz;
+
+    const int i = idx(s, x, y, z);
+    const int st = AX == 0 ? 1 : (AX == 1 ? s.sy : s.sz);
+    double lap = a.lam * (d[i - st] - 2 * d[i] + d[i + st]);
+    lap = a.lam != 0 ? lap : 0.0;  // `var lap float64` stays +0 when lam == 0
+    return adv + lap;
+}
+
+// One thread of advectDiffuseAxis (stencil.go:21-25): the U, V and W faces
+// at (x, y, z0 .. z0+ZR-1) that exist (x <= g+NX, y <= g+NY, z <= g+NZ).
+template <int AX, int ZR>
+__device__ __forceinline__ void sweep_column(const SweepArgs &a, int x, int y, int z0)
+{
+    const int g = a.G.g;
+    if (x > g + a.G.NX || y > g + a.G.NY) return;
+    const bool xin = x < g + a.G.NX, yin = y < g + a.G.NY;
+    bool bad = false;
+    double vu[ZR], vv[ZR], vw[ZR];
+    bool du[ZR], dv[ZR], dw[ZR];
+#pragma unroll
+    for (int r = 0; r < ZR; r++) {
+        const int z = z0 + r;
+        const bool zok = z <= g + a.G.NZ, zin = z < g + a.G.NZ;
+        du[r] = yin && zin;
+        dv[r] = xin && zin;
+        dw[r] = xin && yin && zok;
+        vu[r] = vv[r] = vw[r] = 0;
+        if (du[r]) vu[r] = advect_value<AX, 0>(a, x, y, z, bad);
+        if (dv[r]) vv[r] = advect_value<AX, 1>(a, x, y, z, bad);
+        if (dw[r]) vw[r] = advect_value<AX, 2>(a, x, y, z, bad);
+    }
+#pragma unroll
+    for (int r = 0; r < ZR; r++) {
+        const int z = z0 + r;
+        if (du[r]) a.T0.d[idx(a.T0, x, y, z)] = vu[r];
+        if (dv[r]) a.T1.d[idx(a.T1, x, y, z)] = vv[r];
+        if (dw[r]) a.T2.d[idx(a.T2, x, y, z)] = vw[r];
+    }
+    if (bad) *a.flag = 1;
+}
+
+// ------------------------------------------------------------- projection
+
+// computeDivergence, project.go:42-62, interior cell (x, y, z).  Returns
+// false (writing nothing) outside the interior; else stores and returns the
+// value in `out`.
+__device__ __forceinline__ bool divergence_cell(const DField &U, const DField &V, const DField &W,
+                                                const DField &D, const Geom &G, double scale,
+                                                int x, int y, int z, double &out)
+{
+    const int g = G.g;
+    if (x >= g + G.NX || y >= g + G.NY) return false;
+    const double du = U.d[idx(U, x + 1, y, z)] - U.d[idx(U, x, y, z)];
+    const double dv = V.d[idx(V, x, y + 1, z)] - V.d[idx(V, x, y, z)];
+    const double dw = W.d[idx(W, x, y, z + 1)] - W.d[idx(W, x, y, z)];
+    out = scale * (du + dv + dw);
+    D.d[idx(D, x, y, z)] = out;
+    return true;
+}
+
+// Size of P's ghost shell of thickness g.
+__host__ __device__ inline int64_t pressure_shell_count(int Lx, int Ly, int Lz, int t)
+{
+    return 2LL * t * Lx * Ly + 2LL * t * Lx * (Lz - 2 * t) + 2LL * t * (Ly - 2 * t) * (Lz - 2 * t);
+}
+
+// applyPressureBC, project.go:139-163, as one gather over the whole ghost
+// shell, cell k.  The three serial loops copy each first-layer ghost from the
+// cell with its coordinates clamped into the interior (the edge and corner
+// copies read ghosts written by the earlier loops, which already hold those
+// clamped values).  Cells beyond the first layer (ghost > 1) are never
+// written by the reference after the solve zeroes P; zero_outer reproduces
+// that zero.
+__device__ __forceinline__ void pressure_bc_cell(const DField &P, const Geom &G, int64_t k,
+                                                 bool zero_outer)
+{
+    const int g = G.g;
+    int x, y, z;
+    const int Lx = P.nx, Ly = P.ny, Lz = P.nz, t = g;
+    int64_t r = k;
+    const int64_t fz = 2LL * t * Lx * Ly;
+    if (r < fz) {
+        x = (int)(r % Lx); r /= Lx;
+        y = (int)(r % Ly); r /= Ly;
+        const int s = (int)r;
+        z = s < t ? s : Lz - 2 * t + s;
+    } else {
+        r -= fz;
+        const int64_t fy = 2LL * t * Lx * (Lz - 2 * t);
+        if (r < fy) {
+            x = (int)(r % Lx); r /= Lx;
+            z = t + (int)(r % (Lz - 2 * t)); r /= (Lz - 2 * t);
+            const int s = (int)r;
+            y = s < t ? s : Ly - 2 * t + s;
+        } else {
+            r -= fy;
+            y = t + (int)(r % (Ly - 2 * t)); r /= (Ly - 2 * t);
+            z = t + (int)(r % (Lz - 2 * t)); r /= (Lz - 2 * t);
+            const int s = (int)r;
+            x = s < t ? s : Lx - 2 * t + s;
+        }
+    }
+    const bool layer1 = x >= g - 1 && x <= g + G.NX && y >= g - 1 && y <= g + G.NY &&
+                        z >= g - 1 && z <= g + G.NZ;
+    double v = 0;
+    if (layer1) {
+        const int cx = x < g ? g : (x >= g + G.NX ? g + G.NX - 1 : x);
+        const int cy = y < g ? g : (y >= g + G.NY ? g + G.NY - 1 : y);
+        const int cz = z < g ? g : (z >= g + G.NZ ? g + G.NZ - 1 : z);
+        v = P.d[idx(P, cx, cy, cz)];
+    } else if (!zero_outer) {
+        return;
+    }
+    P.d[idx(P, x, y, z)] = v;
+}
+
+// subtractGradient, project.go:172-205.  The thread of interior column
+// (x, y) at z updates U, V (z < g+NZ) and W; the last column, row and z
+// position also update the wall faces U(g+NX), V(g+NY), W(g+NZ).
+__device__ __forceinline__ void gradient_cell(const DField &U, const DField &V, const DField &W,
+                                              const DField &P, const Geom &G, double scale, int x,
+                                              int y, int z)
+{
+    const int g = G.g;
+    if (x >= g + G.NX || y >= g + G.NY) return;
+    const double *p = P.d;
+    const int ip = idx(P, x, y, z);
+    if (z < g + G.NZ) {
+        const int iu = idx(U, x, y, z);
+        U.d[iu] -= scale * (p[ip] - p[ip - 1]);
+        if (x == g + G.NX - 1) U.d[iu + 1] -= scale * (p[ip + 1] - p[ip]);
+        const int iv = idx(V, x, y, z);
+        V.d[iv] -= scale * (p[ip] - p[ip - P.sy]);
+        if (y == g + G.NY - 1) V.d[iv + V.sy] -= scale * (p[ip + P.sy] - p[ip]);
+    }
+    const int iw = idx(W, x, y, z);
+    W.d[iw] -= scale * (p[ip] - p[ip - P.sz]);
+}
+
+// ---------------------------------------------------- colour-split layout
+//
+// A colour pass of gaussSeidelSweep (project.go:103-130) writes every cell of
+// one colour from its six neighbours -- all of the other colour -- and its
+// divergence; it never reads the cell's own old value (except where the
+// Neumann mirror substitutes it for a ghost neighbour).  The split layout
+// stores interior cells only: row (y', z') of colour c holds the cells
+// x' = x0 + 2k (x0 = first x' of colour c in that row) at index
+// ((z'*NY + y') * NXH + k), NXH = ceil(NX/2).  For a cell of colour c at x',
+// its x-neighbours are entries (x'-1)>>1 and (x'+1)>>1 of the other-colour
+// row, and its y/z neighbours are entry x'>>1 of the adjacent other-colour
+// rows -- the same k.
+
+template <typename T>
+__device__ __forceinline__ T relax(T sum, T rhs, double h2);
+
+template <>
+__device__ __forceinline__ double relax<double>(double sum, double rhs, double h2)
+{
+    return (sum - h2 * rhs) / 6.0;
+}
+
+template <>
+__device__ __forceinline__ float relax<float>(float sum, float rhs, double)
+{
+    return (sum + rhs) * (1.0f / 6.0f);
+}
+
+// Parity of interior cell (x', y', z') in padded coordinates: red = 0.
+__device__ __forceinline__ int parity(int xi, int yi, int zi, int g)
+{
+    return (xi + yi + zi + 3 * g) & 1;
+}
+
+// One thread of a colour pass: cell k of rows (yi, zb .. zb+ZR-1).
+// MODE 0: normal pass.  MODE 1: first pass of a solve from zero (every
+// neighbour is +0).  MODE 2: second pass of a solve from zero (the cell's own
+// old value, used for mirrored ghosts, is +0).  All ZR results are formed
+// before any is stored.
+template <typename T, int MODE, int ZR>
+__device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int color, int k, int yi,
+                                                  int zb)
+{
+    const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
+    if (yi >= NY) return;
+    T *cur = color ? a.black : a.red;
+    const T *oth = color ? a.red : a.black;
+    const T *rhs = color ? a.rhs_black : a.rhs_red;
+    T out[ZR];
+    int at[ZR];
+#pragma unroll
+    for (int r = 0; r < ZR; r++) {
+        const int zi = zb + r;
+        const int x0 = (color + yi + zi + 3 * a.g) & 1;
+        const int xi = x0 + 2 * k;
+        at[r] = -1;
+        if (zi >= NZ || xi >= NX) continue;
+        const int row = zi * NY + yi;
+        const int i = row * NXH + k;
+        at[r] = i;
+        T sum;
+        if (MODE == 1) {
+            sum = T(0) + T(0) + T(0) + T(0) + T(0) + T(0);
+        } else {
+            const T c = MODE == 2 ? T(0) : ((xi == 0 || xi == NX - 1 || yi == 0 || yi == NY - 1 ||
+                                             zi == 0 || zi == NZ - 1)
+                                                ? cur[i]
+                                                : T(0));
+            const T xm = xi == 0 ? c : oth[row * NXH + ((xi - 1) >> 1)];
+            const T xp = xi == NX - 1 ? c : oth[row * NXH + ((xi + 1) >> 1)];
+            const T ym = yi == 0 ? c : oth[i - NXH];
+            const T yp = yi == NY - 1 ? c : oth[i + NXH];
+            const T zm = zi == 0 ? c : oth[i - NY * NXH];
+            const T zp = zi == NZ - 1 ? c : oth[i + NY * NXH];
+            sum = xm + xp + ym + yp + zm + zp;
+        }
+        out[r] = relax<T>(sum, rhs[i], a.h2);
+    }
+#pragma unroll
+    for (int r = 0; r < ZR; r++)
+        if (at[r] >= 0) cur[at[r]] = out[r];
+}
+
+// Interior cell (xi, yi, zi) of a padded natural-layout field -> split.
+template <typename TS, typename TN>
+__device__ __forceinline__ void to_split_cell(const TN *nat, int sy, int sz, int g, int NX, int NY,
+                                              int NXH, TS *red, TS *black, int xi, int yi, int zi)
+{
+    if (xi >= NX || yi >= NY) return;
+    const TN v = nat[(xi + g) + (yi + g) * sy + (zi + g) * sz];
+    const int i = (zi * NY + yi) * NXH + (xi >> 1);
+    if (parity(xi, yi, zi, g)) black[i] = (TS)v; else red[i] = (TS)v;
+}
+
+// FP32 right-hand side -(h2 * div) of the mixed-precision smoother, split.
+__device__ __forceinline__ void rhs32_cell(const double *div, int sy, int sz, int g, int NX, int NY,
+                                           int NXH, double h2, float *red, float *black, int xi,
+                                           int yi, int zi)
+{
+    if (xi >= NX || yi >= NY) return;
+    const float v = (float)(-(h2 * div[(xi + g) + (yi + g) * sy + (zi + g) * sz]));
+    const int i = (zi * NY + yi) * NXH + (xi >> 1);
+    if (parity(xi, yi, zi, g)) black[i] = v; else red[i] = v;
+}
+
+// Split -> interior cell of the FP64 natural-layout P.  ADD: P += split (the
+// correction of the mixed-precision solve); else P = split.
+template <typename TS, bool ADD>
+__device__ __forceinline__ void from_split_cell(double *nat, int sy, int sz, int g, int NX, int NY,
+                                                int NXH, const TS *red, const TS *black, int xi,
+                                                int yi, int zi)
+{
+    if (xi >= NX || yi >= NY) return;
+    const int i = (zi * NY + yi) * NXH + (xi >> 1);
+    const double v = (double)(parity(xi, yi, zi, g) ? black[i] : red[i]);
+    const int n = (xi + g) + (yi + g) * sy + (zi + g) * sz;
+    nat[n] = ADD ? nat[n] + v : v;
+}
+
+// FP64 residual of the mixed-precision solve along interior column (xi, yi):
+// r = (sum p~ - H^2 div) - 6 p with the Neumann mirror fused (p~ = p for a
+// ghost neighbour).  Stores r in FP32 on the split layout and returns the
+// column's max |r|.  `div` shares P's layout.
+__device__ __forceinline__ double residual_column(const DField &P, const double *div, float *r_red,
+                                                  float *r_black, const Geom &G, double h2, int xi,
+                                                  int yi)
+{
+    double m = 0;
+    if (xi >= G.NX || yi >= G.NY) return m;
+    const int g = G.g;
+    const int NXH = (G.NX + 1) / 2;
+    const double *p = P.d;
+    const int sy = P.sy, sz = P.sz;
+    const int x = xi + g, y = yi + g;
+    for (int zi = 0; zi < G.NZ; zi++) {
+        const int z = zi + g;
+        const int i = x + y * sy + z * sz;
+        const double c = p[i];
+        const double xm = (xi == 0) ? c : p[i - 1];
+        const double xp = (xi == G.NX - 1) ? c : p[i + 1];
+        const double ym = (yi == 0) ? c : p[i - sy];
+        const double yp = (yi == G.NY - 1) ? c : p[i + sy];
+        const double zm = (zi == 0) ? c : p[i - sz];
+        const double zp = (zi == G.NZ - 1) ? c : p[i + sz];
+        const double s = ((((xm + xp) + ym) + yp) + zm) + zp;
+        const double res = (s - h2 * div[i]) - 6.0 * c;
+        const int si = (zi * G.NY + yi) * NXH + (xi >> 1);
+        if ((x + y + z) & 1) r_black[si] = (float)res; else r_red[si] = (float)res;
+        const double a = fabs(res);
+        m = a > m ? a : m;
+    }
+    return m;
+}
+
+// Max over a 256-thread workgroup (threads numbered x-fastest), fixed tree.
+__device__ __forceinline__ double block_max256(double v, double *sh)
+{
+    const int t = threadIdx.y * blockDim.x + threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t + s] > sh[t] ? sh[t + s] : sh[t];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();  // sh is reused by the caller's next item
+    return r;
+}
+
+}  // namespace body
+}  // namespace af

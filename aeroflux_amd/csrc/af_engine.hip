@@ -18,6 +18,7 @@
 #include <unistd.h>
 
 #include "../../include/aeroflux.h"
+#include "af_host.h"
 #include "af_internal.h"
 #include "af_mixed.h"
 #include "af_rbgs_split.h"
@@ -27,11 +28,27 @@
 #include <mutex>
 #include <tuple>
 
-using namespace af;
+namespace af {
 
 namespace {
-
 thread_local std::string g_err;
+constexpr int kCacheLine = 128;  // layout.go:13
+
+af_field field_layout(int64_t *off, int64_t nx, int64_t ny, int64_t nz)
+{
+    af_field f;
+    f.Off = *off;
+    f.NX = nx; f.NY = ny; f.NZ = nz;
+    f.SY = nx; f.SZ = nx * ny;
+    f.Count = nx * ny * nz;
+    int64_t sz = f.Count * 8;
+    if (int64_t r = sz % kCacheLine) sz += kCacheLine - r;
+    *off += sz;
+    return f;
+}
+}  // namespace
+
+const char *last_error() { return g_err.c_str(); }
 
 int fail(int code, const char *fmt, ...)
 {
@@ -49,28 +66,6 @@ int hip_fail(hipError_t err, const char *what)
     return fail(AF_ERR_HIP, "aeroflux: %s: %s", what, hipGetErrorString(err));
 }
 
-#define HIP_TRY(expr, what)                         \
-    do {                                            \
-        hipError_t _e = (expr);                     \
-        if (_e != hipSuccess) return hip_fail(_e, what); \
-    } while (0)
-
-constexpr int kCacheLine = 128;  // layout.go:13
-
-af_field field_layout(int64_t *off, int64_t nx, int64_t ny, int64_t nz)
-{
-    af_field f;
-    f.Off = *off;
-    f.NX = nx; f.NY = ny; f.NZ = nz;
-    f.SY = nx; f.SZ = nx * ny;
-    f.Count = nx * ny * nz;
-    int64_t sz = f.Count * 8;
-    if (int64_t r = sz % kCacheLine) sz += kCacheLine - r;
-    *off += sz;
-    return f;
-}
-
-// NewMAC field order and padding, grid.go:101-134.
 int64_t mac_layout(af_field *f, int64_t nx, int64_t ny, int64_t nz, int64_t g)
 {
     const int64_t g2 = 2 * g, cx = nx + g2, cy = ny + g2, cz = nz + g2;
@@ -85,6 +80,12 @@ int64_t mac_layout(af_field *f, int64_t nx, int64_t ny, int64_t nz, int64_t g)
     f[AF_FIELD_TMP2] = field_layout(&off, cx, cy, cz + 1);
     return off;
 }
+
+}  // namespace af
+
+using namespace af;
+
+namespace {
 
 // Current descriptors of the six velocity-sized fields, as the step walks
 // through its commitVelocity swaps.
@@ -576,7 +577,7 @@ int choose_schedule(af_engine *e)
 
 extern "C" {
 
-const char *af_last_error(void) { return g_err.c_str(); }
+const char *af_last_error(void) { return af::last_error(); }
 
 int af_config_normalize(af_config *c)
 {
@@ -739,11 +740,10 @@ int af_new(const af_config *cfg_in, af_engine **out)
     int rc = choose_schedule(e);
     if (rc == AF_OK) rc = choose_sweep_shape(e);
     if (rc != AF_OK) {
-        std::string msg = g_err;
+        const std::string msg = af::last_error();
         release(e);
         delete e;
-        g_err = msg;
-        return rc;
+        return fail(rc, "%s", msg.c_str());
     }
     // Tuning launches wrote only pbuf / the FP32 buffers; restore their zeros.
     if (e->pbuf &&

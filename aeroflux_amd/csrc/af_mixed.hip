@@ -15,8 +15,8 @@
 // max is the post-projection max|div|: cycles stop once it reaches the
 // target.  Cycles after the target is met run as empty launches (a device
 // flag), which keeps the whole solve capturable in one graph.
+#include "af_bodies.h"
 #include "af_mixed.h"
-#include "af_rbgs_split.h"
 
 #pragma clang fp contract(off)
 
@@ -26,18 +26,6 @@ namespace {
 constexpr int BX = 64;
 constexpr int BY = 4;
 constexpr int RB = 256;
-
-__device__ __forceinline__ double block_max(double v, double *sh)
-{
-    const int t = threadIdx.y * BX + threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int s = RB / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] = sh[t + s] > sh[t] ? sh[t + s] : sh[t];
-        __syncthreads();
-    }
-    return sh[0];
-}
 
 // FP64 residual r = (sum p~ - H^2 div) - 6 p with the Neumann mirror fused;
 // stores r in FP32 (split layout) for the next cycle and a per-block max|r|.
@@ -49,34 +37,10 @@ __global__ __launch_bounds__(BX *BY) void k_residual(DField P, const double *__r
 {
     __shared__ double sh[RB];
     if (check_done && *done) return;  // uniform across the block
-    const int g = G.g;
-    const int xi = blockIdx.x * BX + threadIdx.x;
-    const int yi = blockIdx.y * BY + threadIdx.y;
-    const int NXH = (G.NX + 1) / 2;
-    double m = 0;
-    if (xi < G.NX && yi < G.NY) {
-        const double *p = P.d;
-        const int sy = P.sy, sz = P.sz;
-        const int x = xi + g, y = yi + g;
-        for (int zi = 0; zi < G.NZ; zi++) {
-            const int z = zi + g;
-            const int i = x + y * sy + z * sz;
-            const double c = p[i];
-            const double xm = (xi == 0) ? c : p[i - 1];
-            const double xp = (xi == G.NX - 1) ? c : p[i + 1];
-            const double ym = (yi == 0) ? c : p[i - sy];
-            const double yp = (yi == G.NY - 1) ? c : p[i + sy];
-            const double zm = (zi == 0) ? c : p[i - sz];
-            const double zp = (zi == G.NZ - 1) ? c : p[i + sz];
-            const double s = ((((xm + xp) + ym) + yp) + zm) + zp;
-            const double res = (s - h2 * div[i]) - 6.0 * c;
-            const int si = (zi * G.NY + yi) * NXH + (xi >> 1);
-            if ((x + y + z) & 1) r_black[si] = (float)res; else r_red[si] = (float)res;
-            const double a = fabs(res);
-            m = a > m ? a : m;
-        }
-    }
-    const double b = block_max(m, sh);
+    const double m = body::residual_column(P, div, r_red, r_black, G, h2,
+                                           blockIdx.x * BX + threadIdx.x,
+                                           blockIdx.y * BY + threadIdx.y);
+    const double b = body::block_max256(m, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = b;
 }
 

@@ -8,15 +8,9 @@
 // (read other colour + read divergence + write this colour) = 12 B per cell
 // in FP64 instead of the 24 B a pass over the interleaved arena needs, with
 // every access unit-stride.  The arithmetic and its order are the
-// reference's, so the results are bit-identical.
-//
-// Split layout: interior cells only, row (y', z') of colour c holds the
-// cells x' = x0 + 2k (x0 = first x' of colour c in that row) at index
-// ((z'*NY + y') * NXH + k), NXH = ceil(NX/2).  For a cell of colour c at
-// x', its x-neighbours are entries (x'-1)>>1 and (x'+1)>>1 of the
-// other-colour row, and its y/z neighbours are entry x'>>1 of the adjacent
-// other-colour rows -- the same k.
-#include "af_rbgs_split.h"
+// reference's, so the results are bit-identical.  The per-thread bodies and
+// the layout's index rules are in af_bodies.h.
+#include "af_bodies.h"
 #include "af_xcd.h"
 
 #pragma clang fp contract(off)
@@ -24,84 +18,21 @@
 namespace af {
 namespace {
 
-constexpr int BX = 64;
+constexpr int BX = body::BX;
 constexpr int BY = 4;
-
-template <typename T>
-__device__ __forceinline__ T relax_s(T sum, T rhs, double h2);
-
-template <>
-__device__ __forceinline__ double relax_s<double>(double sum, double rhs, double h2)
-{
-    return (sum - h2 * rhs) / 6.0;
-}
-
-template <>
-__device__ __forceinline__ float relax_s<float>(float sum, float rhs, double)
-{
-    return (sum + rhs) * (1.0f / 6.0f);
-}
-
-// Parity of interior cell (x', y', z') in padded coordinates: red = 0.
-__device__ __forceinline__ int parity(int xi, int yi, int zi, int g)
-{
-    return (xi + yi + zi + 3 * g) & 1;
-}
 
 // Planes per thread: a pass over one plane per thread leaves each wave too
 // little work to keep the CUs full, so every thread updates ZR planes.
 constexpr int ZR = 4;
 
-// MODE 0: normal pass.  MODE 1: first pass of a solve from zero (every
-// neighbour is +0).  MODE 2: second pass of a solve from zero (the cell's own
-// old value, used for mirrored ghosts, is +0).  All ZR results are formed
-// before any is stored, so every load can be issued up front.
 template <typename T, int MODE>
 __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color, int3 nb)
 {
     if (a.done && *a.done) return;
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int k = bx * BX + threadIdx.x;
-    const int yi = by * BY + threadIdx.y;
-    const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
-    if (yi >= NY) return;
-    T *cur = color ? a.black : a.red;
-    const T *oth = color ? a.red : a.black;
-    const T *rhs = color ? a.rhs_black : a.rhs_red;
-    T out[ZR];
-    int at[ZR];
-#pragma unroll
-    for (int r = 0; r < ZR; r++) {
-        const int zi = bz * ZR + r;
-        const int x0 = (color + yi + zi + 3 * a.g) & 1;
-        const int xi = x0 + 2 * k;
-        at[r] = -1;
-        if (zi >= NZ || xi >= NX) continue;
-        const int row = zi * NY + yi;
-        const int i = row * NXH + k;
-        at[r] = i;
-        T sum;
-        if (MODE == 1) {
-            sum = T(0) + T(0) + T(0) + T(0) + T(0) + T(0);
-        } else {
-            const T c = MODE == 2 ? T(0) : ((xi == 0 || xi == NX - 1 || yi == 0 || yi == NY - 1 ||
-                                             zi == 0 || zi == NZ - 1)
-                                                ? cur[i]
-                                                : T(0));
-            const T xm = xi == 0 ? c : oth[row * NXH + ((xi - 1) >> 1)];
-            const T xp = xi == NX - 1 ? c : oth[row * NXH + ((xi + 1) >> 1)];
-            const T ym = yi == 0 ? c : oth[i - NXH];
-            const T yp = yi == NY - 1 ? c : oth[i + NXH];
-            const T zm = zi == 0 ? c : oth[i - NY * NXH];
-            const T zp = zi == NZ - 1 ? c : oth[i + NY * NXH];
-            sum = xm + xp + ym + yp + zm + zp;
-        }
-        out[r] = relax_s<T>(sum, rhs[i], a.h2);
-    }
-#pragma unroll
-    for (int r = 0; r < ZR; r++)
-        if (at[r] >= 0) cur[at[r]] = out[r];
+    body::split_pass_column<T, MODE, ZR>(a, color, bx * BX + threadIdx.x, by * BY + threadIdx.y,
+                                         bz * ZR);
 }
 
 // Natural interior -> split (both colours).  `nat` uses the padded layout.
@@ -111,17 +42,10 @@ __global__ __launch_bounds__(BX *BY) void k_to_split(const TN *nat, int sy, int 
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int xi = bx * BX + threadIdx.x;
-    const int yi = by * BY + threadIdx.y;
-    const int zi = bz;
-    if (xi >= NX || yi >= NY) return;
-    const TN v = nat[(xi + g) + (yi + g) * sy + (zi + g) * sz];
-    const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    if (parity(xi, yi, zi, g)) black[i] = (TS)v; else red[i] = (TS)v;
+    body::to_split_cell<TS, TN>(nat, sy, sz, g, NX, NY, NXH, red, black, bx * BX + threadIdx.x,
+                                by * BY + threadIdx.y, bz);
 }
 
-// Split -> natural interior of P (FP64).  ADD: P += split (the correction
-// of the mixed-precision solve); else P = split.
 template <typename TS, bool ADD>
 __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int sz, int g, int NX,
                                                        int NY, int NXH, const TS *red,
@@ -130,14 +54,8 @@ __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int 
     if (done && *done) return;
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int xi = bx * BX + threadIdx.x;
-    const int yi = by * BY + threadIdx.y;
-    const int zi = bz;
-    if (xi >= NX || yi >= NY) return;
-    const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    const double v = (double)(parity(xi, yi, zi, g) ? black[i] : red[i]);
-    const int n = (xi + g) + (yi + g) * sy + (zi + g) * sz;
-    nat[n] = ADD ? nat[n] + v : v;
+    body::from_split_cell<TS, ADD>(nat, sy, sz, g, NX, NY, NXH, red, black, bx * BX + threadIdx.x,
+                                   by * BY + threadIdx.y, bz);
 }
 
 __global__ __launch_bounds__(BX *BY) void k_rhs32_split(const double *div, int sy, int sz, int g,
@@ -146,13 +64,8 @@ __global__ __launch_bounds__(BX *BY) void k_rhs32_split(const double *div, int s
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int xi = bx * BX + threadIdx.x;
-    const int yi = by * BY + threadIdx.y;
-    const int zi = bz;
-    if (xi >= NX || yi >= NY) return;
-    const float v = (float)(-(h2 * div[(xi + g) + (yi + g) * sy + (zi + g) * sz]));
-    const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    if (parity(xi, yi, zi, g)) black[i] = v; else red[i] = v;
+    body::rhs32_cell(div, sy, sz, g, NX, NY, NXH, h2, red, black, bx * BX + threadIdx.x,
+                     by * BY + threadIdx.y, bz);
 }
 
 inline int3 nblocks(int nx, int ny, int nz)

@@ -216,6 +216,97 @@ int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchun
  * counterpart. */
 int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper);
 
+/* ------------------------------------------------------------- batching */
+/* Many engines of uneven sizes stepped together.  No reference counterpart:
+ * the reference steps one Engine per Step call (aeroflux.go:348-409); a
+ * batch steps each of its engines exactly as that Step does.  A batch owns
+ * one HBM pool of a fixed budget; each engine added to it is a slot holding
+ * New's zeroed arena (grid.go:79-134, same field order and offsets) plus its
+ * solve buffers.  One batched step steps every resident slot once: each
+ * stage of the Strang palindrome is one persistent launch over a work-list
+ * of the tiles of all slots, and the step is replayed as one hipGraph.
+ * Membership changes between steps update the work-list, not the graph.
+ * Every slot's results are bit-identical to the same engine stepped alone
+ * with af_step.  Calls on one batch must come from one host thread at a
+ * time; each batch owns one HIP stream at default priority. */
+typedef struct af_batch af_batch;
+
+typedef struct af_batch_config {
+    int32_t Precision;         /* AF_PRECISION_* of every slot */
+    int32_t MaxSlots;          /* slot capacity (0 -> 1024) */
+    int64_t MemoryBudget;      /* bytes of HBM for slots (0 -> 1 GiB) */
+    /* Solve schedule shared by every slot; a slot's normalised config must
+     * carry the same values (0 -> the Config.Normalize defaults). */
+    int64_t GaussSeidelIters;
+    int32_t MixedMaxCycles;
+    int32_t MixedSweeps;
+    int32_t MixedFixSweeps;
+    int32_t MixedChecked;      /* mixed: 0 -> one cycle (slots need
+                                  MixedTarget == 0); 1 -> cycles until each
+                                  slot's MixedTarget (> 0) is met */
+    int32_t SweepShape;        /* 0 -> tuned; k -> shape k-1 (af_sweep_shape) */
+    int32_t Workgroups;        /* persistent workgroups per launch (0 -> tuned;
+                                  else a multiple of 8) */
+} af_batch_config;
+
+typedef struct af_batch_info {
+    int32_t Slots;             /* resident slots */
+    int32_t SweepShape;        /* advection workgroup shape in use (index) */
+    int32_t SweepWorkgroups;   /* persistent workgroups: advection sweeps */
+    int32_t PassWorkgroups;    /* persistent workgroups: red-black passes */
+    int32_t OtherWorkgroups;   /* persistent workgroups: every other stage */
+    int32_t Tuned;             /* 1 once the autotuner has run */
+    int64_t Cells;             /* interior cells over resident slots */
+    int64_t BytesInUse;        /* pool bytes held by slots */
+    int64_t BytesLargestFree;  /* largest free pool block */
+    int64_t MemoryBudget;
+} af_batch_info;
+
+int af_batch_new(const af_batch_config *cfg, af_batch **out);
+/* Releases the pool and every slot; idempotent.  Handle stays valid. */
+int af_batch_close(af_batch *b);
+void af_batch_free(af_batch *b);
+int af_batch_get_info(const af_batch *b, af_batch_info *out);
+
+/* Pool bytes one slot of this config takes (arena + solve buffers), 0 if
+ * the config does not normalise.  For admission decisions. */
+int64_t af_batch_slot_bytes(const af_config *cfg, int32_t precision);
+
+/* Adds an engine: normalises a copy of *cfg (as New), reserves its block in
+ * the pool and zeroes it (MAC.Zero, grid.go:158-163).  AF_ERR_OOM if no pool
+ * block fits; AF_ERR_CONFIG if the config does not match the batch's
+ * precision and solve schedule.  *slot receives the slot id. */
+int af_batch_add(af_batch *b, const af_config *cfg, int32_t *slot);
+/* Returns a slot's block to the pool (Close for one engine). */
+int af_batch_remove(af_batch *b, int32_t slot);
+
+/* n batched steps, one host synchronisation at the end.  Slots whose fault
+ * flag was raised (a NaN departure coordinate, as af_step) are written to
+ * faulted[0 .. min(*nfaulted, cap)); returns AF_ERR_FAULT if any. */
+int af_batch_step_n(af_batch *b, int64_t n, int32_t *faulted, int32_t cap, int32_t *nfaulted);
+
+/* Per-slot counterparts of the engine calls. */
+int af_batch_field_desc(const af_batch *b, int32_t slot, int field_id, af_field *out);
+int af_batch_field_read(af_batch *b, int32_t slot, int field_id, double *host, int64_t count);
+int af_batch_field_write(af_batch *b, int32_t slot, int field_id, const double *host,
+                         int64_t count);
+int af_batch_apply_velocity_bc(af_batch *b, int32_t slot);          /* stencil.go:246-315 */
+int af_batch_max_divergence(af_batch *b, int32_t slot, double *out); /* project.go:210-232 */
+int af_batch_kinetic_energy(af_batch *b, int32_t slot, double *out); /* project.go:238-258 */
+int af_batch_mixed_stats(af_batch *b, int32_t slot, double *last_div, int32_t *cycles);
+
+/* Runs the autotuner now on the resident slots (it otherwise runs at the
+ * first step): advection workgroup shape and persistent workgroup counts,
+ * each timed on batched launches.  Results are unaffected. */
+int af_batch_tune(af_batch *b);
+
+/* Average device time (ms) of `reps` launches of one batched stage over all
+ * resident slots: 0 = red-black colour pass, 1 = advection sweep (X axis),
+ * 2 = divergence, 3 = gradient subtract.  Measurement only: 0 and 1 leave
+ * the state a step reads unchanged (1 writes to the Tmp fields what the next
+ * step's first sweep writes); 2 rewrites Div and 3 changes the velocity. */
+int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *af_last_error(void);
 

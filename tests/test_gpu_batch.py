@@ -1,0 +1,205 @@
+"""Batched engines: each slot's results are the same alone and in the mix.
+
+A batch steps uneven grids together (persistent launches over a tile
+work-list of every slot, one hipGraph per step).  The bar is bit-identity
+with the same engine stepped alone through af_step -- itself bit-exact to
+the oracle in FP64 -- for every field of the arena, in FP64 and in both
+mixed-precision modes, across membership changes, tuned and fixed launch
+shapes, and with a faulting neighbour.
+"""
+import numpy as np
+import pytest
+
+import aeroflux_amd as afm
+from aeroflux_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("P", "U", "V", "W", "Div", "Tmp0", "Tmp1", "Tmp2")
+
+# Uneven grids, one with Ghost = 2, one with an odd x extent.
+GRIDS = [(16, 12, 10, 1), (24, 16, 20, 1), (32, 32, 32, 1), (12, 20, 8, 2), (33, 9, 14, 1)]
+
+
+def cfg_for(shp, i, **extra):
+    nx, ny, nz, g = shp
+    nu = [1e-5, 1e-4, 3e-6, 2e-5, 5e-5][i % 5]
+    h = 1.0 / max(nx, ny, nz)
+    dt = min(0.004, 0.5 * h * h / (6 * nu))
+    return dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, Viscosity=nu, DT=dt, **extra)
+
+
+def fields_for(shp, i):
+    nx, ny, nz, g = shp
+    return ic.taylor_green(nx, ny, nz, g, 1.0 / max(nx, ny, nz), seed=100 + i)
+
+
+def solo(af, cfg, f, steps):
+    e = af.New(af.Config(**cfg))
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), f[name])
+    e.apply_velocity_bc()
+    if steps:
+        e.StepN(steps)
+    out = [e.Mesh.Data(e.Mesh.field(k)) for k in range(8)]
+    extra = (e.MaxDivergence(), e.KineticEnergy())
+    stats = e.mixed_stats() if cfg.get("Precision") == afm.PRECISION_MIXED else None
+    e.Close()
+    return out, extra, stats
+
+
+def load(b, slot, f):
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        b.SetData(slot, fid, f[name])
+    b.apply_velocity_bc(slot)
+
+
+def assert_same(a, b, what):
+    for k in range(8):
+        if a[k].tobytes() != b[k].tobytes():
+            bad = np.argwhere(a[k].view(np.uint64) != b[k].view(np.uint64))
+            z, y, x = bad[0]
+            raise AssertionError(f"{what}: field {NAMES[k]} differs at {len(bad)} cells, first "
+                                 f"(x={x},y={y},z={z}): {a[k][z, y, x]!r} vs {b[k][z, y, x]!r}")
+
+
+def slot_state(b, slot):
+    return [b.Data(slot, k) for k in range(8)]
+
+
+PRECISIONS = {
+    "fp64": (afm.PRECISION_FP64, {}, {}),
+    "mixed_one_cycle": (afm.PRECISION_MIXED, {}, {}),
+    "mixed_checked": (afm.PRECISION_MIXED, {"MixedChecked": 1}, {"MixedTarget": 2e-6}),
+}
+
+
+@pytest.mark.parametrize("prec", list(PRECISIONS))
+def test_batch_equals_alone(af, prec):
+    p, bextra, eextra = PRECISIONS[prec]
+    steps = 4
+    cfgs = [cfg_for(s, i, Precision=p, **eextra) for i, s in enumerate(GRIDS)]
+    fs = [fields_for(s, i) for i, s in enumerate(GRIDS)]
+    b = af.Batch(af.BatchConfig(Precision=p, MemoryBudget=256 << 20, **bextra))
+    slots = [b.Add(af.Config(**c)) for c in cfgs]
+    for s, f in zip(slots, fs):
+        load(b, s, f)
+    b.StepN(steps)
+    info = b.info()
+    assert info["Slots"] == len(GRIDS) and info["Tuned"] == 1
+    for i, (s, c, f) in enumerate(zip(slots, cfgs, fs)):
+        ref, (d, ke), stats = solo(af, c, f, steps)
+        assert_same(slot_state(b, s), ref, f"{prec} slot {s} grid {GRIDS[i]}")
+        assert b.MaxDivergence(s) == d
+        assert b.KineticEnergy(s) == ke
+        if prec == "mixed_checked":
+            assert b.mixed_stats(s) == stats
+    b.Close()
+
+
+def test_membership_changes_between_steps(af):
+    """Continuous batching: engines join and leave between steps; each one's
+    state equals the same engine stepped alone for the steps it was in."""
+    b = af.Batch(af.BatchConfig(MemoryBudget=256 << 20))
+    cfgs = [cfg_for(s, i) for i, s in enumerate(GRIDS)]
+    fs = [fields_for(s, i) for i, s in enumerate(GRIDS)]
+    live = {}
+    # Steps 0-1: engines 0, 1.  Step 2: 2 joins.  Step 3: 0 leaves and 3
+    # takes its slot and pool block.  Step 4: 4 joins.
+    plan = {0: [0, 1], 2: [2], 3: [3], 4: [4]}
+    leave = {3: [0]}
+    done = {}
+    for step in range(6):
+        for i in leave.get(step, []):
+            slot, joined = live.pop(i)
+            done[i] = (slot_state(b, slot), step - joined)
+            b.Remove(slot)
+        for i in plan.get(step, []):
+            slot = b.Add(af.Config(**cfgs[i]))
+            load(b, slot, fs[i])
+            live[i] = (slot, step)
+        b.StepN(1)
+    for i, (slot, joined) in live.items():
+        done[i] = (slot_state(b, slot), 6 - joined)
+    for i, (state, n) in done.items():
+        ref, _, _ = solo(af, cfgs[i], fs[i], n)
+        assert_same(state, ref, f"engine {i} after {n} steps in a changing batch")
+    b.Close()
+
+
+def test_tuned_and_fixed_launch_shapes_agree(af):
+    cfgs = [cfg_for(s, i) for i, s in enumerate(GRIDS[:3])]
+    fs = [fields_for(s, i) for i, s in enumerate(GRIDS[:3])]
+    results = []
+    for shape, wg in ((0, 0), (1, 8), (8, 1024), (5, 64)):
+        b = af.Batch(af.BatchConfig(MemoryBudget=128 << 20, SweepShape=shape, Workgroups=wg))
+        slots = [b.Add(af.Config(**c)) for c in cfgs]
+        for s, f in zip(slots, fs):
+            load(b, s, f)
+        b.StepN(2)
+        results.append([slot_state(b, s) for s in slots])
+        b.Close()
+    for r in results[1:]:
+        for a, c in zip(results[0], r):
+            assert_same(a, c, "launch shape")
+
+
+def test_fault_in_one_slot_is_contained(af):
+    """A NaN departure coordinate in one slot raises that slot's flag only;
+    the other slots' results are unaffected."""
+    cfgs = [cfg_for(s, i) for i, s in enumerate(GRIDS[:3])]
+    fs = [fields_for(s, i) for i, s in enumerate(GRIDS[:3])]
+    bad = {k: v.copy() for k, v in fs[1].items()}
+    bad["U"][5, 5, 5] = np.nan
+    b = af.Batch(af.BatchConfig(MemoryBudget=128 << 20))
+    slots = [b.Add(af.Config(**c)) for c in cfgs]
+    for s, f in zip(slots, (fs[0], bad, fs[2])):
+        load(b, s, f)
+    with pytest.raises(af.AeroFluxError) as ei:
+        b.StepN(2)
+    assert ei.value.code == afm.AF_ERR_FAULT
+    assert ei.value.slots == [slots[1]]
+    for i in (0, 2):
+        ref, _, _ = solo(af, cfgs[i], fs[i], 2)
+        assert_same(slot_state(b, slots[i]), ref, f"healthy slot {i} beside a fault")
+    b.Close()
+
+
+def test_pool_admission_and_reuse(af):
+    c = cfg_for((32, 32, 32, 1), 0)
+    need = af.SlotBytes(af.Config(**c), afm.PRECISION_FP64)
+    assert need > af.MACBytes(32, 32, 32, 1)
+    b = af.Batch(af.BatchConfig(MemoryBudget=2 * need + need // 2, MaxSlots=8))
+    s0 = b.Add(af.Config(**c))
+    s1 = b.Add(af.Config(**c))
+    b.SetData(s0, afm.FIELD_U, np.full(b.field(s0, afm.FIELD_U).shape, 3.0))
+    with pytest.raises(af.AeroFluxError) as ei:
+        b.Add(af.Config(**c))
+    assert ei.value.code == afm.AF_ERR_OOM
+    b.Remove(s0)
+    s2 = b.Add(af.Config(**c))
+    assert s2 == s0
+    # The reused block starts zeroed (MAC.Zero), like a fresh engine.
+    assert all(not b.Data(s2, k).any() for k in range(8))
+    info = b.info()
+    assert info["Slots"] == 2 and info["BytesInUse"] == 2 * need
+    b.Remove(s1)
+    b.Remove(s2)
+    assert b.info()["BytesInUse"] == 0
+    b.Close()
+
+
+def test_batch_rejects_mismatched_schedule(af):
+    b = af.Batch(af.BatchConfig(MemoryBudget=64 << 20))
+    with pytest.raises(af.AeroFluxError) as ei:
+        b.Add(af.Config(**cfg_for((16, 16, 16, 1), 0, GaussSeidelIters=10)))
+    assert ei.value.code == afm.AF_ERR_CONFIG
+    with pytest.raises(af.AeroFluxError) as ei:
+        b.Add(af.Config(**cfg_for((16, 16, 16, 1), 0, Precision=afm.PRECISION_MIXED)))
+    assert ei.value.code == afm.AF_ERR_CONFIG
+    with pytest.raises(af.AeroFluxError):
+        b.Remove(3)
+    b.Close()
+    with pytest.raises(af.AeroFluxError) as ei:
+        b.Add(af.Config(**cfg_for((16, 16, 16, 1), 0)))
+    assert ei.value.code == afm.AF_ERR_CLOSED
