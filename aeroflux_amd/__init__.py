@@ -34,7 +34,7 @@ __all__ = [
     "FIELD_P", "FIELD_U", "FIELD_V", "FIELD_W", "FIELD_DIV", "FIELD_TMP0",
     "FIELD_TMP1", "FIELD_TMP2", "AXIS_X", "AXIS_Y", "AXIS_Z", "COLOR_RED",
     "COLOR_BLACK", "PRECISION_FP64", "PRECISION_MIXED", "lib_path", "load_library",
-    "Batch", "BatchConfig", "SlotBytes",
+    "Batch", "BatchConfig", "SlotBytes", "Serve",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -95,6 +95,28 @@ class _CBatchInfo(C.Structure):
     ]
 
 
+class _CServeJob(C.Structure):
+    _fields_ = [
+        ("NX", C.c_int64), ("NY", C.c_int64), ("NZ", C.c_int64), ("Steps", C.c_int64),
+        ("Viscosity", C.c_double), ("DT", C.c_double), ("H", C.c_double),
+        ("Seed", C.c_uint64), ("Arrival", C.c_double),
+    ]
+
+
+class _CServeResult(C.Structure):
+    _fields_ = [
+        ("Admitted", C.c_double), ("Done", C.c_double), ("MaxDivergence", C.c_double),
+        ("KineticEnergy", C.c_double), ("Status", C.c_int32), ("Slot", C.c_int32),
+    ]
+
+
+class _CServeStats(C.Structure):
+    _fields_ = [
+        ("Wall", C.c_double), ("Steps", C.c_int64), ("CellUpdates", C.c_int64),
+        ("MaxResident", C.c_int32), ("Pad", C.c_int32), ("PeakBytes", C.c_int64),
+    ]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported symbol in include/aeroflux.h.
@@ -151,6 +173,13 @@ _SIGNATURES = [
                                        C.POINTER(C.c_int32)]),
     ("af_batch_tune", C.c_int, [C.c_void_p]),
     ("af_batch_time_stage", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    ("af_init_band_limited", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_double,
+                                        C.c_double]),
+    ("af_batch_init_band_limited", C.c_int, [C.c_void_p, C.c_int32, C.c_uint64, C.c_int32,
+                                              C.c_double, C.c_double]),
+    ("af_serve", C.c_int, [C.c_void_p, C.POINTER(_CServeJob), C.c_int64, C.c_int32, C.c_double,
+                            C.c_double, C.c_int32, C.POINTER(_CServeResult),
+                            C.POINTER(_CServeStats)]),
     ("af_last_error", C.c_char_p, []),
 ]
 
@@ -423,6 +452,12 @@ class Engine:
         _check(load_library().af_mixed_stats(self._handle(), C.byref(d), C.byref(n)))
         return float(d.value), int(n.value)
 
+    def init_band_limited(self, seed: int, kmax: int = 8, slope: float = -5.0 / 3.0,
+                          rms: float = 1.0) -> None:
+        """Seeded band-limited velocity built on the GPU, velocity BC applied
+        (the field of aeroflux_amd.ic.band_limited, up to summation order)."""
+        _check(load_library().af_init_band_limited(self._handle(), seed, kmax, slope, rms))
+
 
 def New(cfg: Config) -> Engine:
     """New, aeroflux.go:176-213.  ``cfg`` is normalised in a copy."""
@@ -567,6 +602,11 @@ class Batch:
                                                    C.byref(n)))
         return float(d.value), int(n.value)
 
+    def init_band_limited(self, slot: int, seed: int, kmax: int = 8,
+                          slope: float = -5.0 / 3.0, rms: float = 1.0) -> None:
+        _check(load_library().af_batch_init_band_limited(self._handle(), slot, seed, kmax, slope,
+                                                         rms))
+
     def Tune(self) -> None:
         _check(load_library().af_batch_tune(self._handle()))
 
@@ -602,3 +642,22 @@ class Batch:
                 self._h = None
         except Exception:
             pass
+
+
+def Serve(batch: Batch, jobs: list[dict], kmax: int = 8, slope: float = -5.0 / 3.0,
+          rms: float = 1.0, realtime: bool = True) -> tuple[list[dict], dict]:
+    """Runs step-N jobs through an empty batch with continuous batching
+    (af_serve).  Each job is a dict with NX, NY, NZ, Steps, Viscosity, DT,
+    H, Seed, Arrival (seconds; sorted).  Returns per-job results (Admitted,
+    Done, MaxDivergence, KineticEnergy, Status, Slot) and run statistics."""
+    n = len(jobs)
+    arr = (_CServeJob * max(n, 1))()
+    for i, j in enumerate(jobs):
+        for name, _ in _CServeJob._fields_:
+            setattr(arr[i], name, j.get(name, 0))
+    res = (_CServeResult * max(n, 1))()
+    st = _CServeStats()
+    _check(load_library().af_serve(batch._handle(), arr, n, kmax, slope, rms, int(realtime),
+                                   res, C.byref(st)))
+    out = [{name: getattr(res[i], name) for name, _ in _CServeResult._fields_} for i in range(n)]
+    return out, {name: getattr(st, name) for name, _ in _CServeStats._fields_ if name != "Pad"}

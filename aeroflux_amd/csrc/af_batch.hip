@@ -17,7 +17,11 @@
 //     engine, so a slot's results are bit-identical to af_step's.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <climits>
+#include <cmath>
+#include <deque>
+#include <thread>
 #include <cstring>
 #include <map>
 #include <new>
@@ -26,6 +30,7 @@
 #include "../../include/aeroflux.h"
 #include "af_bodies.h"
 #include "af_host.h"
+#include "af_ic.h"
 #include "af_internal.h"
 
 #pragma clang fp contract(off)
@@ -401,7 +406,11 @@ struct HSlot {
     af_config cfg{};
     SlotLayout L{};
     int64_t off = 0;
+    int64_t desc_step = -1;  // serving: batched step before which the slot's
+                             // descriptor upload was enqueued
 };
+
+constexpr int kTabRing = 3;  // pinned work-list staging buffers
 
 }  // namespace
 }  // namespace af
@@ -416,7 +425,7 @@ struct af_batch {
     Pool pool;
     std::vector<HSlot> slots;
     BSlot *d_slots = nullptr, *h_slots = nullptr;  // h_slots pinned
-    int *d_tab = nullptr, *h_tab = nullptr;        // h_tab pinned
+    int *d_tab = nullptr, *h_tab = nullptr;        // h_tab pinned, kTabRing tables
     int64_t tab_ints = 0;
     int *d_ctl = nullptr;    // flags | done | cycles, MaxSlots each
     double *d_last = nullptr;
@@ -429,6 +438,7 @@ struct af_batch {
     bool tuned = false;
     int nact = 0;
     int64_t cells = 0;
+    ICScratch ic;  // initial-condition generator scratch (lazy, outside the pool)
 };
 
 namespace {
@@ -476,11 +486,11 @@ Geom slot_geom(const HSlot &h)
     return Geom{(int)h.cfg.NX, (int)h.cfg.NY, (int)h.cfg.NZ, (int)h.cfg.Ghost};
 }
 
-// Rebuilds the host work-list table for the current slots and sweep shape.
-int build_table(af_batch *b, int shape)
+// Rebuilds staging table `ring` for the current slots and sweep shape.
+int build_table(af_batch *b, int shape, int ring = 0)
 {
     const int stride = b->cfg.MaxSlots + 1;
-    int *t = b->h_tab;
+    int *t = b->h_tab + (int64_t)ring * b->tab_ints;
     int *act = t + 2;
     int n = 0;
     for (int s = 0; s < (int)b->slots.size(); s++)
@@ -775,6 +785,7 @@ void release(af_batch *b)
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
+    b->ic.release();
     if (b->pool.base) (void)hipFree(b->pool.base);
     if (b->d_slots) (void)hipFree(b->d_slots);
     if (b->d_tab) (void)hipFree(b->d_tab);
@@ -872,7 +883,7 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
         return bail(err, "allocate batch tables", AF_ERR_OOM);
     if ((err = hipHostMalloc(&b->h_slots, sizeof(BSlot) * (size_t)m, hipHostMallocDefault)) !=
             hipSuccess ||
-        (err = hipHostMalloc(&b->h_tab, sizeof(int) * (size_t)b->tab_ints,
+        (err = hipHostMalloc(&b->h_tab, sizeof(int) * (size_t)b->tab_ints * kTabRing,
                              hipHostMallocDefault)) != hipSuccess ||
         (err = hipHostMalloc(&b->h_ints, sizeof(int) * (size_t)m, hipHostMallocDefault)) !=
             hipSuccess ||
@@ -929,12 +940,10 @@ int af_batch_get_info(const af_batch *b, af_batch_info *out)
     return AF_OK;
 }
 
-int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
+// Checks a normalised engine config against the batch's precision and
+// solve schedule.
+static int check_cfg(const af_batch *b, const af_config &c)
 {
-    if (int rc = check_open(b)) return rc;
-    if (!cfg_in || !slot_out) return fail(AF_ERR_ARG, "aeroflux: null argument");
-    af_config c = *cfg_in;
-    if (int rc = af_config_normalize(&c)) return rc;
     if (c.UseSparseSolver)
         return fail(AF_ERR_UNSUPPORTED,
                     "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
@@ -958,16 +967,33 @@ int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
                             ? "aeroflux: a checked mixed-precision batch needs MixedTarget > 0"
                             : "aeroflux: a one-cycle mixed-precision batch needs MixedTarget == 0");
     }
-    SlotLayout L;
-    if (!slot_layout(c, c.Precision, &L))
-        return fail(AF_ERR_CONFIG, "aeroflux: a %ldx%ldx%ld grid exceeds 2^31 elements per field",
-                    (long)c.NX, (long)c.NY, (long)c.NZ);
-    int slot = -1;
+    return AF_OK;
+}
+
+static void free_slot(af_batch *b, int slot)
+{
+    HSlot &h = b->slots[slot];
+    b->pool.release(h.off, h.L.total);
+    b->cells -= h.cfg.NX * h.cfg.NY * h.cfg.NZ;
+    h.used = false;
+    b->tab_dirty = true;
+}
+
+// Takes a free slot id and pool block for a checked config and enqueues its
+// zeroing (MAC.Zero, grid.go:158-163), descriptor upload and flag resets on
+// the batch stream.  AF_ERR_OOM, with nothing changed, if either is missing.
+// The caller must know that no earlier upload from the slot's pinned
+// descriptor is still pending.
+static int pick_slot(const af_batch *b)
+{
     for (int s = 0; s < (int)b->slots.size(); s++)
-        if (!b->slots[s].used) {
-            slot = s;
-            break;
-        }
+        if (!b->slots[s].used) return s;
+    return -1;
+}
+
+static int place_slot(af_batch *b, const af_config &c, const SlotLayout &L, int *slot_out)
+{
+    const int slot = pick_slot(b);
     if (slot < 0) return fail(AF_ERR_OOM, "aeroflux: batch has no free slot");
     int64_t off;
     if (!b->pool.alloc(L.total, &off))
@@ -976,7 +1002,6 @@ int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
                     "free block %ld)",
                     (long)L.total, (long)b->pool.used, (long)b->pool.cap,
                     (long)b->pool.largest());
-    HIP_TRY(hipSetDevice(b->device), "set device");
     HSlot &h = b->slots[slot];
     h.used = true;
     h.cfg = c;
@@ -1003,12 +1028,6 @@ int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
     d.out = reinterpret_cast<double *>(base + L.out_off);
     b->cells += c.NX * c.NY * c.NZ;
     b->tab_dirty = true;
-    auto undo = [&](hipError_t er, const char *what) {
-        h.used = false;
-        b->pool.release(off, L.total);
-        b->cells -= c.NX * c.NY * c.NZ;
-        return hip_fail(er, what);
-    };
     hipError_t err = launch_zero(reinterpret_cast<double *>(base), L.total / 8, b->stream);
     if (err == hipSuccess)
         err = hipMemcpyAsync(b->d_slots + slot, b->h_slots + slot, sizeof(BSlot),
@@ -1016,14 +1035,41 @@ int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
     if (err == hipSuccess) {
         int *ctl3[3] = {b->d_ctl + slot, b->d_ctl + b->cfg.MaxSlots + slot,
                         b->d_ctl + 2 * b->cfg.MaxSlots + slot};
-        for (int *p : ctl3) {
+        for (int *p : ctl3)
             hipLaunchKernelGGL(kb_fill_int, dim3(1), dim3(256), 0, b->stream, p, 1, 0);
-        }
         err = hipGetLastError();
     }
     if (err == hipSuccess) err = launch_zero(b->d_last + slot, 1, b->stream);
-    if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
-    if (err != hipSuccess) return undo(err, "add engine to batch");
+    if (err != hipSuccess) {
+        free_slot(b, slot);
+        return hip_fail(err, "add engine to batch");
+    }
+    *slot_out = slot;
+    return AF_OK;
+}
+
+int af_batch_add(af_batch *b, const af_config *cfg_in, int32_t *slot_out)
+{
+    if (int rc = check_open(b)) return rc;
+    if (!cfg_in || !slot_out) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    af_config c = *cfg_in;
+    if (int rc = af_config_normalize(&c)) return rc;
+    if (int rc = check_cfg(b, c)) return rc;
+    SlotLayout L;
+    if (!slot_layout(c, c.Precision, &L))
+        return fail(AF_ERR_CONFIG, "aeroflux: a %ldx%ldx%ld grid exceeds 2^31 elements per field",
+                    (long)c.NX, (long)c.NY, (long)c.NZ);
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    // Every earlier upload from the pinned descriptors has run once the
+    // stream is idle.
+    HIP_TRY(hipStreamSynchronize(b->stream), "add engine to batch");
+    int slot;
+    if (int rc = place_slot(b, c, L, &slot)) return rc;
+    hipError_t err = hipStreamSynchronize(b->stream);
+    if (err != hipSuccess) {
+        free_slot(b, slot);
+        return hip_fail(err, "add engine to batch");
+    }
     *slot_out = slot;
     return AF_OK;
 }
@@ -1033,11 +1079,7 @@ int af_batch_remove(af_batch *b, int32_t slot)
     if (int rc = check_slot(b, slot)) return rc;
     HIP_TRY(hipSetDevice(b->device), "set device");
     HIP_TRY(hipStreamSynchronize(b->stream), "remove engine from batch");
-    HSlot &h = b->slots[slot];
-    b->pool.release(h.off, h.L.total);
-    b->cells -= h.cfg.NX * h.cfg.NY * h.cfg.NZ;
-    h.used = false;
-    b->tab_dirty = true;
+    free_slot(b, slot);
     return AF_OK;
 }
 
@@ -1136,6 +1178,26 @@ int af_batch_apply_velocity_bc(af_batch *b, int32_t slot)
     return sync(b, "velocity bc");
 }
 
+int af_batch_init_band_limited(af_batch *b, int32_t slot, uint64_t seed, int32_t kmax,
+                               double slope, double rms)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    if (kmax < 1 || kmax > 64 || !(rms > 0) || std::isinf(rms) || std::isnan(slope))
+        return fail(AF_ERR_ARG, "aeroflux: band-limited field needs 1 <= kmax <= 64, rms > 0");
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    const HSlot &h = b->slots[slot];
+    const DField U = slot_field(b, h, AF_FIELD_U), V = slot_field(b, h, AF_FIELD_V),
+                 W = slot_field(b, h, AF_FIELD_W);
+    HIP_TRY(launch_zero(U.d, h.L.f[AF_FIELD_U].Count, b->stream), "zero velocity");
+    HIP_TRY(launch_zero(V.d, h.L.f[AF_FIELD_V].Count, b->stream), "zero velocity");
+    HIP_TRY(launch_zero(W.d, h.L.f[AF_FIELD_W].Count, b->stream), "zero velocity");
+    HIP_TRY(launch_band_limited(U, V, W, slot_geom(h), h.cfg.H, seed, kmax, slope, rms, b->ic,
+                                b->stream),
+            "band-limited initial condition");
+    HIP_TRY(launch_velocity_bc(U, V, W, slot_geom(h), b->stream), "velocity bc");
+    return sync(b, "band-limited initial condition");
+}
+
 static int slot_reduce(af_batch *b, int32_t slot, bool ke, double *out)
 {
     if (int rc = check_slot(b, slot)) return rc;
@@ -1209,6 +1271,232 @@ int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
             }
         },
         avg_ms);
+}
+
+// ------------------------------------------------------------- serving
+
+int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, double slope,
+             double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats)
+{
+    if (int rc = check_open(b)) return rc;
+    if ((!jobs || !out) && n > 0) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    if (n < 0 || kmax < 1 || kmax > 64 || !(rms > 0))
+        return fail(AF_ERR_ARG, "aeroflux: bad serving arguments");
+    for (const HSlot &h : b->slots)
+        if (h.used) return fail(AF_ERR_ARG, "aeroflux: af_serve needs an empty batch");
+    for (int64_t i = 1; i < n; i++)
+        if (jobs[i].Arrival < jobs[i - 1].Arrival)
+            return fail(AF_ERR_ARG, "aeroflux: jobs must be sorted by arrival time");
+    HIP_TRY(hipSetDevice(b->device), "set device");
+
+    // Job configs, as New would normalise them, and their pool footprints.
+    std::vector<af_config> cfgs((size_t)n);
+    std::vector<SlotLayout> lays((size_t)n);
+    std::vector<int32_t> status((size_t)n, AF_OK);
+    int64_t ic_bytes = 0;
+    for (int64_t i = 0; i < n; i++) {
+        const af_serve_job &j = jobs[i];
+        af_config c{};
+        c.NX = j.NX; c.NY = j.NY; c.NZ = j.NZ;
+        c.H = j.H; c.Viscosity = j.Viscosity; c.DT = j.DT;
+        c.GaussSeidelIters = b->cfg.GaussSeidelIters;
+        c.Precision = b->cfg.Precision;
+        c.MixedMaxCycles = b->cfg.MixedMaxCycles;
+        c.MixedSweeps = b->cfg.MixedSweeps;
+        c.MixedFixSweeps = b->cfg.MixedFixSweeps;
+        if (b->cfg.MixedChecked)
+            return fail(AF_ERR_ARG, "aeroflux: af_serve runs one-cycle mixed or FP64 batches");
+        if (j.Steps < 1) return fail(AF_ERR_ARG, "aeroflux: job %ld has no steps", (long)i);
+        if (int rc = af_config_normalize(&c)) return rc;
+        if (!slot_layout(c, c.Precision, &lays[i]))
+            return fail(AF_ERR_CONFIG, "aeroflux: job %ld grid exceeds 2^31 elements per field",
+                        (long)i);
+        cfgs[i] = c;
+        if (lays[i].total > b->pool.cap) status[i] = AF_ERR_OOM;
+        const Geom G{(int)c.NX, (int)c.NY, (int)c.NZ, (int)c.Ghost};
+        ic_bytes = std::max(ic_bytes, ic_scratch_bytes(G, kmax));
+    }
+    // Pinned per-job results: max|div|, kinetic-energy sum, fault flag.
+    double *res = nullptr;
+    int *flags = nullptr;
+    HIP_TRY(hipStreamSynchronize(b->stream), "serve");
+    HIP_TRY(b->ic.reserve(ic_bytes), "reserve initial-condition scratch");
+    HIP_TRY(hipHostMalloc(&res, sizeof(double) * 2 * (size_t)std::max<int64_t>(n, 1),
+                          hipHostMallocDefault),
+            "allocate pinned results");
+    if (hipHostMalloc(&flags, sizeof(int) * (size_t)std::max<int64_t>(n, 1),
+                      hipHostMallocDefault) != hipSuccess) {
+        (void)hipHostFree(res);
+        return fail(AF_ERR_OOM, "aeroflux: allocate pinned results");
+    }
+    constexpr int DEPTH = 2;  // batched steps in flight behind the host
+    constexpr int R = DEPTH + 1;
+    static_assert(R <= kTabRing, "work-list staging ring too small");
+    hipEvent_t ev[R] = {};
+    struct Res {
+        int64_t job;
+        int slot;
+        int64_t left;
+    };
+    std::vector<Res> resident;
+    std::deque<std::pair<int64_t, std::vector<int64_t>>> pending;
+    std::deque<int64_t> queue;
+    int64_t next = 0, step = 0, completed = -1;
+    af_serve_stats st{};
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now = [&] { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    int rc = AF_OK;
+    auto finish_oldest = [&]() -> hipError_t {
+        auto &p = pending.front();
+        hipError_t e = hipEventSynchronize(ev[p.first % R]);
+        if (e != hipSuccess) return e;
+        const double t = now();
+        for (int64_t j : p.second) {
+            out[j].Done = t;
+            st.Wall = t;
+        }
+        completed = p.first;
+        pending.pop_front();
+        return hipSuccess;
+    };
+    hipError_t err = hipSuccess;
+    for (int k = 0; k < R && err == hipSuccess; k++)
+        err = hipEventCreateWithFlags(&ev[k], hipEventDisableTiming);
+    for (int64_t i = 0; i < n; i++) {
+        out[i] = af_serve_result{};
+        out[i].Slot = -1;
+        out[i].Status = status[i];
+    }
+    while (err == hipSuccess && rc == AF_OK) {
+        const double t = now();
+        while (next < n && (!realtime || jobs[next].Arrival <= t)) queue.push_back(next++);
+        // Admission, first come first served, while the pool can place the
+        // head job.
+        while (!queue.empty() && err == hipSuccess) {
+            const int64_t j = queue.front();
+            if (status[j] == AF_ERR_OOM) {
+                out[j].Admitted = out[j].Done = t;
+                queue.pop_front();
+                continue;
+            }
+            const int cand = pick_slot(b);
+            if (cand < 0) break;
+            // The slot's pinned descriptor is rewritten below; an upload of
+            // it for an earlier job must have run (it has once that job's
+            // first step is complete).
+            while (b->slots[cand].desc_step > completed && !pending.empty() &&
+                   (err = finish_oldest()) == hipSuccess) {
+            }
+            if (err != hipSuccess) break;
+            int slot;
+            if (place_slot(b, cfgs[j], lays[j], &slot) != AF_OK) break;
+            HSlot &h = b->slots[slot];
+            h.desc_step = step;
+            const DField U = slot_field(b, h, AF_FIELD_U), V = slot_field(b, h, AF_FIELD_V),
+                         W = slot_field(b, h, AF_FIELD_W);
+            const Geom G = slot_geom(h);
+            if (err == hipSuccess)
+                err = launch_band_limited(U, V, W, G, h.cfg.H, jobs[j].Seed, kmax, slope, rms,
+                                          b->ic, b->stream);
+            if (err == hipSuccess) err = launch_velocity_bc(U, V, W, G, b->stream);
+            out[j].Admitted = t;
+            out[j].Slot = slot;
+            resident.push_back({j, slot, jobs[j].Steps});
+            queue.pop_front();
+            st.MaxResident = std::max<int32_t>(st.MaxResident, (int32_t)resident.size());
+            st.PeakBytes = std::max(st.PeakBytes, b->pool.used);
+        }
+        if (err != hipSuccess) break;
+        if (resident.empty() && !queue.empty()) {
+            rc = fail(AF_ERR_OOM, "aeroflux: an empty batch cannot place job %ld",
+                      (long)queue.front());
+            break;
+        }
+        auto retire = [&](const std::vector<int64_t> &done_jobs) -> hipError_t {
+            for (int64_t j : done_jobs) {
+                const int slot = out[j].Slot;
+                const HSlot &h = b->slots[slot];
+                const BSlot &d = b->h_slots[slot];
+                const DField U = slot_field(b, h, AF_FIELD_U), V = slot_field(b, h, AF_FIELD_V),
+                             W = slot_field(b, h, AF_FIELD_W);
+                hipError_t e = launch_max_divergence(U, V, W, d.G, d.invH, d.partials, d.out,
+                                                     b->stream);
+                if (e == hipSuccess)
+                    e = launch_kinetic_energy(U, V, W, d.G, d.partials, d.out + 1, b->stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(res + 2 * j, d.out, 2 * sizeof(double),
+                                       hipMemcpyDeviceToHost, b->stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(flags + j, b->d_ctl + slot, sizeof(int),
+                                       hipMemcpyDeviceToHost, b->stream);
+                if (e != hipSuccess) return e;
+                st.CellUpdates += cfgs[j].NX * cfgs[j].NY * cfgs[j].NZ * jobs[j].Steps;
+                free_slot(b, slot);
+            }
+            return hipSuccess;
+        };
+        if (resident.empty()) {
+            while (!pending.empty() && err == hipSuccess) err = finish_oldest();
+            if (err != hipSuccess) break;
+            if (next >= n && queue.empty()) break;
+            if (queue.empty()) {
+                const double wait = jobs[next].Arrival - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            }
+            continue;
+        }
+        if (!b->tuned) {
+            if ((err = hipStreamSynchronize(b->stream)) != hipSuccess) break;
+            if ((rc = upload_table(b, b->shape)) != AF_OK) break;
+            if ((rc = tune(b)) != AF_OK) break;
+        }
+        if ((rc = get_graph(b)) != AF_OK) break;
+        if (b->tab_dirty || b->tab_shape != b->shape) {
+            const int ring = (int)(step % R);
+            if ((rc = build_table(b, b->shape, ring)) != AF_OK) break;
+            if ((err = hipMemcpyAsync(b->d_tab, b->h_tab + (int64_t)ring * b->tab_ints,
+                                      (size_t)b->tab_ints * sizeof(int), hipMemcpyHostToDevice,
+                                      b->stream)) != hipSuccess)
+                break;
+            b->tab_dirty = false;
+            b->tab_shape = b->shape;
+        }
+        if ((err = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) break;
+        if ((err = hipEventRecord(ev[step % R], b->stream)) != hipSuccess) break;
+        std::vector<int64_t> done_now;
+        for (size_t r = 0; r < resident.size();)
+            if (--resident[r].left == 0) {
+                done_now.push_back(resident[r].job);
+                std::swap(resident[r], resident.back());
+                resident.pop_back();
+            } else {
+                r++;
+            }
+        if ((err = retire(done_now)) != hipSuccess) break;
+        pending.emplace_back(step, std::move(done_now));
+        step++;
+        while (pending.size() > (size_t)DEPTH && err == hipSuccess) err = finish_oldest();
+    }
+    while (!pending.empty() && err == hipSuccess) err = finish_oldest();
+    if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
+    for (int k = 0; k < R; k++)
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+    if (err == hipSuccess && rc == AF_OK) {
+        for (int64_t j = 0; j < n; j++) {
+            if (status[j] == AF_ERR_OOM) continue;
+            out[j].MaxDivergence = res[2 * j];
+            const double h = cfgs[j].H;
+            out[j].KineticEnergy = 0.5 * res[2 * j + 1] * (h * h * h);  // project.go:257
+            if (flags[j]) out[j].Status = AF_ERR_FAULT;
+        }
+        st.Steps = step;
+        if (stats) *stats = st;
+    }
+    (void)hipHostFree(res);
+    (void)hipHostFree(flags);
+    if (err != hipSuccess) return hip_fail(err, "serve");
+    return rc;
 }
 
 }  // extern "C"

@@ -19,6 +19,7 @@
 
 #include "../../include/aeroflux.h"
 #include "af_host.h"
+#include "af_ic.h"
 #include "af_internal.h"
 #include "af_mixed.h"
 #include "af_rbgs_split.h"
@@ -134,6 +135,7 @@ struct af_engine {
     int sweep_shape = 4;        // advection workgroup shape (tuned at New)
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
+    ICScratch ic;               // initial-condition generator scratch (lazy)
 };
 
 namespace {
@@ -401,6 +403,7 @@ void release(af_engine *e)
             g = nullptr;
         }
     release_mixed(e->mixed);
+    e->ic.release();
     if (e->arena) (void)hipFree(e->arena);
     if (e->pbuf) (void)hipFree(e->pbuf);
     e->pbuf = nullptr;
@@ -1010,6 +1013,24 @@ int af_project(af_engine *e)
     STAGE_BEGIN(e);
     HIP_TRY(enq_project(e, e->cur), "project");
     return sync_stream(e, "project");
+}
+
+int af_init_band_limited(af_engine *e, uint64_t seed, int32_t kmax, double slope, double rms)
+{
+    STAGE_BEGIN(e);
+    if (kmax < 1 || kmax > 64 || !(rms > 0) || std::isinf(rms) || std::isnan(slope))
+        return fail(AF_ERR_ARG, "aeroflux: band-limited field needs 1 <= kmax <= 64, rms > 0");
+    const Desc &d = e->cur;
+    const DField U = dfield(e, d.f[AF_FIELD_U]), V = dfield(e, d.f[AF_FIELD_V]),
+                 W = dfield(e, d.f[AF_FIELD_W]);
+    HIP_TRY(launch_zero(U.d, d.f[AF_FIELD_U].Count, e->stream), "zero velocity");
+    HIP_TRY(launch_zero(V.d, d.f[AF_FIELD_V].Count, e->stream), "zero velocity");
+    HIP_TRY(launch_zero(W.d, d.f[AF_FIELD_W].Count, e->stream), "zero velocity");
+    HIP_TRY(launch_band_limited(U, V, W, e->G, e->cfg.H, seed, kmax, slope, rms, e->ic,
+                                e->stream),
+            "band-limited initial condition");
+    HIP_TRY(enq_velocity_bc(e, d), "velocity bc");
+    return sync_stream(e, "band-limited initial condition");
 }
 
 int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchunk)

@@ -307,6 +307,68 @@ int af_batch_tune(af_batch *b);
  * step's first sweep writes); 2 rewrites Div and 3 changes the velocity. */
 int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms);
 
+/* -------------------------------------------------------------- serving */
+/* A request loop over a batch (no reference counterpart).  Step-N jobs of
+ * different sizes arrive over time; each is admitted as a slot as soon as
+ * the pool can place it (first come, first served), gets a seeded
+ * band-limited initial velocity on the device, and is stepped with every
+ * other resident job by the batched step graph (continuous batching: the
+ * resident set changes between steps, while earlier steps still run).  When
+ * a job's N steps are done its final MaxDivergence and KineticEnergy are
+ * recorded and its block returns to the pool.  Every job's result equals
+ * the same engine stepped alone (af_new, af_init_band_limited, af_step_n). */
+typedef struct af_serve_job {
+    int64_t NX, NY, NZ;
+    int64_t Steps;             /* N */
+    double Viscosity;
+    double DT;                 /* 0 -> Config.Normalize default */
+    double H;                  /* 0 -> 1 / max(NX, NY, NZ) */
+    uint64_t Seed;             /* initial condition */
+    double Arrival;            /* seconds after the start of the run */
+} af_serve_job;
+
+typedef struct af_serve_result {
+    double Admitted;           /* seconds after start: pool block granted */
+    double Done;               /* seconds after start: last step seen complete */
+    double MaxDivergence;      /* after the last step (project.go:210-232) */
+    double KineticEnergy;      /* after the last step (project.go:238-258) */
+    int32_t Status;            /* AF_OK, AF_ERR_FAULT, or AF_ERR_OOM if the
+                                  job can never fit the pool */
+    int32_t Slot;
+} af_serve_result;
+
+typedef struct af_serve_stats {
+    double Wall;               /* seconds from start to the last completion */
+    int64_t Steps;             /* batched steps replayed */
+    int64_t CellUpdates;       /* sum over jobs of NX*NY*NZ*N */
+    int32_t MaxResident;       /* peak concurrent jobs */
+    int32_t Pad;
+    int64_t PeakBytes;         /* peak pool bytes in use */
+} af_serve_stats;
+
+/* Runs `n` jobs (sorted by Arrival) through batch `b`, which must be empty.
+ * Initial conditions: af_init_band_limited(seed, kmax, slope, rms).
+ * realtime = 1: a job is visible only from its arrival time on the wall
+ * clock; 0: all jobs are queued at the start.  Blocks until every job is
+ * done; out[i] receives job i's result. */
+int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, double slope,
+             double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats);
+
+/* ------------------------------------------------- initial conditions */
+/* Seeded band-limited random velocity, built on the GPU (no reference
+ * counterpart: the reference's tests seed fields through Mesh.Data,
+ * engine_test.go:302-335).  Zeroes U, V and W, writes
+ *   sum over integer k with 1 <= |k| <= kmax of
+ *   |k|^slope cos(2 pi k.x / L + phi_{c,k})
+ * at each component's interior faces (L = N*H per axis; phases are 2 pi
+ * times SplitMix64 uniforms from `seed`), scales the three components so
+ * their mean square over all interior faces is rms^2, and applies the
+ * velocity BC (stencil.go:246-315).  The same field as the host generator
+ * aeroflux_amd.ic.band_limited up to summation order. */
+int af_init_band_limited(af_engine *e, uint64_t seed, int32_t kmax, double slope, double rms);
+int af_batch_init_band_limited(af_batch *b, int32_t slot, uint64_t seed, int32_t kmax,
+                               double slope, double rms);
+
 /* Message of the last failure on this thread ("" if none). */
 const char *af_last_error(void);
 

@@ -1,0 +1,84 @@
+"""Served step-job stream: each result is the same alone and in the mix.
+
+af_serve admits jobs as the pool frees up, steps every resident job with the
+batched step graph (the resident set changes between steps while earlier
+steps still run) and records each job's final MaxDivergence and
+KineticEnergy.  Every job is re-run alone (New, the same device initial
+condition, StepN(N)) and both numbers must match bit for bit; the pool is
+sized so that jobs queue, and the arrivals overlap running work.
+"""
+import numpy as np
+import pytest
+
+import aeroflux_amd as afm
+from aeroflux_amd import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def small_stream(n, seed, rate):
+    jobs = workload.served_jobs(n, seed, rate=rate, sizes=(8, 12, 16, 24, 32), steps=(1, 6))
+    return jobs
+
+
+def alone(af, prec, j):
+    e = af.New(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], H=j["H"], DT=j["DT"],
+                         Viscosity=j["Viscosity"], Precision=prec))
+    e.init_band_limited(j["Seed"])
+    e.StepN(j["Steps"])
+    out = (e.MaxDivergence(), e.KineticEnergy())
+    e.Close()
+    return out
+
+
+@pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
+@pytest.mark.parametrize("realtime", [False, True])
+def test_served_results_equal_alone(af, prec, realtime):
+    jobs = small_stream(14, 5 + prec, rate=200.0)
+    biggest = max(af.SlotBytes(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], Precision=prec),
+                               prec) for j in jobs)
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=3 * biggest, MaxSlots=16))
+    res, st = af.Serve(b, jobs, realtime=realtime)
+    assert st["Steps"] > 0 and 1 <= st["MaxResident"] <= len(jobs)
+    assert st["PeakBytes"] <= 3 * biggest
+    assert st["CellUpdates"] == sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
+    assert b.info()["Slots"] == 0 and b.info()["BytesInUse"] == 0
+    for j, r in zip(jobs, res):
+        assert r["Status"] == afm.AF_OK
+        assert r["Done"] >= r["Admitted"] >= 0
+        if realtime:
+            assert r["Admitted"] >= j["Arrival"]
+        d, ke = alone(af, prec, j)
+        assert r["MaxDivergence"] == d, j
+        assert r["KineticEnergy"] == ke, j
+    b.Close()
+
+
+def test_served_job_larger_than_pool_is_refused(af):
+    jobs = small_stream(4, 9, rate=1000.0)
+    jobs[2] = dict(jobs[2], NX=64, NY=64, NZ=64, H=1 / 64)
+    need = max(af.SlotBytes(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"]), 0) for j in jobs
+               if j["NX"] < 64)
+    b = af.Batch(af.BatchConfig(MemoryBudget=2 * need))
+    res, _ = af.Serve(b, jobs, realtime=False)
+    assert [r["Status"] for r in res] == [0, 0, afm.AF_ERR_OOM, 0]
+    for i in (0, 1, 3):
+        d, ke = alone(af, afm.PRECISION_FP64, jobs[i])
+        assert (res[i]["MaxDivergence"], res[i]["KineticEnergy"]) == (d, ke)
+    b.Close()
+
+
+def test_serve_rejects_bad_input(af):
+    b = af.Batch(af.BatchConfig(MemoryBudget=16 << 20))
+    jobs = small_stream(3, 1, rate=100.0)
+    with pytest.raises(af.AeroFluxError):
+        af.Serve(b, list(reversed(jobs)))  # unsorted arrivals
+    with pytest.raises(af.AeroFluxError):
+        af.Serve(b, [dict(jobs[0], Steps=0)])
+    s = b.Add(af.Config(NX=8, NY=8, NZ=8))
+    with pytest.raises(af.AeroFluxError):
+        af.Serve(b, jobs)  # batch not empty
+    b.Remove(s)
+    res, st = af.Serve(b, [])
+    assert res == [] and st["Steps"] == 0
+    b.Close()
