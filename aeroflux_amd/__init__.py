@@ -68,7 +68,7 @@ class _CConfig(C.Structure):
         ("Precision", C.c_int32), ("MixedMaxCycles", C.c_int32), ("MixedSweeps", C.c_int32),
         ("MixedFixSweeps", C.c_int32), ("MixedTarget", C.c_double),
         ("TileK", C.c_int32), ("TileRows", C.c_int32), ("TileZ", C.c_int32),
-        ("SweepShape", C.c_int32),
+        ("SweepShape", C.c_int32), ("SweepSlab", C.c_int32),
     ]
 
 
@@ -154,6 +154,7 @@ _SIGNATURES = [
                                  C.POINTER(C.c_int32)]),
     ("af_sweep_shape", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]),
+    ("af_sweep_slab", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     ("af_batch_new", C.c_int, [C.POINTER(_CBatchConfig), C.POINTER(C.c_void_p)]),
     ("af_batch_close", C.c_int, [C.c_void_p]),
     ("af_batch_free", None, [C.c_void_p]),
@@ -234,6 +235,7 @@ class Config:
     TileRows: int = 0
     TileZ: int = 0
     SweepShape: int = 0
+    SweepSlab: int = 0
 
     def _to_c(self) -> _CConfig:
         c = _CConfig()
@@ -445,6 +447,11 @@ class Engine:
         s, r, z = C.c_int32(), C.c_int32(), C.c_int32()
         _check(load_library().af_sweep_shape(self._handle(), C.byref(s), C.byref(r), C.byref(z)))
         return int(s.value), int(r.value), int(z.value)
+
+    def sweep_slab(self) -> int:
+        r = C.c_int32()
+        _check(load_library().af_sweep_slab(self._handle(), C.byref(r)))
+        return int(r.value)
 
     def mixed_stats(self) -> tuple[float, int]:
         d = C.c_double()

@@ -4,8 +4,10 @@
 // One launch per sweep covers all three components: threads cover x in
 // [g, g+NX], y in [g, g+NY] and ZR consecutive z in [g, g+NZ], and compute
 // every component face that exists at their coordinates
-// (body::sweep_column).  The workgroup is 64 x BYT threads; the engine's
-// tuner picks (BYT, ZR) per grid.
+// (body::sweep_column).  The workgroup is 64 x BYT threads; blocks run in
+// y-slab order (a face reads the planes above and below it in six fields,
+// more than an XCD's L2 holds for whole planes of a large grid).  The
+// engine's tuner picks (BYT, ZR) and the slab height per grid.
 #include "af_bodies.h"
 #include "af_xcd.h"
 
@@ -20,17 +22,20 @@ template <int AX, int BYT, int ZR>
 __global__ __launch_bounds__(BX *BYT) void k_sweep(SweepArgs a, int3 nb)
 {
     int bx, by, bz;
-    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    xcd_block3_slab(nb.x, nb.y, nb.z, a.slab, bx, by, bz);
     const int g = a.G.g;
     body::sweep_column<AX, ZR>(a, g + bx * BX + threadIdx.x, g + by * BYT + threadIdx.y,
                                g + bz * ZR);
 }
 
 template <int BYT, int ZRT>
-hipError_t launch_shape(int axis, const SweepArgs &a, hipStream_t s)
+hipError_t launch_shape(int axis, SweepArgs a, hipStream_t s)
 {
     const int3 nb = make_int3((a.G.NX + 1 + BX - 1) / BX, (a.G.NY + 1 + BYT - 1) / BYT,
                               (a.G.NZ + 1 + ZRT - 1) / ZRT);
+    // Slab height in block rows (at least one).
+    a.slab = a.slab >= nb.y * BYT ? nb.y : (a.slab + BYT - 1) / BYT;
+    if (a.slab < 1) a.slab = 1;
     const dim3 grid((unsigned)(nb.x * nb.y * nb.z));
     const dim3 block(BX, BYT);
     switch (axis) {

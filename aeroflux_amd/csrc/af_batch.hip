@@ -79,8 +79,14 @@ struct BCtl {
     double *last;
 };
 
+// Tiles of the plane-wise stages flatten each xy-plane over a workgroup's
+// threads, so short rows (small grids) keep every lane busy: a tile is
+// `threads` consecutive points of one plane (x fastest) for one z-block.
+inline int64_t plane_tiles(int64_t plane, int threads) { return (plane + threads - 1) / threads; }
+
 inline int64_t kind_items(int kind, const Geom &G, const DField &P, int shape)
 {
+    const int64_t nxh = (G.NX + 1) / 2;
     switch (kind) {
     case K_VBC: {
         int64_t cu, cv, cw;
@@ -89,19 +95,43 @@ inline int64_t kind_items(int kind, const Geom &G, const DField &P, int shape)
     }
     case K_SWEEP: {
         const int r = kShapeRows[shape], z = kShapeZ[shape];
-        return (int64_t)((G.NX + BX) / BX) * ((G.NY + r) / r) * ((G.NZ + z) / z);
+        return plane_tiles((int64_t)(G.NX + 1) * (G.NY + 1), BX * r) * ((G.NZ + z) / z);
     }
-    case K_CELL: return (int64_t)((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY) * G.NZ;
-    case K_GRAD: return (int64_t)((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY) * (G.NZ + 1);
-    case K_PASS: {
-        const int nxh = (G.NX + 1) / 2;
-        return (int64_t)((nxh + BX - 1) / BX) * ((G.NY + BY - 1) / BY) * ((G.NZ + PZ - 1) / PZ);
-    }
+    case K_CELL: return plane_tiles((int64_t)G.NX * G.NY, BX * BY) * G.NZ;
+    case K_GRAD: return plane_tiles((int64_t)G.NX * G.NY, BX * BY) * (G.NZ + 1);
+    case K_PASS: return plane_tiles(nxh * G.NY, BX * BY) * ((G.NZ + PZ - 1) / PZ);
     case K_PBC:
         return (body::pressure_shell_count(P.nx, P.ny, P.nz, G.g) + CHUNK - 1) / CHUNK;
-    case K_COL: return (int64_t)((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY);
+    case K_COL: return plane_tiles((int64_t)G.NX * G.NY, BX * BY);
     default: return 1;
     }
+}
+
+// q = j / n, r = j % n for 0 <= j < 2^24, n >= 1: a float estimate of the
+// quotient is off by at most one, fixed by one correction step.
+__device__ __forceinline__ void divmod_small(int j, int n, int &q, int &r)
+{
+    q = (int)((float)j * (1.0f / (float)n));
+    r = j - q * n;
+    if (r < 0) {
+        q--;
+        r += n;
+    } else if (r >= n) {
+        q++;
+        r -= n;
+    }
+}
+
+// Point (x, y) of tile `t` of a plane-wise stage: `row` points per row,
+// `plane` points per plane, `threads` per tile; returns the z-block.
+__device__ __forceinline__ int plane_point(int t, int row, int plane, int threads, int &x, int &y)
+{
+    const int nchunk = (plane + threads - 1) / threads;
+    int zb, chunk;
+    divmod_small(t, nchunk, zb, chunk);
+    const int j = chunk * threads + (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    divmod_small(j, row, y, x);
+    return zb;
 }
 
 // Persistent loop over the work-list of stage KIND.  XCD x (= blockIdx.x %
@@ -168,11 +198,11 @@ __global__ __launch_bounds__(BX *BYT) void kb_sweep(BCtl c, int sw)
         a.invH = e.invH;
         a.lam = e.lam_half;
         a.flag = c.flags + s;
-        const int nbx = (e.G.NX + BX) / BX, nby = (e.G.NY + BYT) / BYT;
-        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        const int row = e.G.NX + 1;
+        int x, y;
+        const int zb = plane_point(t, row, row * (e.G.NY + 1), BX * BYT, x, y);
         const int g = e.G.g;
-        body::sweep_column<AX, ZR>(a, g + bx * BX + threadIdx.x, g + by * BYT + threadIdx.y,
-                                   g + bz * ZR);
+        body::sweep_column<AX, ZR>(a, g + x, g + y, g + zb * ZR);
     });
 }
 
@@ -185,9 +215,8 @@ __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
     for_items<K_CELL>(c, [&](int s, int t) {
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
-        const int nbx = (G.NX + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
-        const int bx = t % nbx, r = t / nbx, by = r % nby, zi = r / nby;
-        const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y;
+        int xi, yi;
+        const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
         double v;
         if (!body::divergence_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2),
                                    e.f[AF_FIELD_DIV], G, e.div_scale, xi + G.g, yi + G.g,
@@ -218,10 +247,9 @@ __global__ __launch_bounds__(BX *BY) void kb_pass(BCtl c, int color, int gated)
         a.rhs_black = static_cast<const T *>(e.d_black);
         a.NX = G.NX; a.NY = G.NY; a.NZ = G.NZ; a.NXH = (G.NX + 1) / 2; a.g = G.g;
         a.h2 = e.h2;
-        const int nbx = (a.NXH + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
-        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
-        body::split_pass_column<T, MODE, PZ>(a, color, bx * BX + threadIdx.x,
-                                             by * BY + threadIdx.y, bz * PZ);
+        int k, yi;
+        const int zb = plane_point(t, a.NXH, a.NXH * G.NY, BX * BY, k, yi);
+        body::split_pass_column<T, MODE, PZ>(a, color, k, yi, zb * PZ);
     });
 }
 
@@ -234,12 +262,11 @@ __global__ __launch_bounds__(BX *BY) void kb_from_split(BCtl c, int gated)
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
         const DField &P = e.f[AF_FIELD_P];
-        const int nbx = (G.NX + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
-        const int bx = t % nbx, r = t / nbx, by = r % nby, zi = r / nby;
+        int xi, yi;
+        const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
         body::from_split_cell<TS, ADD>(P.d, P.sy, P.sz, G.g, G.NX, G.NY, (G.NX + 1) / 2,
                                        static_cast<const TS *>(e.s_red),
-                                       static_cast<const TS *>(e.s_black), bx * BX + threadIdx.x,
-                                       by * BY + threadIdx.y, zi);
+                                       static_cast<const TS *>(e.s_black), xi, yi, zi);
     });
 }
 
@@ -262,11 +289,10 @@ __global__ __launch_bounds__(BX *BY) void kb_grad(BCtl c, int sw)
     for_items<K_GRAD>(c, [&](int s, int t) {
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
-        const int nbx = (G.NX + BX - 1) / BX, nby = (G.NY + BY - 1) / BY;
-        const int bx = t % nbx, r = t / nbx, by = r % nby, bz = r / nby;
+        int xi, yi;
+        const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
         body::gradient_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2), e.f[AF_FIELD_P], G,
-                            e.grad_scale, G.g + bx * BX + threadIdx.x, G.g + by * BY + threadIdx.y,
-                            G.g + bz);
+                            e.grad_scale, G.g + xi, G.g + yi, G.g + zi);
     });
 }
 
@@ -278,11 +304,11 @@ __global__ __launch_bounds__(BX *BY) void kb_resid(BCtl c, int gated)
         if (gated && c.done[s]) return;
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
-        const int nbx = (G.NX + BX - 1) / BX;
-        const int bx = t % nbx, by = t / nbx;
-        const double m = body::residual_column(
-            e.f[AF_FIELD_P], e.f[AF_FIELD_DIV].d, static_cast<float *>(e.d_red),
-            static_cast<float *>(e.d_black), G, e.h2, bx * BX + threadIdx.x, by * BY + threadIdx.y);
+        int xi, yi;
+        plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
+        const double m = body::residual_column(e.f[AF_FIELD_P], e.f[AF_FIELD_DIV].d,
+                                               static_cast<float *>(e.d_red),
+                                               static_cast<float *>(e.d_black), G, e.h2, xi, yi);
         const double b = body::block_max256(m, sh);
         if (threadIdx.x == 0 && threadIdx.y == 0) e.partials[t] = b;
     });
@@ -296,7 +322,7 @@ __global__ __launch_bounds__(256) void kb_res_final(BCtl c, int gated)
     for_items<K_ENG>(c, [&](int s, int) {
         if (gated && c.done[s]) return;
         const BSlot &e = c.slots[s];
-        const int np = ((e.G.NX + BX - 1) / BX) * ((e.G.NY + BY - 1) / BY);
+        const int np = (e.G.NX * e.G.NY + BX * BY - 1) / (BX * BY);
         double v = 0;
         for (int k = threadIdx.x; k < np; k += 256) v = e.partials[k] > v ? e.partials[k] : v;
         const double m = body::block_max256(v, sh);

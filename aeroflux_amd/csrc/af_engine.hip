@@ -133,6 +133,7 @@ struct af_engine {
     double *ps_red = nullptr, *ps_black = nullptr, *ds_red = nullptr, *ds_black = nullptr;
     Tile tile64;                // red-black schedule (see Tile)
     int sweep_shape = 4;        // advection workgroup shape (tuned at New)
+    int sweep_slab = 1 << 30;   // advection y-slab rows of the block order (tuned at New)
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
     ICScratch ic;               // initial-condition generator scratch (lazy)
@@ -192,6 +193,7 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
     a.lam = e->cfg.Viscosity * dt * e->invH * e->invH;
     a.flag = e->flag;
     a.shape = e->sweep_shape;
+    a.slab = e->sweep_slab;
     return launch_sweep(axis, a, e->stream);
 }
 
@@ -486,58 +488,75 @@ int tune_blocked(af_engine *e, int k, Tile *out)
     return AF_OK;
 }
 
-// Advection workgroup shape: every shape gives the same bits (faces are
-// independent), so the fastest on this grid is kept.  Timed on the engine's
-// own (zeroed) arena, X sweep; cached per grid for the process.
-std::map<std::tuple<int, int, int, int>, int> g_sweep_cache;
+// Advection workgroup shape and y-slab height of the block order: every
+// choice gives the same bits (faces are independent), so the fastest on this
+// grid is kept.  Timed on the engine's own (zeroed) arena, X sweep; cached
+// per grid for the process.
+std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_sweep_cache;
+
+constexpr int kPlaneOrder = 1 << 30;
 
 int choose_sweep_shape(af_engine *e)
 {
     const af_config &c = e->cfg;
-    if (c.SweepShape > 0) {
-        if (c.SweepShape > sweep_shape_count())
-            return fail(AF_ERR_CONFIG, "aeroflux: sweep shape %d out of range 1..%d",
-                        (int)c.SweepShape, sweep_shape_count());
-        e->sweep_shape = c.SweepShape - 1;
-        return AF_OK;
-    }
+    if (c.SweepShape > sweep_shape_count() || c.SweepShape < 0)
+        return fail(AF_ERR_CONFIG, "aeroflux: sweep shape %d out of range 1..%d",
+                    (int)c.SweepShape, sweep_shape_count());
+    if (c.SweepSlab < -1)
+        return fail(AF_ERR_CONFIG, "aeroflux: sweep slab %d must be -1, 0 or positive",
+                    (int)c.SweepSlab);
+    if (c.SweepShape > 0) e->sweep_shape = c.SweepShape - 1;
+    if (c.SweepSlab != 0) e->sweep_slab = c.SweepSlab < 0 ? kPlaneOrder : (int)c.SweepSlab;
+    if (c.SweepShape > 0 && c.SweepSlab != 0) return AF_OK;
     const Geom &G = e->G;
     const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_sweep_cache.find(key);
-        if (it != g_sweep_cache.end()) {
-            e->sweep_shape = it->second;
+        if (it != g_sweep_cache.end() && c.SweepShape == 0 && c.SweepSlab == 0) {
+            e->sweep_shape = it->second.first;
+            e->sweep_slab = it->second.second;
             return AF_OK;
         }
     }
+    static const int kSlabs[] = {kPlaneOrder, 8, 16, 32, 64};
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a), "create event");
     HIP_TRY(hipEventCreate(&b), "create event");
     float best = 1e30f;
-    int bs = e->sweep_shape;
+    int bs = e->sweep_shape, bl = e->sweep_slab;
     hipError_t err = hipSuccess;
     for (int s = 0; s < sweep_shape_count() && err == hipSuccess; s++) {
-        e->sweep_shape = s;
-        err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
-        if (err == hipSuccess) err = hipEventRecord(a, e->stream);
-        for (int r = 0; r < 3 && err == hipSuccess; r++)
+        if (c.SweepShape > 0 && s != c.SweepShape - 1) continue;
+        for (int sl : kSlabs) {
+            if (c.SweepSlab != 0 && sl != e->sweep_slab) continue;
+            if (err != hipSuccess) break;
+            e->sweep_shape = s;
+            e->sweep_slab = sl;
             err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
-        if (err == hipSuccess) err = hipEventRecord(b, e->stream);
-        if (err == hipSuccess) err = hipEventSynchronize(b);
-        float ms = 0;
-        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
-        if (err == hipSuccess && ms < best) {
-            best = ms;
-            bs = s;
+            if (err == hipSuccess) err = hipEventRecord(a, e->stream);
+            for (int r = 0; r < 3 && err == hipSuccess; r++)
+                err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
+            if (err == hipSuccess) err = hipEventRecord(b, e->stream);
+            if (err == hipSuccess) err = hipEventSynchronize(b);
+            float ms = 0;
+            if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+            if (err == hipSuccess && ms < best) {
+                best = ms;
+                bs = s;
+                bl = sl;
+            }
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (err != hipSuccess) return hip_fail(err, "tune advection workgroup shape");
     e->sweep_shape = bs;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_sweep_cache[key] = bs;
+    e->sweep_slab = bl;
+    if (c.SweepShape == 0 && c.SweepSlab == 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_sweep_cache[key] = std::make_pair(bs, bl);
+    }
     return AF_OK;
 }
 
@@ -1050,6 +1069,13 @@ int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *z
     if (shape) *shape = e->sweep_shape;
     if (rows) *rows = kRows[e->sweep_shape];
     if (zper) *zper = kZ[e->sweep_shape];
+    return AF_OK;
+}
+
+int af_sweep_slab(const af_engine *e, int32_t *rows)
+{
+    if (!e || !rows) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *rows = e->sweep_slab >= (1 << 30) ? -1 : e->sweep_slab;
     return AF_OK;
 }
 

@@ -27,6 +27,7 @@ struct SweepArgs {
     double dt, invH, lam;
     int *flag;             // set to 1 on a NaN departure coordinate
     int shape = 4;         // workgroup shape, 0 .. sweep_shape_count()-1
+    int slab = 1 << 30;    // rows per y-slab of the block order (>= NY+1: plane order)
 };
 
 // Number of sweep workgroup shapes (rows per block x z per thread).

@@ -36,4 +36,25 @@ __device__ __forceinline__ void xcd_block3(int nbx, int nby, int nbz, int &bx, i
     bz = r / nby;
 }
 
+// The same grid in y-slab order: slabs of `sl` block rows, each walked plane
+// by plane (z), then row by row, then x.  A stencil that reads the planes
+// above and below keeps them in its XCD's L2 while it works through the
+// slab, even when whole planes of every field it touches would not fit;
+// sl >= nby is the plane order of xcd_block3.
+__device__ __forceinline__ void xcd_block3_slab(int nbx, int nby, int nbz, int sl, int &bx,
+                                                int &by, int &bz)
+{
+    const int l = xcd_logical_block(nbx * nby * nbz);
+    sl = sl < nby ? sl : nby;
+    const int full = sl * nbz * nbx;  // blocks of a full slab; only the last is partial
+    const int s = l / full;
+    int r = l - s * full;
+    const int rows = (nby - s * sl) < sl ? (nby - s * sl) : sl;
+    const int per_z = rows * nbx;
+    bz = r / per_z;
+    r -= bz * per_z;
+    by = s * sl + r / nbx;
+    bx = r - (r / nbx) * nbx;
+}
+
 }  // namespace af

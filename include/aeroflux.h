@@ -97,6 +97,8 @@ typedef struct af_config {
     int32_t TileZ;            /* blocked: z-planes per workgroup (0 -> tuned) */
     int32_t SweepShape;       /* advection workgroup shape: 0 -> tuned, k ->
                                  shape k-1 (see af_sweep_shape) */
+    int32_t SweepSlab;        /* advection block order: 0 -> tuned, -1 ->
+                                 plane by plane, k -> y-slabs of k rows */
 } af_config;
 
 /* Field, grid.go:12-21 (descriptor after any commitVelocity swaps). */
@@ -215,6 +217,10 @@ int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchun
  * the engine's grid at New unless SweepShape fixes it.  No reference
  * counterpart. */
 int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper);
+
+/* Rows per y-slab of the advection block order in use (-1: plane by plane).
+ * No reference counterpart. */
+int af_sweep_slab(const af_engine *e, int32_t *rows);
 
 /* ------------------------------------------------------------- batching */
 /* Many engines of uneven sizes stepped together.  No reference counterpart:
