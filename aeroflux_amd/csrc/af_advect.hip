@@ -48,11 +48,14 @@ hipError_t launch_shape(int axis, SweepArgs a, hipStream_t s)
 
 }  // namespace
 
-int sweep_shape_count() { return 8; }
+int sweep_shape_count() { return 8 + sweep_lds_variant_count(); }
 
-// Workgroup shapes the tuner chooses from: (rows per block, z per thread).
+// Shapes the tuner chooses from: 0-7 gather from global memory with
+// workgroups of (rows per block, z per thread); 8 and up stage the sources
+// in LDS (af_advect_lds.hip).
 hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s)
 {
+    if (a.shape >= 8) return launch_sweep_lds(axis, a, a.shape - 8, s);
     switch (a.shape) {
     case 0: return launch_shape<1, 1>(axis, a, s);
     case 1: return launch_shape<1, 2>(axis, a, s);

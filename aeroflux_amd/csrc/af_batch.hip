@@ -602,9 +602,12 @@ hipError_t enq_pass(const af_batch *b, int nwg, int color, int mode, int gated)
     return hipGetLastError();
 }
 
+// Shell and per-slot stages have few items: a smaller persistent grid.
+constexpr int kSmallGrid = 1024;
+
 hipError_t enq_vbc(const af_batch *b, int sw)
 {
-    hipLaunchKernelGGL(kb_vbc, dim3(b->nwg_other), dim3(256), 0, b->stream, ctl(b), sw);
+    hipLaunchKernelGGL(kb_vbc, dim3(kSmallGrid), dim3(256), 0, b->stream, ctl(b), sw);
     return hipGetLastError();
 }
 
@@ -640,26 +643,27 @@ hipError_t enq_solve(const af_batch *b)
         hipLaunchKernelGGL((kb_from_split<double, false>), gp, blk, 0, b->stream, c, 0);
     } else {
         const bool checked = b->cfg.MixedChecked != 0;
-        hipLaunchKernelGGL(kb_mixed_reset, gp, dim3(256), 0, b->stream, c, checked ? 1 : 0);
+        hipLaunchKernelGGL(kb_mixed_reset, dim3(kSmallGrid), dim3(256), 0, b->stream, c,
+                           checked ? 1 : 0);
         for (int p = 0; p < 2 * b->cfg.MixedSweeps && err == hipSuccess; p++)
             err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 0);
         if (err != hipSuccess) return err;
         hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, c, 0);
         if (checked) {
             hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 0);
-            hipLaunchKernelGGL(kb_res_final, gp, dim3(256), 0, b->stream, c, 0);
+            hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 0);
             for (int cyc = 1; cyc < b->cfg.MixedMaxCycles; cyc++) {
                 for (int p = 0; p < 2 * b->cfg.MixedFixSweeps && err == hipSuccess; p++)
                     err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 1);
                 if (err != hipSuccess) return err;
                 hipLaunchKernelGGL((kb_from_split<float, true>), gp, blk, 0, b->stream, c, 1);
                 hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 1);
-                hipLaunchKernelGGL(kb_res_final, gp, dim3(256), 0, b->stream, c, 1);
+                hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 1);
             }
         }
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    hipLaunchKernelGGL(kb_pbc, gp, dim3(256), 0, b->stream, c);
+    hipLaunchKernelGGL(kb_pbc, dim3(kSmallGrid), dim3(256), 0, b->stream, c);
     return hipGetLastError();
 }
 

@@ -158,49 +158,78 @@ __device__ __forceinline__ double2 ld2(const double *p)
     return v;
 }
 
+// Where a sweep's values come from.  A loader returns component c's value
+// at padded coordinates (x, y, z) (`one`) or the x-adjacent pair at x, x+1
+// (`pair`).  GlobalLoad reads the fields in HBM; the LDS-staged sweep
+// (af_advect_lds.hip) supplies a loader that serves staged values from LDS.
+// Both feed the same arithmetic below, so both give the same bits.
+struct GlobalLoad {
+    const SweepArgs &a;
+    __device__ __forceinline__ const DField &f(int c) const
+    {
+        return c == 0 ? a.U : (c == 1 ? a.V : a.W);
+    }
+    __device__ __forceinline__ double one(int c, int x, int y, int z) const
+    {
+        const DField &s = f(c);
+        return s.d[idx(s, x, y, z)];
+    }
+    __device__ __forceinline__ double2 pair(int c, int x, int y, int z) const
+    {
+        const DField &s = f(c);
+        return ld2(s.d + idx(s, x, y, z));
+    }
+    // The eight trilinear taps at base (x, y, z), as four x-adjacent pairs.
+    __device__ __forceinline__ void taps(int c, int x, int y, int z, double2 &p00, double2 &p10,
+                                         double2 &p01, double2 &p11) const
+    {
+        const DField &s = f(c);
+        const double *d = s.d + idx(s, x, y, z);
+        p00 = ld2(d);
+        p10 = ld2(d + s.sy);
+        p01 = ld2(d + s.sz);
+        p11 = ld2(d + s.sz + s.sy);
+    }
+};
+
 // The velocity component along sweep axis AX at a face of component C, as
 // faceVelocity (stencil.go:128-155) forms it (same four-point order).
-template <int AX, int C>
-__device__ __forceinline__ double axis_velocity(const SweepArgs &a, int x, int y, int z)
+template <int AX, int C, class L>
+__device__ __forceinline__ double axis_velocity(const L &ld, int x, int y, int z)
 {
-    if (AX == C) {
-        const DField &f = C == 0 ? a.U : (C == 1 ? a.V : a.W);
-        return f.d[idx(f, x, y, z)];
-    }
+    if (AX == C) return ld.one(C, x, y, z);
     if (AX == 0) {
-        const double *u = a.U.d;
-        const double2 lo = ld2(u + (C == 1 ? idx(a.U, x, y - 1, z) : idx(a.U, x, y, z - 1)));
-        const double2 hi = ld2(u + idx(a.U, x, y, z));
+        const double2 lo = C == 1 ? ld.pair(0, x, y - 1, z) : ld.pair(0, x, y, z - 1);
+        const double2 hi = ld.pair(0, x, y, z);
         return 0.25 * (lo.x + lo.y + hi.x + hi.y);
     }
     if (AX == 1) {
-        const double *v = a.V.d;
         if (C == 0) {
-            const double2 lo = ld2(v + idx(a.V, x - 1, y, z));
-            const double2 hi = ld2(v + idx(a.V, x - 1, y + 1, z));
+            const double2 lo = ld.pair(1, x - 1, y, z);
+            const double2 hi = ld.pair(1, x - 1, y + 1, z);
             return 0.25 * (lo.x + lo.y + hi.x + hi.y);
         }
-        return 0.25 * (v[idx(a.V, x, y, z - 1)] + v[idx(a.V, x, y + 1, z - 1)] +
-                       v[idx(a.V, x, y, z)] + v[idx(a.V, x, y + 1, z)]);
+        return 0.25 * (ld.one(1, x, y, z - 1) + ld.one(1, x, y + 1, z - 1) + ld.one(1, x, y, z) +
+                       ld.one(1, x, y + 1, z));
     }
-    const double *w = a.W.d;
     if (C == 0) {
-        const double2 lo = ld2(w + idx(a.W, x - 1, y, z));
-        const double2 hi = ld2(w + idx(a.W, x - 1, y, z + 1));
+        const double2 lo = ld.pair(2, x - 1, y, z);
+        const double2 hi = ld.pair(2, x - 1, y, z + 1);
         return 0.25 * (lo.x + lo.y + hi.x + hi.y);
     }
-    return 0.25 * (w[idx(a.W, x, y - 1, z)] + w[idx(a.W, x, y, z)] +
-                   w[idx(a.W, x, y - 1, z + 1)] + w[idx(a.W, x, y, z + 1)]);
+    return 0.25 * (ld.one(2, x, y - 1, z) + ld.one(2, x, y, z) + ld.one(2, x, y - 1, z + 1) +
+                   ld.one(2, x, y, z + 1));
 }
 
 // advectComponent's value at one face (stencil.go:83-115).  `bad` is set
 // when the departure coordinate is NaN -- the reference's bounds-panic
 // trigger; the value is then NaN and no load leaves the field.
-template <int AX, int C>
-__device__ __forceinline__ double advect_value(const SweepArgs &a, int x, int y, int z, bool &bad)
+template <int AX, int C, class L>
+__device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, int x, int y, int z,
+                                               bool &bad)
 {
     const DField &s = C == 0 ? a.U : (C == 1 ? a.V : a.W);
-    const double v = axis_velocity<AX, C>(a, x, y, z);
+    const double v = axis_velocity<AX, C>(ld, x, y, z);
     const int coord = AX == 0 ? x : (AX == 1 ? y : z);
     const int ext = AX == 0 ? s.nx : (AX == 1 ? s.ny : s.nz);
     double p = (double)coord;
@@ -219,12 +248,8 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, int x, int y,
     const double tx = AX == 0 ? t : 0.0;
     const double ty = AX == 1 ? t : 0.0;
     const double tz = AX == 2 ? t : 0.0;
-    const double *d = s.d;
-    const int i000 = idx(s, x0, y0, z0);
-    const double2 p00 = ld2(d + i000);                // d000, d100
-    const double2 p10 = ld2(d + i000 + s.sy);         // d010, d110
-    const double2 p01 = ld2(d + i000 + s.sz);         // d001, d101
-    const double2 p11 = ld2(d + i000 + s.sz + s.sy);  // d011, d111
+    double2 p00, p10, p01, p11;  // (d000, d100), (d010, d110), (d001, d101), (d011, d111)
+    ld.taps(C, x0, y0, z0, p00, p10, p01, p11);
     const double c00 = p00.x * (1 - tx) + p00.y * tx;
     const double c10 = p10.x * (1 - tx) + p10.y * tx;
     const double c01 = p01.x * (1 - tx) + p01.y * tx;
@@ -233,18 +258,20 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, int x, int y,
     const double c1 = c01 * (1 - ty) + c11 * ty;
     const double adv = c0 * (1 - tz) + c1 * tz;
 
-    const int i = idx(s, x, y, z);
-    double dm, dc, dp;  // d[i - st], d[i], d[i + st]
+    double dm, dc, dp;  // src at i - st, i, i + st
     if (AX == 0) {
-        const double2 q = ld2(d + i - 1);
+        const double2 q = ld.pair(C, x - 1, y, z);
         dm = q.x;
         dc = q.y;
-        dp = d[i + 1];
+        dp = ld.one(C, x + 1, y, z);
+    } else if (AX == 1) {
+        dm = ld.one(C, x, y - 1, z);
+        dc = ld.one(C, x, y, z);
+        dp = ld.one(C, x, y + 1, z);
     } else {
-        const int st = AX == 1 ? s.sy : s.sz;
-        dm = d[i - st];
-        dc = d[i];
-        dp = d[i + st];
+        dm = ld.one(C, x, y, z - 1);
+        dc = ld.one(C, x, y, z);
+        dp = ld.one(C, x, y, z + 1);
     }
     double lap = a.lam * (dm - 2 * dc + dp);
     lap = a.lam != 0 ? lap : 0.0;  // `var lap float64` stays +0 when lam == 0
@@ -270,9 +297,10 @@ __device__ __forceinline__ void sweep_column(const SweepArgs &a, int x, int y, i
         dv[r] = xin && zin;
         dw[r] = xin && yin && zok;
         vu[r] = vv[r] = vw[r] = 0;
-        if (du[r]) vu[r] = advect_value<AX, 0>(a, x, y, z, bad);
-        if (dv[r]) vv[r] = advect_value<AX, 1>(a, x, y, z, bad);
-        if (dw[r]) vw[r] = advect_value<AX, 2>(a, x, y, z, bad);
+        const GlobalLoad ld{a};
+        if (du[r]) vu[r] = advect_value<AX, 0>(a, ld, x, y, z, bad);
+        if (dv[r]) vv[r] = advect_value<AX, 1>(a, ld, x, y, z, bad);
+        if (dw[r]) vw[r] = advect_value<AX, 2>(a, ld, x, y, z, bad);
     }
 #pragma unroll
     for (int r = 0; r < ZR; r++) {

@@ -490,8 +490,9 @@ int tune_blocked(af_engine *e, int k, Tile *out)
 
 // Advection workgroup shape and y-slab height of the block order: every
 // choice gives the same bits (faces are independent), so the fastest on this
-// grid is kept.  Timed on the engine's own (zeroed) arena, X sweep; cached
-// per grid for the process.
+// grid is kept.  Timed on the engine's own (zeroed) arena, X, Y and Z sweeps
+// twice each (the sweeps only write the Tmp fields, which are still zero
+// afterwards: advecting zeros gives +0); cached per grid for the process.
 std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_sweep_cache;
 
 constexpr int kPlaneOrder = 1 << 30;
@@ -530,13 +531,15 @@ int choose_sweep_shape(af_engine *e)
         if (c.SweepShape > 0 && s != c.SweepShape - 1) continue;
         for (int sl : kSlabs) {
             if (c.SweepSlab != 0 && sl != e->sweep_slab) continue;
+            if (s >= 8 && c.SweepSlab == 0 && sl != kPlaneOrder) continue;  // no slab order
             if (err != hipSuccess) break;
             e->sweep_shape = s;
             e->sweep_slab = sl;
             err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
             if (err == hipSuccess) err = hipEventRecord(a, e->stream);
-            for (int r = 0; r < 3 && err == hipSuccess; r++)
-                err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
+            for (int r = 0; r < 2 && err == hipSuccess; r++)
+                for (int ax = 0; ax < 3 && err == hipSuccess; ax++)
+                    err = enq_sweep(e, e->cur, ax, e->cfg.DT * 0.5);
             if (err == hipSuccess) err = hipEventRecord(b, e->stream);
             if (err == hipSuccess) err = hipEventSynchronize(b);
             float ms = 0;
@@ -1064,8 +1067,9 @@ int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchun
 int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper)
 {
     if (!e) return fail(AF_ERR_ARG, "aeroflux: null engine");
-    static const int kRows[] = {1, 1, 2, 2, 4, 4, 8, 16};
-    static const int kZ[] = {1, 2, 1, 2, 1, 2, 1, 1};
+    // Gathering shapes: rows x z per thread; LDS-staged: rows x z-chunk.
+    static const int kRows[] = {1, 1, 2, 2, 4, 4, 8, 16, 4, 4, 8, 8};
+    static const int kZ[] = {1, 2, 1, 2, 1, 2, 1, 1, 16, 32, 16, 32};
     if (shape) *shape = e->sweep_shape;
     if (rows) *rows = kRows[e->sweep_shape];
     if (zper) *zper = kZ[e->sweep_shape];

@@ -30,8 +30,11 @@ struct SweepArgs {
     int slab = 1 << 30;    // rows per y-slab of the block order (>= NY+1: plane order)
 };
 
-// Number of sweep workgroup shapes (rows per block x z per thread).
+// Number of sweep shapes: gathering shapes (rows per block x z per thread)
+// followed by the LDS-staged variants.
 int sweep_shape_count();
+int sweep_lds_variant_count();
+hipError_t launch_sweep_lds(int axis, const SweepArgs &a, int variant, hipStream_t s);
 
 // Launchers.  All enqueue on `s` and return the hipGetLastError() code.
 hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t s);

@@ -410,7 +410,7 @@ def test_mixed_precision_is_reproducible(af):
     assert outs[0] == outs[1]
 
 
-@pytest.mark.parametrize("shape", range(1, 9))
+@pytest.mark.parametrize("shape", range(1, 13))
 def test_step_bitwise_across_sweep_shapes(af, oracle_lib, shape):
     """Every advection workgroup shape gives the reference's bits."""
     cfg = dict(NX=20, NY=18, NZ=14, H=1 / 20, DT=0.01, Viscosity=1e-4, GaussSeidelIters=6)
@@ -424,4 +424,34 @@ def test_step_bitwise_across_sweep_shapes(af, oracle_lib, shape):
     for s in range(2):
         ge.Step(); assert oe.step() == 0
     assert_bitwise(ge, oe, range(8), f"sweep shape {shape}")
+    ge.Close()
+
+
+LDS_SHAPES = range(9, 13)  # the LDS-staged sweep variants (SweepShape 9..12)
+
+
+@pytest.mark.parametrize("shape", LDS_SHAPES)
+@pytest.mark.parametrize("shp", [(70, 9, 11, 1), (5, 4, 3, 2), (64, 12, 37, 1)])
+@pytest.mark.parametrize("ax", [0, 1, 2])
+@pytest.mark.parametrize("reach", ["staged", "beyond"])
+def test_lds_sweep_bitwise(af, oracle_lib, shape, shp, ax, reach):
+    """The LDS-staged sweep serves taps from its staged box when the
+    departure moves at most two cells and from global memory beyond; both
+    give the reference's bits, as do the wall column and partial tiles
+    (x extents 70 and 64 leave a partial or no partial 64-wide tile)."""
+    nx, ny, nz, g = shp
+    h = 1.0 / max(nx, ny, nz)
+    dt = 0.5 * h
+    # "staged": |u| * dt/2 / h <= 0.75 cells; "beyond": up to ~6 cells, clamps.
+    f = random_fields(shp, 40 + ax, scale=0.5 if reach == "staged" else 4.0)
+    kw = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, DT=dt, Viscosity=0.1 * h * h / dt)
+    ge = af.New(af.Config(**kw, SweepShape=shape))
+    assert ge.sweep_shape()[0] == shape - 1
+    oe = oracle_lib.OracleEngine(dict(kw, Workers=4))
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        ge.Mesh.SetData(ge.Mesh.field(fid), f[name])
+        oe.field(fid)[...] = f[name]
+    ge.sweep_axis(ax, dt * 0.5)
+    assert oe.sweep_axis(ax, dt * 0.5) == 0
+    assert_bitwise(ge, oe, (5, 6, 7), f"lds sweep shape {shape} ax={ax}")
     ge.Close()
