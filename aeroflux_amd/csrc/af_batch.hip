@@ -57,8 +57,14 @@ enum Kind {
     K_COUNT
 };
 
-constexpr int kShapeRows[8] = {1, 1, 2, 2, 4, 4, 8, 16};
-constexpr int kShapeZ[8] = {1, 2, 1, 2, 1, 2, 1, 1};
+// Sweep shapes: workgroups of BX x rows threads over flattened plane tiles,
+// z positions per thread.  (The single engine's LDS-staged sweep keeps its
+// per-field descriptors in scalar registers as kernel arguments; here they
+// are per slot and spill to vector registers, which costs more than the
+// staging saves, so the batch gathers.)
+constexpr int kShapes = 8;
+constexpr int kShapeRows[kShapes] = {1, 1, 2, 2, 4, 4, 8, 16};
+constexpr int kShapeZ[kShapes] = {1, 2, 1, 2, 1, 2, 1, 1};
 
 // Device descriptor of one slot.
 struct BSlot {
@@ -756,11 +762,13 @@ hipError_t enq_timed_pass(const af_batch *b, int nwg)
     return enq_pass<double>(b, nwg, 0, 0, 0);
 }
 
-// Autotuner: advection workgroup shape, then persistent workgroup counts for
-// the sweeps and the colour passes, each timed on batched launches over the
-// resident slots.  The X sweep writes to the Tmp fields what the next step's
-// first sweep writes; colour passes touch only solve buffers that every
-// solve rewrites before reading.  Results therefore do not depend on tuning.
+// Autotuner: advection shape (timed on X, Y and Z sweeps), then persistent
+// workgroup counts for the sweeps and the colour passes, each timed on
+// batched launches over the resident slots.  A sweep writes only the face
+// cells of the Tmp fields, and the next step's first sweep rewrites every one
+// of them before anything reads them; colour passes touch only solve buffers
+// that every solve rewrites before reading.  Results therefore do not depend
+// on tuning.
 int tune(af_batch *b)
 {
     if (b->nact == 0) return AF_OK;
@@ -770,11 +778,16 @@ int tune(af_batch *b)
     const int wg0 = b->cfg.Workgroups > 0 ? b->cfg.Workgroups : 2048;
     if (b->cfg.SweepShape <= 0) {
         double best = 1e30;
-        for (int s = 0; s < 8; s++) {
+        for (int s = 0; s < kShapes; s++) {
             if (int rc = upload_table(b, s)) return rc;
-            double ms;
-            if (int rc = time_launches(b, reps, [&] { return enq_sweep(b, s, wg0, 0, 0); }, &ms))
-                return rc;
+            double ms = 0;
+            for (int ax = 0; ax < 3; ax++) {
+                double t;
+                if (int rc = time_launches(b, reps, [&] { return enq_sweep(b, s, wg0, ax, 0); },
+                                           &t))
+                    return rc;
+                ms += t;
+            }
             if (ms < best) {
                 best = ms;
                 best_shape = s;
@@ -873,9 +886,9 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
         c.GaussSeidelIters < 1 || c.MixedMaxCycles < 1 || c.MixedSweeps < 1 ||
         c.MixedFixSweeps < 1)
         return fail(AF_ERR_CONFIG, "aeroflux: bad batch capacity, budget or solve schedule");
-    if (c.SweepShape < 0 || c.SweepShape > 8)
-        return fail(AF_ERR_CONFIG, "aeroflux: sweep shape %d out of range 1..8",
-                    (int)c.SweepShape);
+    if (c.SweepShape < 0 || c.SweepShape > kShapes)
+        return fail(AF_ERR_CONFIG, "aeroflux: sweep shape %d out of range 1..%d",
+                    (int)c.SweepShape, kShapes);
     if (c.Workgroups < 0 || c.Workgroups % XCDS != 0)
         return fail(AF_ERR_CONFIG, "aeroflux: persistent workgroups must be a multiple of 8");
 
@@ -1305,6 +1318,9 @@ int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
 
 // ------------------------------------------------------------- serving
 
+// Resident interior cells at which af_serve runs the autotuner.
+constexpr int64_t kServeTuneCells = (int64_t)128 << 20;
+
 int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, double slope,
              double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats)
 {
@@ -1476,8 +1492,13 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
             }
             continue;
         }
-        if (!b->tuned) {
+        // Tune once the resident set is representative: a working set that
+        // fills the GPU, or everything the stream will admit.  Until then the
+        // default shape and grids run.
+        if (!b->tuned && (b->cells >= kServeTuneCells || (next >= n && queue.empty()))) {
             if ((err = hipStreamSynchronize(b->stream)) != hipSuccess) break;
+            while (!pending.empty() && err == hipSuccess) err = finish_oldest();
+            if (err != hipSuccess) break;
             if ((rc = upload_table(b, b->shape)) != AF_OK) break;
             if ((rc = tune(b)) != AF_OK) break;
         }

@@ -302,8 +302,9 @@ int af_batch_kinetic_energy(af_batch *b, int32_t slot, double *out); /* project.
 int af_batch_mixed_stats(af_batch *b, int32_t slot, double *last_div, int32_t *cycles);
 
 /* Runs the autotuner now on the resident slots (it otherwise runs at the
- * first step): advection workgroup shape and persistent workgroup counts,
- * each timed on batched launches.  Results are unaffected. */
+ * first step): advection shape and persistent workgroup counts, each timed
+ * on batched launches.  Results of later steps are unaffected; until the
+ * next step the Tmp fields hold the timing sweeps' output. */
 int af_batch_tune(af_batch *b);
 
 /* Average device time (ms) of `reps` launches of one batched stage over all
