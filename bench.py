@@ -173,7 +173,29 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
             "pool_bytes": info["BytesInUse"],
         }
         b.Close()
+    # The plain FP64 GPU path on the same workload: each engine alone
+    # (New, the same initial condition, StepN), one after the other; device
+    # time only (creation and uploads excluded).
+    el_plain, divs_plain = 0.0, []
+    for e, f in zip(engines, fields):
+        eng = af.New(af.Config(**e))
+        load_ic(eng, f)
+        eng.StepN(warmup)
+        hip_sync()
+        t0 = time.perf_counter()
+        eng.StepN(steps)
+        hip_sync()
+        el_plain += time.perf_counter() - t0
+        divs_plain.append(eng.MaxDivergence())
+        eng.Close()
+    out["plain_fp64_one_engine_at_a_time"] = {
+        "value": cells * steps / el_plain, "unit": "cell-updates/s",
+        "ms_per_step": 1e3 * el_plain / steps,
+        "final_max_divergence_equal_to_fp64_batch": bool(np.array_equal(np.array(divs_plain),
+                                                                        divs["fp64"])),
+    }
     rel = (divs["mixed"] - divs["fp64"]) / divs["fp64"]
+    out["speedup_mixed_vs_plain_fp64"] = out["mixed"]["value"] / (cells * steps / el_plain)
     out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     out["final_max_divergence_rel_diff"] = {"max_abs": float(np.abs(rel).max()),
                                             "mean": float(rel.mean())}
