@@ -356,10 +356,22 @@ class Engine:
         """Step, aeroflux.go:348-409."""
         _check(load_library().af_step(self._handle()))
 
-    def StepN(self, n: int) -> None:
-        """n Steps with one synchronisation (no reference counterpart)."""
+    def StepN(self, n: int, first_fault: bool = False) -> None:
+        """n Steps with one synchronisation (no reference counterpart).  A
+        contained device fault raises AeroFluxError after the n steps.  With
+        ``first_fault`` the flag is read after every step instead (one
+        synchronisation per step, as n calls of Step), the call stops at the
+        first faulting step and the error's ``step`` names it."""
+        lib = load_library()
+        if not first_fault:
+            _check(lib.af_step_n(self._handle(), n, None))
+            return
         first = C.c_int64(-1)
-        _check(load_library().af_step_n(self._handle(), n, C.byref(first)))
+        try:
+            _check(lib.af_step_n(self._handle(), n, C.byref(first)))
+        except AeroFluxError as err:
+            err.step = int(first.value)
+            raise
 
     def _scalar(self, fn) -> float:
         out = C.c_double()

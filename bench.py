@@ -123,9 +123,15 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
     """Batched uneven engines: n_eng engines with each axis drawn from
     {32,48,64,96,128}, H = 1/max(dims), nu log-uniform in [1e-6,1e-4],
     DT = min(0.004, 0.5 H^2/(6 nu)), seeded Taylor-Green + noise per engine.
-    The same engines are stepped by an FP64 batch and by a one-cycle mixed
-    batch (same sweep count); both report cell-updates/s and each engine's
-    final max|div|."""
+    The same engines are stepped by
+      * the plain FP64 GPU path: each engine alone (New, StepN), one after
+        the other;
+      * an FP64 batch (bit-identical to the plain path);
+      * a one-cycle mixed batch (the same 24 sweep pairs in FP32);
+      * a checked mixed batch that iterates every projection until its FP64
+        residual bounds max|div| by that engine's final max|div| on the FP64
+        path (20 FP32 sweep pairs, then checked cycles of 2, at most 4).
+    Each reports cell-updates/s and each engine's final max|div|."""
     from aeroflux_amd import workload
 
     engines = workload.batch_engines(n_eng, seed=3)
@@ -133,14 +139,36 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
               for i, e in enumerate(engines)]
     cells = sum(e["NX"] * e["NY"] * e["NZ"] for e in engines)
     out = {"engines": n_eng, "cells": cells, "steps": steps, "warmup": warmup}
-    divs = {}
-    for name, prec in (("fp64", af.PRECISION_FP64), ("mixed", af.PRECISION_MIXED)):
-        need = sum(af.SlotBytes(af.Config(**e, Precision=prec), prec) for e in engines)
+
+    # Plain FP64 path: device time of StepN only (creation and uploads are
+    # outside the timed region; one synchronisation per engine).
+    el_plain, d_plain = 0.0, []
+    for e, f in zip(engines, fields):
+        eng = af.New(af.Config(**e))
+        load_ic(eng, f)
+        if warmup:
+            eng.StepN(warmup)
+        hip_sync()
+        t0 = time.perf_counter()
+        eng.StepN(steps)
+        hip_sync()
+        el_plain += time.perf_counter() - t0
+        d_plain.append(eng.MaxDivergence())
+        eng.Close()
+    d_plain = np.array(d_plain)
+    plain = cells * steps / el_plain
+    out["plain_fp64"] = {"value": plain, "unit": "cell-updates/s",
+                         "ms_per_step": 1e3 * el_plain / steps,
+                         "how": "each engine alone through New/StepN, one after the other"}
+
+    def run(prec, bc_extra, cfg_extra):
+        need = sum(af.SlotBytes(af.Config(**e, Precision=prec, **cfg_extra(i)), prec)
+                   for i, e in enumerate(engines))
         b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=need + (64 << 20),
-                                    MaxSlots=max(n_eng, 8)))
+                                    MaxSlots=max(n_eng, 8), **bc_extra))
         slots = []
-        for e, f in zip(engines, fields):
-            s = b.Add(af.Config(**e, Precision=prec))
+        for i, (e, f) in enumerate(zip(engines, fields)):
+            s = b.Add(af.Config(**e, Precision=prec, **cfg_extra(i)))
             for fid, key in ((1, "U"), (2, "V"), (3, "W")):
                 b.SetData(s, fid, f[key])
             b.apply_velocity_bc(s)
@@ -153,52 +181,49 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
         b.StepN(steps)
         hip_sync()
         el = time.perf_counter() - t0
-        divs[name] = np.array([b.MaxDivergence(s) for s in slots])
+        d = np.array([b.MaxDivergence(s) for s in slots])
         t_pass = b.time_stage(0, 20)
-        stage = {"rbgs_pass_ms": t_pass, "advect_sweep_ms": b.time_stage(1, 5)}
         info = b.info()
         pass_bytes = (6 if prec == af.PRECISION_MIXED else 12) * cells
-        out[name] = {
+        res = {
             "value": cells * steps / el,
             "unit": "cell-updates/s",
             "ms_per_step": 1e3 * el / steps,
+            "speedup_vs_plain_fp64": cells * steps / el / plain,
             "hbm_gbps_model": bytes_per_cell_step(24) * cells * steps / el / 1e9,
             "pass_roofline": {"bound": "hbm", "achieved": pass_bytes / (t_pass * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": pass_bytes / (t_pass * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                               "algorithmic_bytes_per_launch": pass_bytes},
-            "stages": stage,
+            "stages": {"rbgs_pass_ms": t_pass, "advect_sweep_ms": b.time_stage(1, 5)},
             "tuned": {"sweep_shape": info["SweepShape"],
                       "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
             "pool_bytes": info["BytesInUse"],
         }
+        if prec == af.PRECISION_MIXED:
+            cyc = [b.mixed_stats(s)[1] for s in slots]
+            res["cycles_last_solve"] = {"min": int(min(cyc)), "max": int(max(cyc))}
         b.Close()
-    # The plain FP64 GPU path on the same workload: each engine alone
-    # (New, the same initial condition, StepN), one after the other; device
-    # time only (creation and uploads excluded).
-    el_plain, divs_plain = 0.0, []
-    for e, f in zip(engines, fields):
-        eng = af.New(af.Config(**e))
-        load_ic(eng, f)
-        eng.StepN(warmup)
-        hip_sync()
-        t0 = time.perf_counter()
-        eng.StepN(steps)
-        hip_sync()
-        el_plain += time.perf_counter() - t0
-        divs_plain.append(eng.MaxDivergence())
-        eng.Close()
-    out["plain_fp64_one_engine_at_a_time"] = {
-        "value": cells * steps / el_plain, "unit": "cell-updates/s",
-        "ms_per_step": 1e3 * el_plain / steps,
-        "final_max_divergence_equal_to_fp64_batch": bool(np.array_equal(np.array(divs_plain),
-                                                                        divs["fp64"])),
-    }
-    rel = (divs["mixed"] - divs["fp64"]) / divs["fp64"]
-    out["speedup_mixed_vs_plain_fp64"] = out["mixed"]["value"] / (cells * steps / el_plain)
-    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-    out["final_max_divergence_rel_diff"] = {"max_abs": float(np.abs(rel).max()),
-                                            "mean": float(rel.mean())}
+        rel = (d - d_plain) / d_plain
+        res["final_max_divergence_vs_plain_fp64"] = {
+            "rel_diff_max_abs": float(np.abs(rel).max()), "rel_diff_mean": float(rel.mean()),
+            "engines_at_or_below": int((d <= d_plain).sum())}
+        return res, d
+
+    none = lambda i: {}
+    out["fp64"], d64 = run(af.PRECISION_FP64, {}, none)
+    out["fp64"]["final_max_divergence_equal_to_plain"] = bool(np.array_equal(d64, d_plain))
+    out["mixed"], _ = run(af.PRECISION_MIXED, {}, none)
+    out["mixed"]["schedule"] = "24 FP32 sweep pairs in one FP64 defect-correction cycle"
+    out["mixed_to_target"], _ = run(
+        af.PRECISION_MIXED,
+        {"MixedChecked": 1, "MixedSweeps": 20, "MixedFixSweeps": 2, "MixedMaxCycles": 4},
+        lambda i: {"MixedSweeps": 20, "MixedFixSweeps": 2, "MixedMaxCycles": 4,
+                   "MixedTarget": float(d_plain[i])})
+    out["mixed_to_target"]["schedule"] = (
+        "20 FP32 sweep pairs, then FP64-residual-checked cycles of 2 pairs (at most 4) until "
+        "max|div| <= the engine's final max|div| on the plain FP64 path")
+    out["speedup_mixed_vs_fp64_batch"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out
 
 
@@ -239,13 +264,16 @@ def big_leg(af, n: int, steps: int) -> dict:
     return out
 
 
-def served_leg(af, n_jobs: int, rate: float, budget: float) -> dict:
+def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dict:
     """Served step-job stream: n_jobs jobs arriving as a Poisson process at
     `rate`/s, axis sizes from {32,64,96,128,192,256}, N uniform in [10,200],
     nu log-uniform in [1e-6,1e-4], DT = min(0.004, 0.5 H^2/(6 nu)), seeded
     band-limited initial velocity (|k| <= 8, |k|^-5/3, RMS 1) built on the
     device at admission, one-cycle mixed-precision projection, arenas
-    admitted first come first served under the HBM budget."""
+    admitted first come first served under the HBM budget.  The first
+    n_plain jobs are also run on the plain FP64 path, each alone (New,
+    init_band_limited, StepN; StepN timed), for its cell-updates/s and each
+    job's final max|div|."""
     from aeroflux_amd import workload
 
     jobs = workload.served_jobs(n_jobs, seed=7, rate=rate)
@@ -254,6 +282,29 @@ def served_leg(af, n_jobs: int, rate: float, budget: float) -> dict:
     res, st = af.Serve(b, jobs, realtime=True)
     info = b.info()
     b.Close()
+    plain = None
+    if n_plain > 0:
+        el, cu, rel = 0.0, 0, []
+        for j, r in list(zip(jobs, res))[:n_plain]:
+            e = af.New(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], H=j["H"], DT=j["DT"],
+                                 Viscosity=j["Viscosity"]))
+            e.init_band_limited(j["Seed"])
+            hip_sync()
+            t0 = time.perf_counter()
+            e.StepN(j["Steps"])
+            hip_sync()
+            el += time.perf_counter() - t0
+            cu += j["NX"] * j["NY"] * j["NZ"] * j["Steps"]
+            d = e.MaxDivergence()
+            e.Close()
+            if r["Status"] == 0:
+                rel.append((r["MaxDivergence"] - d) / d)
+        rel = np.array(rel)
+        plain = {"jobs": n_plain, "value": cu / el, "unit": "cell-updates/s",
+                 "how": "each job alone through New/init_band_limited/StepN, one after the other",
+                 "served_mixed_final_max_divergence_rel_diff": {
+                     "max_abs": float(np.abs(rel).max()) if rel.size else 0.0,
+                     "mean": float(rel.mean()) if rel.size else 0.0}}
     ok = [r for r in res if r["Status"] == 0]
     lat = [r["Done"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
     que = [r["Admitted"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
@@ -279,6 +330,9 @@ def served_leg(af, n_jobs: int, rate: float, budget: float) -> dict:
         "memory_budget_bytes": int(budget),
         "tuned": {"sweep_shape": info["SweepShape"],
                   "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
+        "plain_fp64_sample": plain,
+        "speedup_vs_plain_fp64": (st["CellUpdates"] / st["Wall"] / plain["value"]
+                                  if plain and st["Wall"] > 0 else None),
     }
 
 
@@ -348,6 +402,8 @@ def main() -> int:
                     help="jobs in the served stream (the full stream is 10000)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--served-budget", type=float, default=200e9)
+    ap.add_argument("--served-plain-jobs", type=int, default=60,
+                    help="jobs of the stream also run alone on the plain FP64 path")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -493,7 +549,8 @@ def main() -> int:
     if not args.no_batched:
         out["batched"] = batched_leg(af, ic, args.batched_engines, args.batched_steps, 2)
     if not args.no_served:
-        out["served"] = served_leg(af, args.served_jobs, args.served_rate, args.served_budget)
+        out["served"] = served_leg(af, args.served_jobs, args.served_rate, args.served_budget,
+                                   min(args.served_plain_jobs, args.served_jobs))
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, f, args.cpu_seconds, args.cpu_workers)

@@ -231,6 +231,31 @@ def test_nan_departure_is_a_contained_fault(af):
     e.Close()  # idempotent
 
 
+def test_step_n_fault_reporting(af):
+    """StepN(n) reads the fault flag once, after n steps; StepN(n,
+    first_fault=True) reads it after every step and names the first faulting
+    step.  Both leave the engine usable."""
+    cfg = dict(NX=8, NY=8, NZ=8, H=1 / 8, DT=0.01)
+    e = af.New(af.Config(**cfg))
+    bad = ic.taylor_green(8, 8, 8, 1, 1 / 8, seed=2)
+    bad["V"][3, 5, 2] = math.nan
+    for first_fault in (False, True):
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            e.Mesh.SetData(e.Mesh.field(fid), bad[name])
+        with pytest.raises(afm.AeroFluxError, match="^aeroflux: device fault contained") as ei:
+            e.StepN(3, first_fault=first_fault)
+        assert ei.value.code == afm.AF_ERR_FAULT
+        if first_fault:
+            assert ei.value.step == 0
+    clean = ic.taylor_green(8, 8, 8, 1, 1 / 8, seed=2)
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), clean[name])
+    e.StepN(3)
+    e.StepN(2, first_fault=True)
+    assert math.isfinite(e.MaxDivergence())
+    e.Close()
+
+
 def test_close_then_use_errors(af):
     e = af.New(af.Config(NX=4, NY=4, NZ=4))
     e.Close()
