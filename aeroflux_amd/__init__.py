@@ -82,7 +82,7 @@ class _CBatchConfig(C.Structure):
         ("Precision", C.c_int32), ("MaxSlots", C.c_int32), ("MemoryBudget", C.c_int64),
         ("GaussSeidelIters", C.c_int64), ("MixedMaxCycles", C.c_int32),
         ("MixedSweeps", C.c_int32), ("MixedFixSweeps", C.c_int32), ("MixedChecked", C.c_int32),
-        ("SweepShape", C.c_int32), ("Workgroups", C.c_int32),
+        ("SweepShape", C.c_int32), ("Workgroups", C.c_int32), ("SweepStaged", C.c_int32),
     ]
 
 
@@ -91,7 +91,7 @@ class _CBatchInfo(C.Structure):
         ("Slots", C.c_int32), ("SweepShape", C.c_int32), ("SweepWorkgroups", C.c_int32),
         ("PassWorkgroups", C.c_int32), ("OtherWorkgroups", C.c_int32), ("Tuned", C.c_int32),
         ("Cells", C.c_int64), ("BytesInUse", C.c_int64), ("BytesLargestFree", C.c_int64),
-        ("MemoryBudget", C.c_int64),
+        ("MemoryBudget", C.c_int64), ("SweepStaged", C.c_int32), ("StagedSlots", C.c_int32),
     ]
 
 
@@ -510,6 +510,7 @@ class BatchConfig:
     MixedChecked: int = 0
     SweepShape: int = 0
     Workgroups: int = 0
+    SweepStaged: int = 0
 
     def _to_c(self) -> _CBatchConfig:
         c = _CBatchConfig()

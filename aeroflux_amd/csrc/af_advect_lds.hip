@@ -54,7 +54,7 @@ __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int 
         auto src_of = [&](int c) -> const DField & { return c == 0 ? a.U : (c == 1 ? a.V : a.W); };
         slds::tile<AX, BY, D, ZC>(a, src_of, (slds::lds_t *)lds_mem, bx, by, bz, bad);
     }
-    if (bad) *a.flag = 1;
+    if (bad) *body::gp(a.flag) = 1;
 }
 
 template <int BY, int D, int ZC>

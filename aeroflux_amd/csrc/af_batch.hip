@@ -30,6 +30,7 @@
 #include "../../include/aeroflux.h"
 #include "af_bodies.h"
 #include "af_host.h"
+#include "af_sweep_lds.h"
 #include "af_ic.h"
 #include "af_internal.h"
 
@@ -54,17 +55,51 @@ enum Kind {
     K_PBC,    // pressure ghost-shell cells, CHUNK per item
     K_COL,    // 64 x 4 interior columns (z loop inside)
     K_ENG,    // one item per slot
+    K_LDS,    // LDS-staged advection tiles (64 x rows x z-chunk) + wall blocks
     K_COUNT
 };
 
-// Sweep shapes: workgroups of BX x rows threads over flattened plane tiles,
-// z positions per thread.  (The single engine's LDS-staged sweep keeps its
-// per-field descriptors in scalar registers as kernel arguments; here they
-// are per slot and spill to vector registers, which costs more than the
-// staging saves, so the batch gathers.)
+// Sweep shapes of the global gather: workgroups of BX x rows threads over
+// flattened plane tiles, z positions per thread.
 constexpr int kShapes = 8;
 constexpr int kShapeRows[kShapes] = {1, 1, 2, 2, 4, 4, 8, 16};
 constexpr int kShapeZ[kShapes] = {1, 2, 1, 2, 1, 2, 1, 1};
+
+// LDS-staged sweep variants (af_sweep_lds.h): rows per workgroup and
+// z-planes per chunk; staged reach two cells along x and y, one along z.
+constexpr int kLdsVariants = 4;
+
+// A sweep plan: the gather shape for every slot, and the LDS-staged variant
+// (-1: none) for the slots lds_slot accepts, as one integer.
+inline int make_plan(int gather, int lds) { return gather + kShapes * (lds + 1); }
+inline int plan_gather(int plan) { return plan % kShapes; }
+inline int plan_lds(int plan) { return plan / kShapes - 1; }
+
+// Slots whose x extent fills their 64-wide tiles to at least 3/4 take the
+// LDS-staged sweep; narrower ones keep the flattened-plane gather, which
+// leaves no lane idle.
+inline bool lds_slot(const Geom &G)
+{
+    const int tiles = (G.NX + BX - 1) / BX;
+    return G.NX >= BX && 4 * G.NX >= 3 * BX * tiles;
+}
+
+template <int LBY, int ZC>
+inline int64_t lds_items_t(const Geom &G)
+{
+    const int3 nb = slds::tiles<LBY, ZC>(G);
+    return (int64_t)nb.x * nb.y * nb.z + slds::wall_blocks<LBY>(G);
+}
+
+inline int64_t lds_items(const Geom &G, int v)
+{
+    switch (v) {
+    case 0: return lds_items_t<4, 16>(G);
+    case 1: return lds_items_t<4, 32>(G);
+    case 2: return lds_items_t<8, 16>(G);
+    default: return lds_items_t<8, 32>(G);
+    }
+}
 
 // Device descriptor of one slot.
 struct BSlot {
@@ -90,9 +125,11 @@ struct BCtl {
 // `threads` consecutive points of one plane (x fastest) for one z-block.
 inline int64_t plane_tiles(int64_t plane, int threads) { return (plane + threads - 1) / threads; }
 
-inline int64_t kind_items(int kind, const Geom &G, const DField &P, int shape)
+inline int64_t kind_items(int kind, const Geom &G, const DField &P, int plan)
 {
     const int64_t nxh = (G.NX + 1) / 2;
+    const int shape = plan_gather(plan), lds = plan_lds(plan);
+    const bool staged = lds >= 0 && lds_slot(G);
     switch (kind) {
     case K_VBC: {
         int64_t cu, cv, cw;
@@ -100,6 +137,7 @@ inline int64_t kind_items(int kind, const Geom &G, const DField &P, int shape)
         return (cu + cv + cw + CHUNK - 1) / CHUNK;
     }
     case K_SWEEP: {
+        if (staged) return 0;
         const int r = kShapeRows[shape], z = kShapeZ[shape];
         return plane_tiles((int64_t)(G.NX + 1) * (G.NY + 1), BX * r) * ((G.NZ + z) / z);
     }
@@ -109,6 +147,7 @@ inline int64_t kind_items(int kind, const Geom &G, const DField &P, int shape)
     case K_PBC:
         return (body::pressure_shell_count(P.nx, P.ny, P.nz, G.g) + CHUNK - 1) / CHUNK;
     case K_COL: return plane_tiles((int64_t)G.NX * G.NY, BX * BY);
+    case K_LDS: return staged ? lds_items(G, lds) : 0;
     default: return 1;
     }
 }
@@ -212,6 +251,57 @@ __global__ __launch_bounds__(BX *BYT) void kb_sweep(BCtl c, int sw)
     });
 }
 
+// advectDiffuseAxis, stencil.go:21-119, with the sources staged in LDS
+// (af_sweep_lds.h) for the slots that take it: one workgroup per item of
+// the K_LDS work-list, which numbers a slot's tiles x fastest, then y, then
+// z-chunk, followed by its wall-column blocks.  XCD x (= blockIdx.x % 8)
+// takes the x-th eighth of the list in order; workgroups past the list
+// return.  The grid must hold at least 8 * ceil(items / 8) workgroups.
+template <int AX, int LBY, int ZC>
+__global__ __launch_bounds__(BX *LBY) void kb_sweep_lds(BCtl c, int sw)
+{
+    using Bx = slds::Box<AX, LBY, 2>;
+    __shared__ double lds_mem[Bx::TOTAL];
+    const int *tab = c.tab;
+    const int nact = tab[0], stride = tab[1];
+    const int *act = tab + 2;
+    const int *first = tab + 2 + stride * (1 + K_LDS);
+    const int total = first[nact];
+    const int per = (total + XCDS - 1) / XCDS;
+    const int k = (int)(blockIdx.x / XCDS);
+    const int i = (int)(blockIdx.x % XCDS) * per + k;
+    if (k >= per || i >= total) return;
+    int a = 0, r = nact - 1;  // largest a with first[a] <= i
+    while (a < r) {
+        const int m = (a + r + 1) >> 1;
+        if (first[m] <= i) a = m; else r = m - 1;
+    }
+    const int s = act[a], t = i - first[a];
+    const BSlot &e = c.slots[s];
+    SweepArgs sa;
+    sa.U = velf(e, sw, 0); sa.V = velf(e, sw, 1); sa.W = velf(e, sw, 2);
+    sa.T0 = tmpf(e, sw, 0); sa.T1 = tmpf(e, sw, 1); sa.T2 = tmpf(e, sw, 2);
+    sa.G = e.G;
+    sa.dt = e.dt_half;
+    sa.invH = e.invH;
+    sa.lam = e.lam_half;
+    sa.flag = c.flags + s;
+    const int3 nb = slds::tiles<LBY, ZC>(e.G);
+    const int ntile = nb.x * nb.y * nb.z;
+    bool bad = false;
+    if (t >= ntile) {
+        slds::wall_face<AX>(sa, (t - ntile) * (BX * LBY) + threadIdx.y * BX + threadIdx.x, bad);
+    } else {
+        const int rest = t / nb.x;
+        // Descriptors by run-time index come from the slot table in global
+        // memory (a local copy indexed at run time would live in scratch).
+        auto src_of = [&](int q) -> const DField & { return velf(e, sw, q); };
+        slds::tile<AX, LBY, 2, ZC>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
+                                   rest % nb.y, rest / nb.y, bad);
+    }
+    if (bad) *body::gp(sa.flag) = 1;
+}
+
 // computeDivergence, project.go:42-62, also writing the solve's split
 // right-hand side: the FP64 divergence itself, or the mixed solve's FP32
 // -(H^2 div).  Same values as a separate split pass over Div.
@@ -231,9 +321,9 @@ __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
         const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
         const bool black = body::parity(xi, yi, zi, G.g);
         if (MIXED)
-            static_cast<float *>(black ? e.d_black : e.d_red)[i] = (float)(-(e.h2 * v));
+            body::gp(static_cast<float *>(black ? e.d_black : e.d_red))[i] = (float)(-(e.h2 * v));
         else
-            static_cast<double *>(black ? e.d_black : e.d_red)[i] = v;
+            body::gp(static_cast<double *>(black ? e.d_black : e.d_red))[i] = v;
     });
 }
 
@@ -316,7 +406,7 @@ __global__ __launch_bounds__(BX *BY) void kb_resid(BCtl c, int gated)
                                                static_cast<float *>(e.d_red),
                                                static_cast<float *>(e.d_black), G, e.h2, xi, yi);
         const double b = body::block_max256(m, sh);
-        if (threadIdx.x == 0 && threadIdx.y == 0) e.partials[t] = b;
+        if (threadIdx.x == 0 && threadIdx.y == 0) body::gp(e.partials)[t] = b;
     });
 }
 
@@ -330,7 +420,8 @@ __global__ __launch_bounds__(256) void kb_res_final(BCtl c, int gated)
         const BSlot &e = c.slots[s];
         const int np = (e.G.NX * e.G.NY + BX * BY - 1) / (BX * BY);
         double v = 0;
-        for (int k = threadIdx.x; k < np; k += 256) v = e.partials[k] > v ? e.partials[k] : v;
+        const auto part = body::gp((const double *)e.partials);
+        for (int k = threadIdx.x; k < np; k += 256) v = part[k] > v ? part[k] : v;
         const double m = body::block_max256(v, sh);
         if (threadIdx.x == 0) {
             const double d = e.res_to_div * m;
@@ -466,7 +557,11 @@ struct af_batch {
     bool tab_dirty = true;
     int tab_shape = -1;
     hipGraphExec_t graph = nullptr;
-    int shape = 4, nwg_sweep = 2048, nwg_pass = 2048, nwg_other = 2048;
+    int shape = 4;  // sweep plan (make_plan): gather shape + LDS-staged variant
+    int nwg_sweep = 2048, nwg_pass = 2048, nwg_other = 2048;
+    int lds_grid = 8;      // workgroups of the LDS-staged sweep launches in the graph
+    int lds_need = 8;      // workgroups the current table's K_LDS list needs
+    int staged_slots = 0;  // resident slots on the LDS-staged sweep
     bool tuned = false;
     int nact = 0;
     int64_t cells = 0;
@@ -518,8 +613,8 @@ Geom slot_geom(const HSlot &h)
     return Geom{(int)h.cfg.NX, (int)h.cfg.NY, (int)h.cfg.NZ, (int)h.cfg.Ghost};
 }
 
-// Rebuilds staging table `ring` for the current slots and sweep shape.
-int build_table(af_batch *b, int shape, int ring = 0)
+// Rebuilds staging table `ring` for the current slots and sweep plan.
+int build_table(af_batch *b, int plan, int ring = 0)
 {
     const int stride = b->cfg.MaxSlots + 1;
     int *t = b->h_tab + (int64_t)ring * b->tab_ints;
@@ -535,28 +630,49 @@ int build_table(af_batch *b, int shape, int ring = 0)
         first[0] = 0;
         for (int a = 0; a < n; a++) {
             const HSlot &h = b->slots[act[a]];
-            acc += kind_items(k, slot_geom(h), slot_field(b, h, AF_FIELD_P), shape);
-            if (acc > INT_MAX)
+            acc += kind_items(k, slot_geom(h), slot_field(b, h, AF_FIELD_P), plan);
+            if (acc > INT_MAX - 8 * 1024)
                 return fail(AF_ERR_CONFIG, "aeroflux: batch work-list exceeds 2^31 tiles");
             first[a + 1] = (int)acc;
         }
     }
     b->nact = n;
+    const int lds_total = t[2 + (int64_t)stride * (1 + K_LDS) + n];
+    b->lds_need = 8 * ((lds_total + 7) / 8);
+    if (b->lds_need < 8) b->lds_need = 8;
+    b->staged_slots = 0;
+    if (plan_lds(plan) >= 0)
+        for (int a = 0; a < n; a++) b->staged_slots += lds_slot(slot_geom(b->slots[act[a]]));
     return AF_OK;
+}
+
+void drop_graph(af_batch *b);
+
+// The LDS-staged sweep runs one workgroup per work item, so its grid is
+// fixed in the step graph: when a table needs more, the graph is dropped
+// (the caller has synchronised the stream) and recaptured with a grid 25%
+// larger than needed, so a growing resident set recaptures rarely.
+void fit_lds_grid(af_batch *b)
+{
+    if (b->lds_need <= b->lds_grid) return;
+    const int64_t want = 8 * (((int64_t)b->lds_need + b->lds_need / 4 + 7) / 8);
+    b->lds_grid = (int)(want < (int64_t)(INT_MAX & ~7) ? want : (int64_t)(INT_MAX & ~7));
+    drop_graph(b);
 }
 
 // Uploads the table through the pinned staging copy, after any earlier
 // copy from it has run.
-int upload_table(af_batch *b, int shape)
+int upload_table(af_batch *b, int plan)
 {
-    if (!b->tab_dirty && b->tab_shape == shape) return AF_OK;
+    if (!b->tab_dirty && b->tab_shape == plan) return AF_OK;
     HIP_TRY(hipStreamSynchronize(b->stream), "upload batch work-list");
-    if (int rc = build_table(b, shape)) return rc;
+    if (int rc = build_table(b, plan)) return rc;
+    fit_lds_grid(b);
     HIP_TRY(hipMemcpyAsync(b->d_tab, b->h_tab, (size_t)b->tab_ints * sizeof(int),
                            hipMemcpyHostToDevice, b->stream),
             "upload batch work-list");
     b->tab_dirty = false;
-    b->tab_shape = shape;
+    b->tab_shape = plan;
     return AF_OK;
 }
 
@@ -581,17 +697,41 @@ hipError_t sweep_shape_launch(const af_batch *b, int nwg, int axis, int sw)
     return hipGetLastError();
 }
 
-hipError_t enq_sweep(const af_batch *b, int shape, int nwg, int axis, int sw)
+template <int LBY, int ZC>
+hipError_t sweep_lds_launch(const af_batch *b, int axis, int sw)
 {
-    switch (shape) {
-    case 0: return sweep_shape_launch<1, 1>(b, nwg, axis, sw);
-    case 1: return sweep_shape_launch<1, 2>(b, nwg, axis, sw);
-    case 2: return sweep_shape_launch<2, 1>(b, nwg, axis, sw);
-    case 3: return sweep_shape_launch<2, 2>(b, nwg, axis, sw);
-    case 4: return sweep_shape_launch<4, 1>(b, nwg, axis, sw);
-    case 5: return sweep_shape_launch<4, 2>(b, nwg, axis, sw);
-    case 6: return sweep_shape_launch<8, 1>(b, nwg, axis, sw);
-    default: return sweep_shape_launch<16, 1>(b, nwg, axis, sw);
+    const BCtl c = ctl(b);
+    const dim3 grid(b->lds_grid), blk(BX, LBY);
+    switch (axis) {
+    case 0: hipLaunchKernelGGL((kb_sweep_lds<0, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
+    case 1: hipLaunchKernelGGL((kb_sweep_lds<1, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
+    default: hipLaunchKernelGGL((kb_sweep_lds<2, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
+    }
+    return hipGetLastError();
+}
+
+// One sweep of every slot under `plan`: the gather over the K_SWEEP list,
+// then (if the plan has a variant) the LDS-staged sweep over the K_LDS list.
+hipError_t enq_sweep(const af_batch *b, int plan, int nwg, int axis, int sw)
+{
+    hipError_t err;
+    switch (plan_gather(plan)) {
+    case 0: err = sweep_shape_launch<1, 1>(b, nwg, axis, sw); break;
+    case 1: err = sweep_shape_launch<1, 2>(b, nwg, axis, sw); break;
+    case 2: err = sweep_shape_launch<2, 1>(b, nwg, axis, sw); break;
+    case 3: err = sweep_shape_launch<2, 2>(b, nwg, axis, sw); break;
+    case 4: err = sweep_shape_launch<4, 1>(b, nwg, axis, sw); break;
+    case 5: err = sweep_shape_launch<4, 2>(b, nwg, axis, sw); break;
+    case 6: err = sweep_shape_launch<8, 1>(b, nwg, axis, sw); break;
+    default: err = sweep_shape_launch<16, 1>(b, nwg, axis, sw); break;
+    }
+    if (err != hipSuccess) return err;
+    switch (plan_lds(plan)) {
+    case -1: return hipSuccess;
+    case 0: return sweep_lds_launch<4, 16>(b, axis, sw);
+    case 1: return sweep_lds_launch<4, 32>(b, axis, sw);
+    case 2: return sweep_lds_launch<8, 16>(b, axis, sw);
+    default: return sweep_lds_launch<8, 32>(b, axis, sw);
     }
 }
 
@@ -762,39 +902,58 @@ hipError_t enq_timed_pass(const af_batch *b, int nwg)
     return enq_pass<double>(b, nwg, 0, 0, 0);
 }
 
-// Autotuner: advection shape (timed on X, Y and Z sweeps), then persistent
-// workgroup counts for the sweeps and the colour passes, each timed on
-// batched launches over the resident slots.  A sweep writes only the face
-// cells of the Tmp fields, and the next step's first sweep rewrites every one
-// of them before anything reads them; colour passes touch only solve buffers
-// that every solve rewrites before reading.  Results therefore do not depend
-// on tuning.
+// Autotuner: the gather shape (timed on X, Y and Z sweeps with every slot
+// gathering), then the LDS-staged variant for the slots that can take it
+// (or none), then persistent workgroup counts for the gather sweeps and the
+// colour passes, each timed on batched launches over the resident slots.  A
+// sweep writes only the face cells of the Tmp fields, and the next step's
+// first sweep rewrites every one of them before anything reads them; colour
+// passes touch only solve buffers that every solve rewrites before reading.
+// Results therefore do not depend on tuning.
 int tune(af_batch *b)
 {
     if (b->nact == 0) return AF_OK;
     static const int kWg[] = {1024, 2048, 4096, 8192};
     const int reps = 3;
-    int best_shape = b->cfg.SweepShape > 0 ? b->cfg.SweepShape - 1 : b->shape;
     const int wg0 = b->cfg.Workgroups > 0 ? b->cfg.Workgroups : 2048;
+    auto time_plan = [&](int plan, int nwg, double *ms) -> int {
+        if (int rc = upload_table(b, plan)) return rc;
+        *ms = 0;
+        for (int ax = 0; ax < 3; ax++) {
+            double t;
+            if (int rc = time_launches(b, reps, [&] { return enq_sweep(b, plan, nwg, ax, 0); },
+                                       &t))
+                return rc;
+            *ms += t;
+        }
+        return AF_OK;
+    };
+    int gather = b->cfg.SweepShape > 0 ? b->cfg.SweepShape - 1 : plan_gather(b->shape);
+    double best = 1e30;
     if (b->cfg.SweepShape <= 0) {
-        double best = 1e30;
         for (int s = 0; s < kShapes; s++) {
-            if (int rc = upload_table(b, s)) return rc;
-            double ms = 0;
-            for (int ax = 0; ax < 3; ax++) {
-                double t;
-                if (int rc = time_launches(b, reps, [&] { return enq_sweep(b, s, wg0, ax, 0); },
-                                           &t))
-                    return rc;
-                ms += t;
-            }
+            double ms;
+            if (int rc = time_plan(make_plan(s, -1), wg0, &ms)) return rc;
             if (ms < best) {
                 best = ms;
-                best_shape = s;
+                gather = s;
             }
         }
     }
-    b->shape = best_shape;
+    int lds = b->cfg.SweepStaged > 0 ? b->cfg.SweepStaged - 1 : -1;
+    if (b->cfg.SweepStaged == 0) {
+        if (best >= 1e30)
+            if (int rc = time_plan(make_plan(gather, -1), wg0, &best)) return rc;
+        for (int v = 0; v < kLdsVariants; v++) {
+            double ms;
+            if (int rc = time_plan(make_plan(gather, v), wg0, &ms)) return rc;
+            if (ms < best) {
+                best = ms;
+                lds = v;
+            }
+        }
+    }
+    b->shape = make_plan(gather, lds);
     if (int rc = upload_table(b, b->shape)) return rc;
     b->nwg_sweep = b->nwg_pass = b->nwg_other = wg0;
     if (b->cfg.Workgroups <= 0) {
@@ -891,6 +1050,9 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
                     (int)c.SweepShape, kShapes);
     if (c.Workgroups < 0 || c.Workgroups % XCDS != 0)
         return fail(AF_ERR_CONFIG, "aeroflux: persistent workgroups must be a multiple of 8");
+    if (c.SweepStaged < -1 || c.SweepStaged > kLdsVariants)
+        return fail(AF_ERR_CONFIG, "aeroflux: staged sweep variant %d out of range -1..%d",
+                    (int)c.SweepStaged, kLdsVariants);
 
     af_batch *b = new (std::nothrow) af_batch();
     if (!b) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
@@ -909,7 +1071,8 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
     const int m = c.MaxSlots;
     b->slots.resize(m);
     b->tab_ints = 2 + (int64_t)(m + 1) * (1 + K_COUNT);
-    if (c.SweepShape > 0) b->shape = c.SweepShape - 1;
+    b->shape = make_plan(c.SweepShape > 0 ? c.SweepShape - 1 : 4,
+                         c.SweepStaged > 0 ? c.SweepStaged - 1 : -1);
     if (c.Workgroups > 0) b->nwg_sweep = b->nwg_pass = b->nwg_other = c.Workgroups;
     if ((err = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(err, "create stream", AF_ERR_HIP);
@@ -970,7 +1133,10 @@ int af_batch_get_info(const af_batch *b, af_batch_info *out)
     if (!b || !out) return fail(AF_ERR_ARG, "aeroflux: null argument");
     af_batch_info i{};
     for (const HSlot &h : b->slots) i.Slots += h.used ? 1 : 0;
-    i.SweepShape = b->shape;
+    i.SweepShape = plan_gather(b->shape);
+    i.SweepStaged = plan_lds(b->shape);
+    if (i.SweepStaged >= 0)
+        for (const HSlot &h : b->slots) i.StagedSlots += h.used && lds_slot(slot_geom(h)) ? 1 : 0;
     i.SweepWorkgroups = b->nwg_sweep;
     i.PassWorkgroups = b->nwg_pass;
     i.OtherWorkgroups = b->nwg_other;
@@ -1502,10 +1668,15 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
             if ((rc = upload_table(b, b->shape)) != AF_OK) break;
             if ((rc = tune(b)) != AF_OK) break;
         }
-        if ((rc = get_graph(b)) != AF_OK) break;
         if (b->tab_dirty || b->tab_shape != b->shape) {
             const int ring = (int)(step % R);
             if ((rc = build_table(b, b->shape, ring)) != AF_OK) break;
+            if (b->lds_need > b->lds_grid) {
+                // The staged sweep's grid no longer covers the work-list: the
+                // graph is rebuilt once the steps in flight have run.
+                if ((err = hipStreamSynchronize(b->stream)) != hipSuccess) break;
+                fit_lds_grid(b);
+            }
             if ((err = hipMemcpyAsync(b->d_tab, b->h_tab + (int64_t)ring * b->tab_ints,
                                       (size_t)b->tab_ints * sizeof(int), hipMemcpyHostToDevice,
                                       b->stream)) != hipSuccess)
@@ -1513,6 +1684,7 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
             b->tab_dirty = false;
             b->tab_shape = b->shape;
         }
+        if ((rc = get_graph(b)) != AF_OK) break;
         if ((err = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) break;
         if ((err = hipEventRecord(ev[step % R], b->stream)) != hipSuccess) break;
         std::vector<int64_t> done_now;

@@ -19,6 +19,19 @@ namespace body {
 
 constexpr int BX = 64;  // lanes along x
 
+// Field memory as global memory.  A descriptor read from device memory (the
+// batched slots' table) carries generic pointers, and loads and stores
+// through generic pointers become flat instructions, which count against
+// the LDS wait counter as well: a kernel that stages through LDS would then
+// wait for its in-flight global prefetch at every LDS read.  Every field
+// access in these bodies goes through gp(), so it is a global instruction
+// whichever way the descriptor arrived.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 __device__ __forceinline__ int idx(const DField &f, int x, int y, int z)
 {
     return x + y * f.sy + z * f.sz;  // Field.Index, grid.go:24-26
@@ -124,9 +137,9 @@ __device__ __forceinline__ void velocity_bc_cell(const DField &U, const DField &
             flips++;
         }
     }
-    double v = wall ? 0.0 : f.d[idx(f, c[0], c[1], c[2])];
+    double v = wall ? 0.0 : gp(f.d)[idx(f, c[0], c[1], c[2])];
     if (flips & 1) v = -v;  // two exact negations are the identity
-    f.d[idx(f, x, y, z)] = v;
+    gp(f.d)[idx(f, x, y, z)] = v;
 }
 
 // -------------------------------------------------------------- advection
@@ -151,11 +164,12 @@ __device__ __forceinline__ void velocity_bc_cell(const DField &U, const DField &
 // hardware accepts dword-aligned vector loads).  Every trilinear tap pair
 // and most four-point averages are x-adjacent, which halves the load
 // instructions of a sweep.
+typedef double dbl2_a8 __attribute__((ext_vector_type(2), aligned(8)));
 __device__ __forceinline__ double2 ld2(const double *p)
 {
-    double2 v;
-    __builtin_memcpy(&v, p, sizeof(v));
-    return v;
+    typedef __attribute__((address_space(1))) const dbl2_a8 gdbl2;
+    const dbl2_a8 v = *(gdbl2 *)p;
+    return make_double2(v.x, v.y);
 }
 
 // Where a sweep's values come from.  A loader returns component c's value
@@ -172,7 +186,7 @@ struct GlobalLoad {
     __device__ __forceinline__ double one(int c, int x, int y, int z) const
     {
         const DField &s = f(c);
-        return s.d[idx(s, x, y, z)];
+        return gp(s.d)[idx(s, x, y, z)];
     }
     __device__ __forceinline__ double2 pair(int c, int x, int y, int z) const
     {
@@ -305,11 +319,11 @@ __device__ __forceinline__ void sweep_column(const SweepArgs &a, int x, int y, i
 #pragma unroll
     for (int r = 0; r < ZR; r++) {
         const int z = z0 + r;
-        if (du[r]) a.T0.d[idx(a.T0, x, y, z)] = vu[r];
-        if (dv[r]) a.T1.d[idx(a.T1, x, y, z)] = vv[r];
-        if (dw[r]) a.T2.d[idx(a.T2, x, y, z)] = vw[r];
+        if (du[r]) gp(a.T0.d)[idx(a.T0, x, y, z)] = vu[r];
+        if (dv[r]) gp(a.T1.d)[idx(a.T1, x, y, z)] = vv[r];
+        if (dw[r]) gp(a.T2.d)[idx(a.T2, x, y, z)] = vw[r];
     }
-    if (bad) *a.flag = 1;
+    if (bad) *gp(a.flag) = 1;
 }
 
 // ------------------------------------------------------------- projection
@@ -323,11 +337,12 @@ __device__ __forceinline__ bool divergence_cell(const DField &U, const DField &V
 {
     const int g = G.g;
     if (x >= g + G.NX || y >= g + G.NY) return false;
-    const double du = U.d[idx(U, x + 1, y, z)] - U.d[idx(U, x, y, z)];
-    const double dv = V.d[idx(V, x, y + 1, z)] - V.d[idx(V, x, y, z)];
-    const double dw = W.d[idx(W, x, y, z + 1)] - W.d[idx(W, x, y, z)];
+    const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
+    const double du = u[idx(U, x + 1, y, z)] - u[idx(U, x, y, z)];
+    const double dv = v[idx(V, x, y + 1, z)] - v[idx(V, x, y, z)];
+    const double dw = w[idx(W, x, y, z + 1)] - w[idx(W, x, y, z)];
     out = scale * (du + dv + dw);
-    D.d[idx(D, x, y, z)] = out;
+    gp(D.d)[idx(D, x, y, z)] = out;
     return true;
 }
 
@@ -380,11 +395,11 @@ __device__ __forceinline__ void pressure_bc_cell(const DField &P, const Geom &G,
         const int cx = x < g ? g : (x >= g + G.NX ? g + G.NX - 1 : x);
         const int cy = y < g ? g : (y >= g + G.NY ? g + G.NY - 1 : y);
         const int cz = z < g ? g : (z >= g + G.NZ ? g + G.NZ - 1 : z);
-        v = P.d[idx(P, cx, cy, cz)];
+        v = gp(P.d)[idx(P, cx, cy, cz)];
     } else if (!zero_outer) {
         return;
     }
-    P.d[idx(P, x, y, z)] = v;
+    gp(P.d)[idx(P, x, y, z)] = v;
 }
 
 // subtractGradient, project.go:172-205.  The thread of interior column
@@ -396,18 +411,19 @@ __device__ __forceinline__ void gradient_cell(const DField &U, const DField &V, 
 {
     const int g = G.g;
     if (x >= g + G.NX || y >= g + G.NY) return;
-    const double *p = P.d;
+    const auto p = gp((const double *)P.d);
+    const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
     const int ip = idx(P, x, y, z);
     if (z < g + G.NZ) {
         const int iu = idx(U, x, y, z);
-        U.d[iu] -= scale * (p[ip] - p[ip - 1]);
-        if (x == g + G.NX - 1) U.d[iu + 1] -= scale * (p[ip + 1] - p[ip]);
+        u[iu] -= scale * (p[ip] - p[ip - 1]);
+        if (x == g + G.NX - 1) u[iu + 1] -= scale * (p[ip + 1] - p[ip]);
         const int iv = idx(V, x, y, z);
-        V.d[iv] -= scale * (p[ip] - p[ip - P.sy]);
-        if (y == g + G.NY - 1) V.d[iv + V.sy] -= scale * (p[ip + P.sy] - p[ip]);
+        v[iv] -= scale * (p[ip] - p[ip - P.sy]);
+        if (y == g + G.NY - 1) v[iv + V.sy] -= scale * (p[ip + P.sy] - p[ip]);
     }
     const int iw = idx(W, x, y, z);
-    W.d[iw] -= scale * (p[ip] - p[ip - P.sz]);
+    w[iw] -= scale * (p[ip] - p[ip - P.sz]);
 }
 
 // ---------------------------------------------------- colour-split layout
@@ -455,9 +471,9 @@ __device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int col
 {
     const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
     if (yi >= NY) return;
-    T *cur = color ? a.black : a.red;
-    const T *oth = color ? a.red : a.black;
-    const T *rhs = color ? a.rhs_black : a.rhs_red;
+    const auto cur = gp(color ? a.black : a.red);
+    const auto oth = gp((const T *)(color ? a.red : a.black));
+    const auto rhs = gp(color ? a.rhs_black : a.rhs_red);
     T out[ZR];
     int at[ZR];
 #pragma unroll
@@ -499,9 +515,9 @@ __device__ __forceinline__ void to_split_cell(const TN *nat, int sy, int sz, int
                                               int NXH, TS *red, TS *black, int xi, int yi, int zi)
 {
     if (xi >= NX || yi >= NY) return;
-    const TN v = nat[(xi + g) + (yi + g) * sy + (zi + g) * sz];
+    const TN v = gp(nat)[(xi + g) + (yi + g) * sy + (zi + g) * sz];
     const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    if (parity(xi, yi, zi, g)) black[i] = (TS)v; else red[i] = (TS)v;
+    if (parity(xi, yi, zi, g)) gp(black)[i] = (TS)v; else gp(red)[i] = (TS)v;
 }
 
 // FP32 right-hand side -(h2 * div) of the mixed-precision smoother, split.
@@ -510,9 +526,9 @@ __device__ __forceinline__ void rhs32_cell(const double *div, int sy, int sz, in
                                            int yi, int zi)
 {
     if (xi >= NX || yi >= NY) return;
-    const float v = (float)(-(h2 * div[(xi + g) + (yi + g) * sy + (zi + g) * sz]));
+    const float v = (float)(-(h2 * gp(div)[(xi + g) + (yi + g) * sy + (zi + g) * sz]));
     const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    if (parity(xi, yi, zi, g)) black[i] = v; else red[i] = v;
+    if (parity(xi, yi, zi, g)) gp(black)[i] = v; else gp(red)[i] = v;
 }
 
 // Split -> interior cell of the FP64 natural-layout P.  ADD: P += split (the
@@ -524,9 +540,9 @@ __device__ __forceinline__ void from_split_cell(double *nat, int sy, int sz, int
 {
     if (xi >= NX || yi >= NY) return;
     const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    const double v = (double)(parity(xi, yi, zi, g) ? black[i] : red[i]);
+    const double v = (double)(parity(xi, yi, zi, g) ? gp(black)[i] : gp(red)[i]);
     const int n = (xi + g) + (yi + g) * sy + (zi + g) * sz;
-    nat[n] = ADD ? nat[n] + v : v;
+    gp(nat)[n] = ADD ? gp(nat)[n] + v : v;
 }
 
 // FP64 residual of the mixed-precision solve along interior column (xi, yi):
@@ -541,7 +557,9 @@ __device__ __forceinline__ double residual_column(const DField &P, const double 
     if (xi >= G.NX || yi >= G.NY) return m;
     const int g = G.g;
     const int NXH = (G.NX + 1) / 2;
-    const double *p = P.d;
+    const auto p = gp((const double *)P.d);
+    const auto dv = gp(div);
+    const auto rr = gp(r_red), rb = gp(r_black);
     const int sy = P.sy, sz = P.sz;
     const int x = xi + g, y = yi + g;
     for (int zi = 0; zi < G.NZ; zi++) {
@@ -555,9 +573,9 @@ __device__ __forceinline__ double residual_column(const DField &P, const double 
         const double zm = (zi == 0) ? c : p[i - sz];
         const double zp = (zi == G.NZ - 1) ? c : p[i + sz];
         const double s = ((((xm + xp) + ym) + yp) + zm) + zp;
-        const double res = (s - h2 * div[i]) - 6.0 * c;
+        const double res = (s - h2 * dv[i]) - 6.0 * c;
         const int si = (zi * G.NY + yi) * NXH + (xi >> 1);
-        if ((x + y + z) & 1) r_black[si] = (float)res; else r_red[si] = (float)res;
+        if ((x + y + z) & 1) rb[si] = (float)res; else rr[si] = (float)res;
         const double a = fabs(res);
         m = a > m ? a : m;
     }

@@ -99,7 +99,7 @@ struct LdsLoad {
 template <class B, int NT>
 struct Stager {
     static constexpr int N = (3 * B::PLANE + NT - 1) / NT;
-    const double *src[N];  // element's (x, y) address in its field, plane 0
+    const __attribute__((address_space(1))) double *src[N];  // element's (x, y) address, plane 0
     int sz[N];             // its field's plane stride
     int nz[N];             // its field's plane count
     int off[N];            // LDS offset within a ring slot
@@ -122,7 +122,7 @@ struct Stager {
             const DField &f = src_of(c);
             const int x = ox + col, y = oy + r;
             ok[k] = e < 3 * B::PLANE && (unsigned)x < (unsigned)f.nx && (unsigned)y < (unsigned)f.ny;
-            src[k] = f.d + (ok[k] ? x + y * f.sy : 0);
+            src[k] = body::gp((const double *)f.d) + (ok[k] ? x + y * f.sy : 0);
             sz[k] = f.sz;
             nz[k] = f.nz;
             off[k] = c * B::FIELD + r * B::XW + col;
@@ -168,7 +168,7 @@ __device__ __forceinline__ void wall_face(const SweepArgs &a, int k, bool &bad)
     const int xe = g + a.G.NX, y = g + k % a.G.NY, z = g + k / a.G.NY;
     const body::GlobalLoad ld{a};
     const double v = body::advect_value<AX, 0>(a, ld, xe, y, z, bad);
-    a.T0.d[body::idx(a.T0, xe, y, z)] = v;
+    body::gp(a.T0.d)[body::idx(a.T0, xe, y, z)] = v;
 }
 
 // One tile (bx, by, bz), computed by the whole workgroup (64 x BY threads)
@@ -206,9 +206,9 @@ __device__ __forceinline__ void tile(const SweepArgs &a, const Src &src_of, lds_
         if (du) vu = body::advect_value<AX, 0>(a, ld, x, y, z, bad);
         if (dv) vv = body::advect_value<AX, 1>(a, ld, x, y, z, bad);
         if (dw) vw = body::advect_value<AX, 2>(a, ld, x, y, z, bad);
-        if (du) a.T0.d[body::idx(a.T0, x, y, z)] = vu;
-        if (dv) a.T1.d[body::idx(a.T1, x, y, z)] = vv;
-        if (dw) a.T2.d[body::idx(a.T2, x, y, z)] = vw;
+        if (du) body::gp(a.T0.d)[body::idx(a.T0, x, y, z)] = vu;
+        if (dv) body::gp(a.T1.d)[body::idx(a.T1, x, y, z)] = vv;
+        if (dw) body::gp(a.T2.d)[body::idx(a.T2, x, y, z)] = vw;
         __syncthreads();
     }
 }

@@ -196,7 +196,8 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
                               "frac": pass_bytes / (t_pass * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                               "algorithmic_bytes_per_launch": pass_bytes},
             "stages": {"rbgs_pass_ms": t_pass, "advect_sweep_ms": b.time_stage(1, 5)},
-            "tuned": {"sweep_shape": info["SweepShape"],
+            "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
+                      "staged_slots": info["StagedSlots"],
                       "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
             "pool_bytes": info["BytesInUse"],
         }
@@ -328,7 +329,7 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dic
         "max_resident_jobs": st["MaxResident"],
         "peak_pool_bytes": st["PeakBytes"],
         "memory_budget_bytes": int(budget),
-        "tuned": {"sweep_shape": info["SweepShape"],
+        "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
                   "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
         "plain_fp64_sample": plain,
         "speedup_vs_plain_fp64": (st["CellUpdates"] / st["Wall"] / plain["value"]

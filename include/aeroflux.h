@@ -250,9 +250,16 @@ typedef struct af_batch_config {
     int32_t MixedChecked;      /* mixed: 0 -> one cycle (slots need
                                   MixedTarget == 0); 1 -> cycles until each
                                   slot's MixedTarget (> 0) is met */
-    int32_t SweepShape;        /* 0 -> tuned; k -> shape k-1 (af_sweep_shape) */
+    int32_t SweepShape;        /* global-gather advection shape: 0 -> tuned;
+                                  k (1..8) -> shape k-1 (af_sweep_shape) */
     int32_t Workgroups;        /* persistent workgroups per launch (0 -> tuned;
                                   else a multiple of 8) */
+    int32_t SweepStaged;       /* LDS-staged advection for slots whose x
+                                  extent fills 64-wide tiles to >= 3/4:
+                                  0 -> tuned; -1 -> off (every slot
+                                  gathers); k (1..4) -> variant k-1, i.e.
+                                  (rows, z-planes) of (4,16), (4,32), (8,16),
+                                  (8,32) per workgroup */
 } af_batch_config;
 
 typedef struct af_batch_info {
@@ -266,6 +273,8 @@ typedef struct af_batch_info {
     int64_t BytesInUse;        /* pool bytes held by slots */
     int64_t BytesLargestFree;  /* largest free pool block */
     int64_t MemoryBudget;
+    int32_t SweepStaged;       /* LDS-staged variant in use (-1: none) */
+    int32_t StagedSlots;       /* resident slots that use it */
 } af_batch_info;
 
 int af_batch_new(const af_batch_config *cfg, af_batch **out);

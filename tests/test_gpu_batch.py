@@ -144,6 +144,66 @@ def test_tuned_and_fixed_launch_shapes_agree(af):
             assert_same(a, c, "launch shape")
 
 
+# Grids whose x extent fills 64-wide tiles to at least 3/4 take the
+# LDS-staged sweep when the batch enables it; (33, 9, 14) and (70, 8, 6)
+# stay on the gather beside them.  (64, 7, 33, 2) has Ghost = 2.
+STAGED_GRIDS = [(64, 12, 10, 1), (96, 9, 14, 1), (128, 6, 5, 1), (64, 7, 33, 2), (33, 9, 14, 1),
+                (70, 8, 6, 2)]
+
+
+def staged_fields(i, shp, amp):
+    nx, ny, nz, g = shp
+    return ic.taylor_green(nx, ny, nz, g, 1.0 / max(nx, ny, nz), seed=200 + i, amp=amp,
+                           noise=0.3 * amp)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("amp", [1.0, 24.0])
+def test_staged_sweep_batch_equals_alone(af, variant, amp):
+    """Slots on the LDS-staged sweep -- departures within the staged reach at
+    amplitude 1, up to ~6 cells (clamped at the walls) at amplitude 24 -- and
+    the slots beside them on the gather give the bits of the same engines
+    stepped alone.  variant 0 lets the tuner choose."""
+    steps = 2
+    cfgs = [cfg_for(s, i) for i, s in enumerate(STAGED_GRIDS)]
+    fs = [staged_fields(i, s, amp) for i, s in enumerate(STAGED_GRIDS)]
+    b = af.Batch(af.BatchConfig(MemoryBudget=256 << 20, SweepStaged=variant))
+    slots = [b.Add(af.Config(**c)) for c in cfgs]
+    for s, f in zip(slots, fs):
+        load(b, s, f)
+    b.StepN(steps)
+    info = b.info()
+    if variant:
+        assert info["SweepStaged"] == variant - 1 and info["StagedSlots"] == 4
+    else:
+        assert -1 <= info["SweepStaged"] < 4
+    for i, (s, c, f) in enumerate(zip(slots, cfgs, fs)):
+        ref, (d, _), _ = solo(af, c, f, steps)
+        assert_same(slot_state(b, s), ref, f"staged variant {variant} grid {STAGED_GRIDS[i]}")
+        assert b.MaxDivergence(s) == d
+    b.Close()
+
+
+def test_staged_sweep_membership_growth(af):
+    """The staged sweep runs one workgroup per tile, so its grid is sized to
+    the work-list: adding slots past it rebuilds the step graph.  Results
+    stay those of the engines alone."""
+    b = af.Batch(af.BatchConfig(MemoryBudget=512 << 20, SweepStaged=3))
+    grids = [(64, 8, 8, 1), (128, 24, 20, 1), (96, 40, 33, 1)]
+    cfgs = [cfg_for(s, i) for i, s in enumerate(grids)]
+    fs = [staged_fields(i, s, 1.0) for i, s in enumerate(grids)]
+    live = []
+    for i in range(len(grids)):
+        s = b.Add(af.Config(**cfgs[i]))
+        load(b, s, fs[i])
+        live.append(s)
+        b.StepN(1)
+    for i, s in enumerate(live):
+        ref, _, _ = solo(af, cfgs[i], fs[i], len(grids) - i)
+        assert_same(slot_state(b, s), ref, f"engine {i} of a growing staged batch")
+    b.Close()
+
+
 def test_fault_in_one_slot_is_contained(af):
     """A NaN departure coordinate in one slot raises that slot's flag only;
     the other slots' results are unaffected."""
@@ -151,7 +211,7 @@ def test_fault_in_one_slot_is_contained(af):
     fs = [fields_for(s, i) for i, s in enumerate(GRIDS[:3])]
     bad = {k: v.copy() for k, v in fs[1].items()}
     bad["U"][5, 5, 5] = np.nan
-    b = af.Batch(af.BatchConfig(MemoryBudget=128 << 20))
+    b = af.Batch(af.BatchConfig(MemoryBudget=128 << 20, SweepStaged=-1))
     slots = [b.Add(af.Config(**c)) for c in cfgs]
     for s, f in zip(slots, (fs[0], bad, fs[2])):
         load(b, s, f)
@@ -162,6 +222,28 @@ def test_fault_in_one_slot_is_contained(af):
     for i in (0, 2):
         ref, _, _ = solo(af, cfgs[i], fs[i], 2)
         assert_same(slot_state(b, slots[i]), ref, f"healthy slot {i} beside a fault")
+    b.Close()
+
+
+def test_fault_in_a_staged_slot_is_contained(af):
+    """The same on the LDS-staged sweep: the faulting slot (64 wide, staged)
+    is named, the gather and staged slots beside it are unaffected."""
+    grids = [STAGED_GRIDS[0], STAGED_GRIDS[1], STAGED_GRIDS[4]]
+    cfgs = [cfg_for(s, i) for i, s in enumerate(grids)]
+    fs = [staged_fields(i, s, 1.0) for i, s in enumerate(grids)]
+    bad = {k: v.copy() for k, v in fs[0].items()}
+    bad["V"][4, 6, 30] = np.nan
+    b = af.Batch(af.BatchConfig(MemoryBudget=128 << 20, SweepStaged=1))
+    slots = [b.Add(af.Config(**c)) for c in cfgs]
+    for s, f in zip(slots, (bad, fs[1], fs[2])):
+        load(b, s, f)
+    with pytest.raises(af.AeroFluxError) as ei:
+        b.StepN(2)
+    assert ei.value.code == afm.AF_ERR_FAULT
+    assert ei.value.slots == [slots[0]]
+    for i in (1, 2):
+        ref, _, _ = solo(af, cfgs[i], fs[i], 2)
+        assert_same(slot_state(b, slots[i]), ref, f"healthy slot {i} beside a staged fault")
     b.Close()
 
 

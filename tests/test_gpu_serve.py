@@ -54,6 +54,27 @@ def test_served_results_equal_alone(af, prec, realtime):
     b.Close()
 
 
+@pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
+def test_served_staged_sweep_results_equal_alone(af, prec):
+    """Jobs 64 and 96 cells wide run the LDS-staged sweep beside narrow ones
+    on the gather; as jobs arrive the staged work-list outgrows its grid and
+    the step graph is rebuilt between steps.  Every result equals the job
+    run alone."""
+    jobs = workload.served_jobs(10, 11 + prec, rate=400.0, sizes=(16, 64, 96), steps=(1, 4))
+    assert any(j["NX"] >= 64 for j in jobs) and any(j["NX"] < 64 for j in jobs)
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=1 << 30, MaxSlots=16,
+                                SweepStaged=2))
+    res, st = af.Serve(b, jobs, realtime=True)
+    assert st["CellUpdates"] == sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
+    assert b.info()["SweepStaged"] == 1
+    for j, r in zip(jobs, res):
+        assert r["Status"] == afm.AF_OK
+        d, ke = alone(af, prec, j)
+        assert r["MaxDivergence"] == d, j
+        assert r["KineticEnergy"] == ke, j
+    b.Close()
+
+
 def test_served_job_larger_than_pool_is_refused(af):
     jobs = small_stream(4, 9, rate=1000.0)
     jobs[2] = dict(jobs[2], NX=64, NY=64, NZ=64, H=1 / 64)
