@@ -117,6 +117,10 @@ class _CServeStats(C.Structure):
     ]
 
 
+# af_serve_progress: (const af_serve_stats *, int64_t jobs_done, void *user).
+_ServeProgress = C.CFUNCTYPE(None, C.POINTER(_CServeStats), C.c_int64, C.c_void_p)
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported symbol in include/aeroflux.h.
@@ -180,7 +184,7 @@ _SIGNATURES = [
                                               C.c_double, C.c_double]),
     ("af_serve", C.c_int, [C.c_void_p, C.POINTER(_CServeJob), C.c_int64, C.c_int32, C.c_double,
                             C.c_double, C.c_int32, C.POINTER(_CServeResult),
-                            C.POINTER(_CServeStats)]),
+                            C.POINTER(_CServeStats), _ServeProgress, C.c_double, C.c_void_p]),
     ("af_last_error", C.c_char_p, []),
 ]
 
@@ -665,11 +669,14 @@ class Batch:
 
 
 def Serve(batch: Batch, jobs: list[dict], kmax: int = 8, slope: float = -5.0 / 3.0,
-          rms: float = 1.0, realtime: bool = True) -> tuple[list[dict], dict]:
+          rms: float = 1.0, realtime: bool = True, progress=None,
+          progress_interval: float = 30.0) -> tuple[list[dict], dict]:
     """Runs step-N jobs through an empty batch with continuous batching
     (af_serve).  Each job is a dict with NX, NY, NZ, Steps, Viscosity, DT,
     H, Seed, Arrival (seconds; sorted).  Returns per-job results (Admitted,
-    Done, MaxDivergence, KineticEnergy, Status, Slot) and run statistics."""
+    Done, MaxDivergence, KineticEnergy, Status, Slot) and run statistics.
+    progress(stats_so_far, jobs_done), if given, is called between batched
+    steps at most once per progress_interval seconds."""
     n = len(jobs)
     arr = (_CServeJob * max(n, 1))()
     for i, j in enumerate(jobs):
@@ -677,7 +684,14 @@ def Serve(batch: Batch, jobs: list[dict], kmax: int = 8, slope: float = -5.0 / 3
             setattr(arr[i], name, j.get(name, 0))
     res = (_CServeResult * max(n, 1))()
     st = _CServeStats()
+    if progress is None:
+        cb = C.cast(None, _ServeProgress)
+    else:
+        def report(p, done, _user):
+            progress({name: getattr(p.contents, name) for name, _ in _CServeStats._fields_
+                      if name != "Pad"}, int(done))
+        cb = _ServeProgress(report)
     _check(load_library().af_serve(batch._handle(), arr, n, kmax, slope, rms, int(realtime),
-                                   res, C.byref(st)))
+                                   res, C.byref(st), cb, float(progress_interval), None))
     out = [{name: getattr(res[i], name) for name, _ in _CServeResult._fields_} for i in range(n)]
     return out, {name: getattr(st, name) for name, _ in _CServeStats._fields_ if name != "Pad"}

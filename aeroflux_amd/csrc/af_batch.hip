@@ -1465,10 +1465,15 @@ int af_batch_mixed_stats(af_batch *b, int32_t slot, double *last_div, int32_t *c
 int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
 {
     if (int rc = check_open(b)) return rc;
-    if (reps < 1 || !avg_ms || stage < 0 || stage > 3)
+    if (reps < 1 || !avg_ms || stage < 0 || stage > 9)
         return fail(AF_ERR_ARG, "aeroflux: bad timing arguments");
+    const bool mixed = b->cfg.Precision == AF_PRECISION_MIXED;
+    if (stage == 9 && !mixed)
+        return fail(AF_ERR_ARG, "aeroflux: the residual stage needs a mixed-precision batch");
     HIP_TRY(hipSetDevice(b->device), "set device");
     if (int rc = upload_table(b, b->shape)) return rc;
+    const BCtl c = ctl(b);
+    const dim3 gp(b->nwg_other), blk(BX, BY);
     return time_launches(
         b, reps,
         [&]() -> hipError_t {
@@ -1476,7 +1481,22 @@ int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
             case 0: return enq_timed_pass(b, b->nwg_pass);
             case 1: return enq_sweep(b, b->shape, b->nwg_sweep, 0, 0);
             case 2: return enq_div(b, 0);
-            default: return enq_grad(b, 0);
+            case 3: return enq_grad(b, 0);
+            case 4: return enq_vbc(b, 0);
+            case 5: return enq_sweep(b, b->shape, b->nwg_sweep, 1, 0);
+            case 6: return enq_sweep(b, b->shape, b->nwg_sweep, 2, 0);
+            case 7:
+                hipLaunchKernelGGL(kb_pbc, dim3(kSmallGrid), dim3(256), 0, b->stream, c);
+                return hipGetLastError();
+            case 8:
+                if (mixed)
+                    hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, c, 0);
+                else
+                    hipLaunchKernelGGL((kb_from_split<double, false>), gp, blk, 0, b->stream, c, 0);
+                return hipGetLastError();
+            default:
+                hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 0);
+                return hipGetLastError();
             }
         },
         avg_ms);
@@ -1488,7 +1508,8 @@ int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
 constexpr int64_t kServeTuneCells = (int64_t)128 << 20;
 
 int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, double slope,
-             double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats)
+             double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats,
+             af_serve_progress progress, double progress_interval, void *user)
 {
     if (int rc = check_open(b)) return rc;
     if ((!jobs || !out) && n > 0) return fail(AF_ERR_ARG, "aeroflux: null argument");
@@ -1553,7 +1574,8 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
     std::vector<Res> resident;
     std::deque<std::pair<int64_t, std::vector<int64_t>>> pending;
     std::deque<int64_t> queue;
-    int64_t next = 0, step = 0, completed = -1;
+    int64_t next = 0, step = 0, completed = -1, jobs_done = 0;
+    double last_report = 0;
     af_serve_stats st{};
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
@@ -1568,6 +1590,7 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
             out[j].Done = t;
             st.Wall = t;
         }
+        jobs_done += (int64_t)p.second.size();
         completed = p.first;
         pending.pop_front();
         return hipSuccess;
@@ -1700,6 +1723,13 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         pending.emplace_back(step, std::move(done_now));
         step++;
         while (pending.size() > (size_t)DEPTH && err == hipSuccess) err = finish_oldest();
+        if (progress && err == hipSuccess && now() - last_report >= progress_interval) {
+            last_report = now();
+            af_serve_stats so_far = st;
+            so_far.Wall = last_report;
+            so_far.Steps = step;
+            progress(&so_far, jobs_done, user);
+        }
     }
     while (!pending.empty() && err == hipSuccess) err = finish_oldest();
     if (err == hipSuccess) err = hipStreamSynchronize(b->stream);

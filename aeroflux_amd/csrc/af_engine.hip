@@ -197,11 +197,30 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
     return launch_sweep(axis, a, e->stream);
 }
 
-hipError_t enq_divergence(af_engine *e, const Desc &d)
+// Whether the projection's divergence also writes the solve's split
+// right-hand side (the colour-split FP64 solve and the mixed solve read it).
+bool fused_rhs(const af_engine *e)
 {
+    return e->cfg.Precision == AF_PRECISION_MIXED || e->tile64.k == 0;
+}
+
+// computeDivergence; with `fuse` also the solve's split right-hand side.
+hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false)
+{
+    int split = 0;
+    void *red = nullptr, *black = nullptr;
+    if (fuse && e->cfg.Precision == AF_PRECISION_MIXED) {
+        split = 2;
+        red = e->mixed.r_red;
+        black = e->mixed.r_black;
+    } else if (fuse && e->tile64.k == 0) {
+        split = 1;
+        red = e->ds_red;
+        black = e->ds_black;
+    }
     return launch_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
                              dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_DIV]), e->G,
-                             e->div_scale, e->stream);
+                             e->div_scale, e->stream, split, red, black, e->h2);
 }
 
 hipError_t enq_gradient(af_engine *e, const Desc &d)
@@ -269,10 +288,11 @@ SplitArgs<double> split_args(const af_engine *e)
 // the compacted colour arrays (the first from P == 0, which is the solve's
 // zeroing), and the result is merged back into P's interior before the final
 // ghost fill.  Bit-identical to the reference's sequence.
-hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D)
+hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D, bool rhs_ready)
 {
-    hipError_t err = launch_to_split<double, double>(D.d, D.sy, D.sz, e->G, e->ds_red,
-                                                     e->ds_black, e->stream);
+    hipError_t err = rhs_ready ? hipSuccess
+                               : launch_to_split<double, double>(D.d, D.sy, D.sz, e->G, e->ds_red,
+                                                                 e->ds_black, e->stream);
     const SplitArgs<double> a = split_args(e);
     const int passes = (int)(2 * e->cfg.GaussSeidelIters);
     for (int p = 0; p < passes && err == hipSuccess; p++)
@@ -284,12 +304,12 @@ hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D)
     return err;
 }
 
-hipError_t enq_solve_fp64(af_engine *e, const Desc &d)
+hipError_t enq_solve_fp64(af_engine *e, const Desc &d, bool rhs_ready)
 {
     const DField P = dfield(e, d.f[AF_FIELD_P]);
     const DField D = dfield(e, d.f[AF_FIELD_DIV]);
     if (e->tile64.k > 0) return enq_solve_fp64_blocked(e, P, D);
-    if (e->tile64.k == 0) return enq_solve_fp64_split(e, P, D);
+    if (e->tile64.k == 0) return enq_solve_fp64_split(e, P, D, rhs_ready);
     hipError_t err = launch_rbgs_first(P, D, e->G, e->h2, e->stream);
     if (err != hipSuccess) return err;
     err = launch_rbgs(P, D, e->G, e->h2, AF_COLOR_BLACK, true, e->stream);
@@ -301,7 +321,9 @@ hipError_t enq_solve_fp64(af_engine *e, const Desc &d)
     return err;
 }
 
-hipError_t enq_solve(af_engine *e, const Desc &d)
+// rhs_ready: the divergence of this projection already wrote the split
+// right-hand side (enq_divergence with fuse).
+hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 {
     if (e->cfg.Precision == AF_PRECISION_MIXED) {
         MixedCtx c;
@@ -314,17 +336,19 @@ hipError_t enq_solve(af_engine *e, const Desc &d)
         c.cycles = e->cfg.MixedMaxCycles;
         c.target = e->cfg.MixedTarget;
         c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
+        c.rhs_ready = rhs_ready;
         return enqueue_mixed_solve(e->mixed, c, e->stream);
     }
-    return enq_solve_fp64(e, d);
+    return enq_solve_fp64(e, d, rhs_ready);
 }
 
 // project, project.go:12-34
 hipError_t enq_project(af_engine *e, const Desc &d)
 {
+    const bool fuse = fused_rhs(e);
     hipError_t err = enq_velocity_bc(e, d);
-    if (err == hipSuccess) err = enq_divergence(e, d);
-    if (err == hipSuccess) err = enq_solve(e, d);
+    if (err == hipSuccess) err = enq_divergence(e, d, fuse);
+    if (err == hipSuccess) err = enq_solve(e, d, fuse);
     if (err == hipSuccess) err = enq_gradient(e, d);
     if (err == hipSuccess) err = enq_velocity_bc(e, d);
     return err;

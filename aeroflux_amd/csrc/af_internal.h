@@ -39,8 +39,12 @@ hipError_t launch_sweep_lds(int axis, const SweepArgs &a, int variant, hipStream
 // Launchers.  All enqueue on `s` and return the hipGetLastError() code.
 hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t s);
 hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s);
+// split: 0 -> Div only; 1 -> also the FP64 colour-split copy (red, black
+// are double arrays); 2 -> also the FP32 split right-hand side -(h2 * div)
+// of the mixed-precision solve (float arrays).
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
-                             hipStream_t s);
+                             hipStream_t s, int split = 0, void *red = nullptr,
+                             void *black = nullptr, double h2 = 0);
 // Red-black colour pass.  fused_bc: a neighbour outside the interior reads the
 // cell's own value (the Neumann mirror applied just before the pass).
 hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fused_bc,

@@ -34,16 +34,29 @@ __global__ __launch_bounds__(256) void k_velocity_bc(DField U, DField V, DField 
     body::velocity_bc_cell(U, V, W, G, k, cu, cv, cw);
 }
 
-// computeDivergence, project.go:42-62.
+// computeDivergence, project.go:42-62.  SPLIT 1 also writes the value to
+// the colour-split layout of the FP64 solve, SPLIT 2 writes the FP32
+// right-hand side -(H^2 div) of the mixed-precision solve there: the same
+// values k_to_split / k_rhs32_split would form from Div, without reading it
+// back.
+template <int SPLIT>
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
-                                                       Geom G, double scale, int3 nb)
+                                                       Geom G, double scale, int3 nb, void *red,
+                                                       void *black, double h2)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
+    const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, zi = bz;
     double v;
-    body::divergence_cell(U, V, W, D, G, scale, g + bx * BX + threadIdx.x,
-                          g + by * BY + threadIdx.y, g + bz, v);
+    if (!body::divergence_cell(U, V, W, D, G, scale, g + xi, g + yi, g + zi, v) || SPLIT == 0)
+        return;
+    const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
+    void *dst = body::parity(xi, yi, zi, g) ? black : red;
+    if (SPLIT == 1)
+        body::gp(static_cast<double *>(dst))[i] = v;
+    else
+        body::gp(static_cast<float *>(dst))[i] = (float)(-(h2 * v));
 }
 
 // gaussSeidelSweep, project.go:103-130: one lane per cell of the colour.
@@ -250,10 +263,23 @@ hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t 
 }
 
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
-                             hipStream_t s)
+                             hipStream_t s, int split, void *red, void *black, double h2)
 {
     const int3 nb = nblocks(G.NX, G.NY, G.NZ);
-    hipLaunchKernelGGL(k_divergence, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale, nb);
+    switch (split) {
+    case 0:
+        hipLaunchKernelGGL(k_divergence<0>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
+                           nb, red, black, h2);
+        break;
+    case 1:
+        hipLaunchKernelGGL(k_divergence<1>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
+                           nb, red, black, h2);
+        break;
+    default:
+        hipLaunchKernelGGL(k_divergence<2>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
+                           nb, red, black, h2);
+        break;
+    }
     return hipGetLastError();
 }
 

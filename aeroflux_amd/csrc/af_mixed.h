@@ -27,6 +27,7 @@ struct MixedCtx {
     int cycles;        // maximum cycles
     double target;     // max|div| target after the projection (0: one cycle)
     double res_to_div; // DT/(rho*H^2): max|div| = res_to_div * max|residual|
+    bool rhs_ready = false;  // the split right-hand side is already in r_red/r_black
 };
 
 hipError_t init_mixed(MixedState &m, Geom G, hipStream_t s);

@@ -136,8 +136,9 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
 
     hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles);
     // Cycle 0: p = 0, r = b = -(H^2 div).
-    if ((err = launch_rhs32_split(c.D.d, c.D.sy, c.D.sz, G, c.h2, m.r_red, m.r_black, s)) !=
-        hipSuccess)
+    if (!c.rhs_ready &&
+        (err = launch_rhs32_split(c.D.d, c.D.sy, c.D.sz, G, c.h2, m.r_red, m.r_black, s)) !=
+            hipSuccess)
         return err;
     if ((err = smooth(m, c, 2 * c.sweeps, false, s)) != hipSuccess) return err;
     if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, false,

@@ -87,6 +87,12 @@ struct LdsLoad {
         p10 = body::ld2(d + s.sy);
         p01 = body::ld2(d + s.sz);
         p11 = body::ld2(d + s.sz + s.sy);
+        // The fallback's loads complete inside this branch (vmcnt(0)).  The
+        // LDS branch beside it writes the same registers; with these loads
+        // possibly in flight it would have to wait for every outstanding
+        // vector memory operation -- the next plane's prefetch included --
+        // before each face's interpolation.
+        __builtin_amdgcn_s_waitcnt(0x0F70);
     }
 };
 
@@ -99,13 +105,19 @@ struct LdsLoad {
 template <class B, int NT>
 struct Stager {
     static constexpr int N = (3 * B::PLANE + NT - 1) / NT;
-    const __attribute__((address_space(1))) double *src[N];  // element's (x, y) address, plane 0
-    int sz[N];             // its field's plane stride
-    int nz[N];             // its field's plane count
-    int off[N];            // LDS offset within a ring slot
-    bool ok[N];            // inside the box and inside the field
+    static_assert(NT <= 3 * B::PLANE, "a lane past the box repeats an element of it");
+    const __attribute__((address_space(1))) double *src[N];  // element's (x, y), plane 0
+    int sz[N];    // its field's plane stride
+    int zmax[N];  // its field's last plane
+    int off[N];   // LDS offset within a ring slot
     double v[N];
 
+    // No lane branches: lanes past the box repeat an element of it (same
+    // load, same value to the same slot), and elements outside their field
+    // (columns, rows or planes beyond the padded field) load a clamped
+    // in-field value into a slot no face reads -- LdsLoad reads only cells
+    // of the field.
+    //
     // src(c) gives source field c's descriptor.  It must read memory that
     // can be indexed at run time (kernel arguments, global memory): selecting
     // among the members of a local copy would force that copy into scratch.
@@ -115,32 +127,35 @@ struct Stager {
         const int t = threadIdx.y * blockDim.x + threadIdx.x;
 #pragma unroll
         for (int k = 0; k < N; k++) {
-            const int e = t + k * NT;
-            const int c = e < 3 * B::PLANE ? e / B::PLANE : 0;
+            const int e0 = t + k * NT;
+            const int e = e0 < 3 * B::PLANE ? e0 : e0 - 3 * B::PLANE;
+            const int c = e / B::PLANE;
             const int r = (e - c * B::PLANE) / B::XW;
             const int col = e - c * B::PLANE - r * B::XW;
             const DField &f = src_of(c);
             const int x = ox + col, y = oy + r;
-            ok[k] = e < 3 * B::PLANE && (unsigned)x < (unsigned)f.nx && (unsigned)y < (unsigned)f.ny;
-            src[k] = body::gp((const double *)f.d) + (ok[k] ? x + y * f.sy : 0);
+            const int xc = x < 0 ? 0 : (x >= f.nx ? f.nx - 1 : x);
+            const int yc = y < 0 ? 0 : (y >= f.ny ? f.ny - 1 : y);
+            src[k] = body::gp((const double *)f.d) + xc + yc * f.sy;
             sz[k] = f.sz;
-            nz[k] = f.nz;
+            zmax[k] = f.nz - 1;
             off[k] = c * B::FIELD + r * B::XW + col;
         }
     }
     __device__ __forceinline__ void fetch(int z)
     {
 #pragma unroll
-        for (int k = 0; k < N; k++)
-            if (ok[k] && (unsigned)z < (unsigned)nz[k]) v[k] = src[k][(size_t)z * sz[k]];
+        for (int k = 0; k < N; k++) {
+            const int zc = z < 0 ? 0 : (z > zmax[k] ? zmax[k] : z);
+            v[k] = src[k][(size_t)zc * sz[k]];
+        }
     }
     __device__ __forceinline__ void store(lds_t *lds, int z) const
     {
         if (z < 0) return;
         const int base = (z % B::R) * B::PLANE;
 #pragma unroll
-        for (int k = 0; k < N; k++)
-            if (ok[k] && z < nz[k]) lds[base + off[k]] = v[k];
+        for (int k = 0; k < N; k++) lds[base + off[k]] = v[k];
     }
 };
 
@@ -195,22 +210,36 @@ __device__ __forceinline__ void tile(const SweepArgs &a, const Src &src_of, lds_
         st.store(lds, zs - B::ZLO + q);
     }
     st.fetch(zs - B::ZLO + B::R - 1);
+    // A plane's results are stored at the start of the next iteration, just
+    // before that iteration's prefetch, so the prefetch is the last vector
+    // memory operation in flight when the following iteration waits for it:
+    // that wait then never includes the write-back of the faces.
+    double vu = 0, vv = 0, vw = 0;
+    bool du = false, dv = false, dw = false;
+    int zp = zs;
+    auto put = [&]() {
+        if (du) body::gp(a.T0.d)[body::idx(a.T0, x, y, zp)] = vu;
+        if (dv) body::gp(a.T1.d)[body::idx(a.T1, x, y, zp)] = vv;
+        if (dw) body::gp(a.T2.d)[body::idx(a.T2, x, y, zp)] = vw;
+    };
     for (int z = zs; z < zend; z++) {
         st.store(lds, z - B::ZLO + B::R - 1);
         __syncthreads();
+        put();
         if (z + 1 < zend) st.fetch(z + 1 - B::ZLO + B::R - 1);
         const LdsLoad<B> ld{a, lds, ox, oy, z - B::ZLO};
         const bool zin = z < ze;
-        double vu = 0, vv = 0, vw = 0;
-        const bool du = xin && yin && zin, dv = xin && yok && zin, dw = xin && yin;
+        vu = vv = vw = 0;
+        du = xin && yin && zin;
+        dv = xin && yok && zin;
+        dw = xin && yin;
+        zp = z;
         if (du) vu = body::advect_value<AX, 0>(a, ld, x, y, z, bad);
         if (dv) vv = body::advect_value<AX, 1>(a, ld, x, y, z, bad);
         if (dw) vw = body::advect_value<AX, 2>(a, ld, x, y, z, bad);
-        if (du) body::gp(a.T0.d)[body::idx(a.T0, x, y, z)] = vu;
-        if (dv) body::gp(a.T1.d)[body::idx(a.T1, x, y, z)] = vv;
-        if (dw) body::gp(a.T2.d)[body::idx(a.T2, x, y, z)] = vw;
         __syncthreads();
     }
+    put();
 }
 
 }  // namespace slds

@@ -182,7 +182,16 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
         hip_sync()
         el = time.perf_counter() - t0
         d = np.array([b.MaxDivergence(s) for s in slots])
+        cyc = ([b.mixed_stats(s)[1] for s in slots] if prec == af.PRECISION_MIXED else None)
         t_pass = b.time_stage(0, 20)
+        # Every stage of the step, timed alone over all slots (the divergence
+        # and gradient last: they change Div and the velocity).
+        sw = [b.time_stage(k, 5) for k in (1, 5, 6)]
+        t_vbc, t_pbc, t_merge = b.time_stage(4, 10), b.time_stage(7, 10), b.time_stage(8, 10)
+        t_res = b.time_stage(9, 10) if prec == af.PRECISION_MIXED else 0.0
+        t_div, t_grad = b.time_stage(2, 10), b.time_stage(3, 10)
+        npass = 2 * (bc_extra.get("MixedSweeps", 24) if prec == af.PRECISION_MIXED else 24)
+        model = 10 * t_vbc + 2 * sum(sw) + 2 * (t_div + npass * t_pass + t_merge + t_pbc + t_grad)
         info = b.info()
         pass_bytes = (6 if prec == af.PRECISION_MIXED else 12) * cells
         res = {
@@ -195,14 +204,19 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
                               "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": pass_bytes / (t_pass * 1e-3) / 1e9 / HBM_PEAK_GBPS,
                               "algorithmic_bytes_per_launch": pass_bytes},
-            "stages": {"rbgs_pass_ms": t_pass, "advect_sweep_ms": b.time_stage(1, 5)},
+            "stages": {"rbgs_pass_ms": t_pass, "advect_sweep_ms": sw[0],
+                       "advect_sweep_xyz_ms": sw, "velocity_bc_ms": t_vbc,
+                       "divergence_ms": t_div, "gradient_ms": t_grad, "merge_ms": t_merge,
+                       "pressure_bc_ms": t_pbc, "residual_ms": t_res,
+                       "one_cycle_step_model_ms": model,
+                       "model": "10 velocity BCs + 2 x (X+Y+Z sweeps) + 2 x (divergence + "
+                                f"{npass} passes + merge + pressure BC + gradient)"},
             "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
                       "staged_slots": info["StagedSlots"],
                       "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
             "pool_bytes": info["BytesInUse"],
         }
-        if prec == af.PRECISION_MIXED:
-            cyc = [b.mixed_stats(s)[1] for s in slots]
+        if cyc is not None:
             res["cycles_last_solve"] = {"min": int(min(cyc)), "max": int(max(cyc))}
         b.Close()
         rel = (d - d_plain) / d_plain
@@ -280,7 +294,11 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dic
     jobs = workload.served_jobs(n_jobs, seed=7, rate=rate)
     b = af.Batch(af.BatchConfig(Precision=af.PRECISION_MIXED, MemoryBudget=int(budget),
                                 MaxSlots=4096))
-    res, st = af.Serve(b, jobs, realtime=True)
+    def report(so_far, done):
+        print(f"served: {so_far['Wall']:.0f} s, {done}/{n_jobs} jobs done, "
+              f"{so_far['Steps']} batched steps", file=sys.stderr, flush=True)
+
+    res, st = af.Serve(b, jobs, realtime=True, progress=report, progress_interval=30.0)
     info = b.info()
     b.Close()
     plain = None

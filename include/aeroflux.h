@@ -317,10 +317,15 @@ int af_batch_mixed_stats(af_batch *b, int32_t slot, double *last_div, int32_t *c
 int af_batch_tune(af_batch *b);
 
 /* Average device time (ms) of `reps` launches of one batched stage over all
- * resident slots: 0 = red-black colour pass, 1 = advection sweep (X axis),
- * 2 = divergence, 3 = gradient subtract.  Measurement only: 0 and 1 leave
- * the state a step reads unchanged (1 writes to the Tmp fields what the next
- * step's first sweep writes); 2 rewrites Div and 3 changes the velocity. */
+ * resident slots: 0 = red-black colour pass, 1, 5, 6 = advection sweep along
+ * X, Y, Z, 2 = divergence, 3 = gradient subtract, 4 = velocity BC, 7 =
+ * pressure ghost fill, 8 = merge of the solve into P, 9 = FP64 residual of
+ * the mixed-precision solve (mixed batches only).  Measurement only: 0, 1,
+ * 4-7 and 9 leave the state a step reads unchanged (the sweeps write the
+ * Tmp fields, which a step rewrites before reading; 9 writes solve buffers
+ * the next divergence rewrites); 2 rewrites Div, 3 changes the velocity and
+ * 8 rewrites P's interior from the last solve's buffers (which it already
+ * equals unless the batch iterates checked cycles). */
 int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms);
 
 /* -------------------------------------------------------------- serving */
@@ -362,13 +367,21 @@ typedef struct af_serve_stats {
     int64_t PeakBytes;         /* peak pool bytes in use */
 } af_serve_stats;
 
+/* Progress of a served run: statistics so far (Wall = seconds since the
+ * start, Steps = batched steps enqueued, CellUpdates = work of the jobs
+ * retired) and the number of jobs complete. */
+typedef void (*af_serve_progress)(const af_serve_stats *so_far, int64_t jobs_done, void *user);
+
 /* Runs `n` jobs (sorted by Arrival) through batch `b`, which must be empty.
  * Initial conditions: af_init_band_limited(seed, kmax, slope, rms).
  * realtime = 1: a job is visible only from its arrival time on the wall
  * clock; 0: all jobs are queued at the start.  Blocks until every job is
- * done; out[i] receives job i's result. */
+ * done; out[i] receives job i's result.  If `progress` is not null it is
+ * called on the calling thread between batched steps, at most once per
+ * `progress_interval` seconds. */
 int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, double slope,
-             double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats);
+             double rms, int32_t realtime, af_serve_result *out, af_serve_stats *stats,
+             af_serve_progress progress, double progress_interval, void *user);
 
 /* ------------------------------------------------- initial conditions */
 /* Seeded band-limited random velocity, built on the GPU (no reference

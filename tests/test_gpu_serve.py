@@ -75,6 +75,25 @@ def test_served_staged_sweep_results_equal_alone(af, prec):
     b.Close()
 
 
+def test_serve_progress_reports(af):
+    """The progress callback runs between batched steps with the running
+    statistics: steps and completed jobs never decrease and stay within the
+    run's totals."""
+    jobs = small_stream(8, 3, rate=500.0)
+    b = af.Batch(af.BatchConfig(MemoryBudget=64 << 20, MaxSlots=16))
+    seen = []
+    res, st = af.Serve(b, jobs, realtime=False, progress=lambda s, d: seen.append((s, d)),
+                       progress_interval=0.0)
+    assert seen and len(seen) == st["Steps"]
+    steps = [s["Steps"] for s, _ in seen]
+    done = [d for _, d in seen]
+    assert steps == sorted(steps) and done == sorted(done)
+    assert steps[-1] == st["Steps"] and done[-1] <= len(jobs)
+    assert all(s["CellUpdates"] <= st["CellUpdates"] for s, _ in seen)
+    assert all(r["Status"] == afm.AF_OK for r in res)
+    b.Close()
+
+
 def test_served_job_larger_than_pool_is_refused(af):
     jobs = small_stream(4, 9, rate=1000.0)
     jobs[2] = dict(jobs[2], NX=64, NY=64, NZ=64, H=1 / 64)
