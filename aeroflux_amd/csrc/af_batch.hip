@@ -838,8 +838,10 @@ hipError_t enq_step(const af_batch *b)
         sw ^= 1;
     }
     if (err == hipSuccess) err = enq_project(b, sw);
+    // The projection's closing velocity BC already holds for the first
+    // reverse sweep (the BC is idempotent), as in the single engine's step.
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
-        err = enq_vbc(b, sw);
+        if (k > 0) err = enq_vbc(b, sw);
         if (err == hipSuccess) err = enq_sweep(b, b->shape, b->nwg_sweep, rev[k], sw);
         sw ^= 1;
     }

@@ -235,12 +235,20 @@ __device__ __forceinline__ double axis_velocity(const L &ld, int x, int y, int z
                    ld.one(2, x, y, z + 1));
 }
 
-// advectComponent's value at one face (stencil.go:83-115).  `bad` is set
-// when the departure coordinate is NaN -- the reference's bounds-panic
-// trigger; the value is then NaN and no load leaves the field.
+// advectComponent's value at one face (stencil.go:83-115), in three parts
+// that the LDS-staged sweep also calls separately: the departure point
+// (tap base and weights), the trilinear combination of the eight taps, and
+// the diffusion term.  `bad` is set when the departure coordinate is NaN --
+// the reference's bounds-panic trigger; the value is then NaN and no load
+// leaves the field.
+struct Departure {
+    int x0, y0, z0;      // tap base
+    double tx, ty, tz;   // weights (0 on the axes the backtrace does not move)
+};
+
 template <int AX, int C, class L>
-__device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, int x, int y, int z,
-                                               bool &bad)
+__device__ __forceinline__ Departure departure(const SweepArgs &a, const L &ld, int x, int y,
+                                               int z, bool &nan)
 {
     const DField &s = C == 0 ? a.U : (C == 1 ? a.V : a.W);
     const double v = axis_velocity<AX, C>(ld, x, y, z);
@@ -250,28 +258,38 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, 
     p -= a.dt * v * a.invH;
     const double pmax = (double)(ext - 1);
     p = p < 0 ? 0.0 : (p > pmax ? pmax : p);  // NaN fails both tests
-    const bool nan = p != p;
-    bad = bad || nan;
+    nan = p != p;
     int i0 = nan ? 0 : (int)p;
     i0 = i0 >= ext - 1 ? ext - 2 : i0;
     const double t = p - (double)i0;
+    Departure d;
+    d.x0 = AX == 0 ? i0 : x;
+    d.y0 = AX == 1 ? i0 : y;
+    d.z0 = AX == 2 ? i0 : z;
+    d.tx = AX == 0 ? t : 0.0;
+    d.ty = AX == 1 ? t : 0.0;
+    d.tz = AX == 2 ? t : 0.0;
+    return d;
+}
 
-    const int x0 = AX == 0 ? i0 : x;
-    const int y0 = AX == 1 ? i0 : y;
-    const int z0 = AX == 2 ? i0 : z;
-    const double tx = AX == 0 ? t : 0.0;
-    const double ty = AX == 1 ? t : 0.0;
-    const double tz = AX == 2 ? t : 0.0;
-    double2 p00, p10, p01, p11;  // (d000, d100), (d010, d110), (d001, d101), (d011, d111)
-    ld.taps(C, x0, y0, z0, p00, p10, p01, p11);
+// Taps as (d000, d100), (d010, d110), (d001, d101), (d011, d111).
+__device__ __forceinline__ double trilinear(const double2 &p00, const double2 &p10,
+                                            const double2 &p01, const double2 &p11,
+                                            const Departure &d)
+{
+    const double tx = d.tx, ty = d.ty, tz = d.tz;
     const double c00 = p00.x * (1 - tx) + p00.y * tx;
     const double c10 = p10.x * (1 - tx) + p10.y * tx;
     const double c01 = p01.x * (1 - tx) + p01.y * tx;
     const double c11 = p11.x * (1 - tx) + p11.y * tx;
     const double c0 = c00 * (1 - ty) + c10 * ty;
     const double c1 = c01 * (1 - ty) + c11 * ty;
-    const double adv = c0 * (1 - tz) + c1 * tz;
+    return c0 * (1 - tz) + c1 * tz;
+}
 
+template <int AX, int C, class L>
+__device__ __forceinline__ double diffusion(const SweepArgs &a, const L &ld, int x, int y, int z)
+{
     double dm, dc, dp;  // src at i - st, i, i + st
     if (AX == 0) {
         const double2 q = ld.pair(C, x - 1, y, z);
@@ -288,8 +306,20 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, 
         dp = ld.one(C, x, y, z + 1);
     }
     double lap = a.lam * (dm - 2 * dc + dp);
-    lap = a.lam != 0 ? lap : 0.0;  // `var lap float64` stays +0 when lam == 0
-    return adv + lap;
+    return a.lam != 0 ? lap : 0.0;  // `var lap float64` stays +0 when lam == 0
+}
+
+template <int AX, int C, class L>
+__device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, int x, int y, int z,
+                                               bool &bad)
+{
+    bool nan;
+    const Departure d = departure<AX, C>(a, ld, x, y, z, nan);
+    bad = bad || nan;
+    double2 p00, p10, p01, p11;
+    ld.taps(C, d.x0, d.y0, d.z0, p00, p10, p01, p11);
+    const double adv = trilinear(p00, p10, p01, p11, d);
+    return adv + diffusion<AX, C>(a, ld, x, y, z);
 }
 
 // One thread of advectDiffuseAxis (stencil.go:21-25): the U, V and W faces

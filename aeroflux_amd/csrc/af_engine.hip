@@ -368,8 +368,12 @@ hipError_t enq_step(af_engine *e, Desc &d)
         commit(d);
     }
     if (err == hipSuccess) err = enq_project(e, d);
+    // The projection ends with applyVelocityBC and nothing touches the
+    // velocity before the reverse sweeps' first applyVelocityBC.  The BC
+    // writes only cells it never reads, so applying it twice leaves the
+    // values of applying it once: the second application is skipped.
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
-        err = enq_velocity_bc(e, d);
+        if (k > 0) err = enq_velocity_bc(e, d);
         if (err == hipSuccess) err = enq_sweep(e, d, rev[k], half);
         commit(d);
     }

@@ -36,10 +36,10 @@ struct Box {
 };
 
 // Serves a face's values from the LDS ring.  The velocity averages and the
-// diffusion stencil of a face inside the tile always lie in the staged box
-// (the box is built from them), so they are read from LDS unchecked; the
+// diffusion stencil of any position of the tile lie in the staged box (the
+// box is built from them), so they are read from LDS unchecked; the
 // trilinear taps are read from LDS when their 2x2x2 cube lies in the box and
-// inside the field, else from global memory.
+// inside the field, else from global memory (by the caller).
 template <class B>
 struct LdsLoad {
     const SweepArgs &a;
@@ -63,36 +63,28 @@ struct LdsLoad {
         const int k = slot(c, x, y, z);
         return make_double2(lds[k], lds[k + 1]);
     }
-    __device__ __forceinline__ void taps(int c, int x, int y, int z, double2 &p00, double2 &p10,
-                                         double2 &p01, double2 &p11) const
+    // The eight trilinear taps at base (x, y, z) from LDS when their 2x2x2
+    // cube lies in the box and inside the field (returns true); otherwise
+    // returns false and reads an arbitrary staged slot, and the caller takes
+    // the taps from global memory.  No branch: the three faces of a plane
+    // issue their LDS reads together.
+    __device__ __forceinline__ bool taps_staged(int c, int x, int y, int z, double2 &p00,
+                                                double2 &p10, double2 &p01, double2 &p11) const
     {
         const DField &s = f(c);
         // Base and base+1 inside the box on every axis, and inside the field
         // (the box is clipped to the field when staged).
-        const bool in = (unsigned)(x - ox) < (unsigned)(B::XW - 1) &&
-                        (unsigned)(y - oy) < (unsigned)(B::YW - 1) &&
-                        (unsigned)(z - zlo) < (unsigned)(B::R - 1) && x >= 0 && x + 1 < s.nx &&
-                        y >= 0 && y + 1 < s.ny && z >= 0 && z + 1 < s.nz;
-        if (in) {
-            const int k = slot(c, x, y, z);
-            const int k1 = slot(c, x, y, z + 1);
-            p00 = make_double2(lds[k], lds[k + 1]);
-            p10 = make_double2(lds[k + B::XW], lds[k + B::XW + 1]);
-            p01 = make_double2(lds[k1], lds[k1 + 1]);
-            p11 = make_double2(lds[k1 + B::XW], lds[k1 + B::XW + 1]);
-            return;
-        }
-        const double *d = s.d + body::idx(s, x, y, z);
-        p00 = body::ld2(d);
-        p10 = body::ld2(d + s.sy);
-        p01 = body::ld2(d + s.sz);
-        p11 = body::ld2(d + s.sz + s.sy);
-        // The fallback's loads complete inside this branch (vmcnt(0)).  The
-        // LDS branch beside it writes the same registers; with these loads
-        // possibly in flight it would have to wait for every outstanding
-        // vector memory operation -- the next plane's prefetch included --
-        // before each face's interpolation.
-        __builtin_amdgcn_s_waitcnt(0x0F70);
+        const bool in = ((unsigned)(x - ox) < (unsigned)(B::XW - 1)) &
+                        ((unsigned)(y - oy) < (unsigned)(B::YW - 1)) &
+                        ((unsigned)(z - zlo) < (unsigned)(B::R - 1)) & (x >= 0) &
+                        (x + 1 < s.nx) & (y >= 0) & (y + 1 < s.ny) & (z >= 0) & (z + 1 < s.nz);
+        const int k = in ? slot(c, x, y, z) : c * B::FIELD;
+        const int k1 = in ? slot(c, x, y, z + 1) : c * B::FIELD;
+        p00 = make_double2(lds[k], lds[k + 1]);
+        p10 = make_double2(lds[k + B::XW], lds[k + B::XW + 1]);
+        p01 = make_double2(lds[k1], lds[k1 + 1]);
+        p11 = make_double2(lds[k1 + B::XW], lds[k1 + B::XW + 1]);
+        return in;
     }
 };
 
@@ -186,6 +178,35 @@ __device__ __forceinline__ void wall_face(const SweepArgs &a, int k, bool &bad)
     body::gp(a.T0.d)[body::idx(a.T0, xe, y, z)] = v;
 }
 
+// A face of the current plane formed from LDS (body::advect_value's parts,
+// same arithmetic): departure, taps if staged, and diffusion term.
+struct Face {
+    body::Departure d;
+    double adv, lap;
+    bool far, nan;  // taps beyond the staged box; NaN departure coordinate
+};
+
+template <int AX, int C, class B>
+__device__ __forceinline__ Face face(const SweepArgs &a, const LdsLoad<B> &ld, int x, int y, int z)
+{
+    Face f;
+    f.d = body::departure<AX, C>(a, ld, x, y, z, f.nan);
+    double2 p00, p10, p01, p11;
+    f.far = !ld.taps_staged(C, f.d.x0, f.d.y0, f.d.z0, p00, p10, p01, p11);
+    f.adv = body::trilinear(p00, p10, p01, p11, f.d);
+    f.lap = body::diffusion<AX, C>(a, ld, x, y, z);
+    return f;
+}
+
+// The trilinear value from global memory, for a face whose taps lie beyond
+// the staged box.
+__device__ __forceinline__ double far_taps(const DField &s, const body::Departure &d)
+{
+    const double *p = s.d + body::idx(s, d.x0, d.y0, d.z0);
+    return body::trilinear(body::ld2(p), body::ld2(p + s.sy), body::ld2(p + s.sz),
+                           body::ld2(p + s.sz + s.sy), d);
+}
+
 // One tile (bx, by, bz), computed by the whole workgroup (64 x BY threads)
 // (src_of as for Stager::init):
 // the faces of x in [g+64bx, g+64bx+64) (x < g+NX), y = g+BY*by+ty
@@ -229,14 +250,27 @@ __device__ __forceinline__ void tile(const SweepArgs &a, const Src &src_of, lds_
         if (z + 1 < zend) st.fetch(z + 1 - B::ZLO + B::R - 1);
         const LdsLoad<B> ld{a, lds, ox, oy, z - B::ZLO};
         const bool zin = z < ze;
-        vu = vv = vw = 0;
         du = xin && yin && zin;
         dv = xin && yok && zin;
         dw = xin && yin;
         zp = z;
-        if (du) vu = body::advect_value<AX, 0>(a, ld, x, y, z, bad);
-        if (dv) vv = body::advect_value<AX, 1>(a, ld, x, y, z, bad);
-        if (dw) vw = body::advect_value<AX, 2>(a, ld, x, y, z, bad);
+        // Every lane forms each face from LDS without a branch (positions
+        // outside the grid read staged slots and are not stored); a face
+        // whose taps lie beyond the staged box then takes them from global
+        // memory.  Faces go one after the other: all three at once would
+        // hold three faces' taps in registers and halve the waves per SIMD.
+        Face f = face<AX, 0>(a, ld, x, y, z);
+        if (du && f.far) f.adv = far_taps(a.U, f.d);
+        bad = bad || (du && f.nan);
+        vu = f.adv + f.lap;
+        f = face<AX, 1>(a, ld, x, y, z);
+        if (dv && f.far) f.adv = far_taps(a.V, f.d);
+        bad = bad || (dv && f.nan);
+        vv = f.adv + f.lap;
+        f = face<AX, 2>(a, ld, x, y, z);
+        if (dw && f.far) f.adv = far_taps(a.W, f.d);
+        bad = bad || (dw && f.nan);
+        vw = f.adv + f.lap;
         __syncthreads();
     }
     put();
