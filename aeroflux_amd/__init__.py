@@ -68,7 +68,7 @@ class _CConfig(C.Structure):
         ("Precision", C.c_int32), ("MixedMaxCycles", C.c_int32), ("MixedSweeps", C.c_int32),
         ("MixedFixSweeps", C.c_int32), ("MixedTarget", C.c_double),
         ("TileK", C.c_int32), ("TileRows", C.c_int32), ("TileZ", C.c_int32),
-        ("SweepShape", C.c_int32), ("SweepSlab", C.c_int32),
+        ("SweepShape", C.c_int32), ("SweepSlab", C.c_int32), ("PassSlab", C.c_int32),
     ]
 
 
@@ -159,6 +159,7 @@ _SIGNATURES = [
     ("af_sweep_shape", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]),
     ("af_sweep_slab", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    ("af_pass_slab", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     ("af_batch_new", C.c_int, [C.POINTER(_CBatchConfig), C.POINTER(C.c_void_p)]),
     ("af_batch_close", C.c_int, [C.c_void_p]),
     ("af_batch_free", None, [C.c_void_p]),
@@ -240,6 +241,7 @@ class Config:
     TileZ: int = 0
     SweepShape: int = 0
     SweepSlab: int = 0
+    PassSlab: int = 0
 
     def _to_c(self) -> _CConfig:
         c = _CConfig()
@@ -467,6 +469,11 @@ class Engine:
     def sweep_slab(self) -> int:
         r = C.c_int32()
         _check(load_library().af_sweep_slab(self._handle(), C.byref(r)))
+        return int(r.value)
+
+    def pass_slab(self) -> int:
+        r = C.c_int32()
+        _check(load_library().af_pass_slab(self._handle(), C.byref(r)))
         return int(r.value)
 
     def mixed_stats(self) -> tuple[float, int]:

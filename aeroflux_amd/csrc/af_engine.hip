@@ -134,6 +134,7 @@ struct af_engine {
     Tile tile64;                // red-black schedule (see Tile)
     int sweep_shape = 4;        // advection workgroup shape (tuned at New)
     int sweep_slab = 1 << 30;   // advection y-slab rows of the block order (tuned at New)
+    int pass_slab = 1 << 30;    // colour-pass y-slab (workgroup rows) of the block order (tuned)
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
     ICScratch ic;               // initial-condition generator scratch (lazy)
@@ -281,6 +282,7 @@ SplitArgs<double> split_args(const af_engine *e)
     a.rhs_black = e->ds_black;
     a.NX = e->G.NX; a.NY = e->G.NY; a.NZ = e->G.NZ; a.NXH = (e->G.NX + 1) / 2; a.g = e->G.g;
     a.h2 = e->h2;
+    a.slab = e->pass_slab;
     return a;
 }
 
@@ -337,6 +339,7 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
         c.target = e->cfg.MixedTarget;
         c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
         c.rhs_ready = rhs_ready;
+        c.slab = e->pass_slab;
         return enqueue_mixed_solve(e->mixed, c, e->stream);
     }
     return enq_solve_fp64(e, d, rhs_ready);
@@ -591,6 +594,83 @@ int choose_sweep_shape(af_engine *e)
     return AF_OK;
 }
 
+// The colour passes' workgroup order: plane order, or y-slabs of 2-32
+// workgroup rows walked through z.  A pass reads the planes above and below
+// each cell, and once three planes of the other colour no longer fit an
+// XCD's L2 the plane order fetches them again; a slab keeps them.  Timed on
+// the engine's own passes (FP64 split, or the mixed solve's FP32 ones), which
+// touch only solve buffers every solve rewrites before reading; cached per
+// grid and precision.
+std::map<std::tuple<int, int, int, int, int>, int> g_pass_cache;
+
+int choose_pass_slab(af_engine *e)
+{
+    const bool mixed = e->cfg.Precision == AF_PRECISION_MIXED;
+    if (e->cfg.PassSlab < -1)
+        return fail(AF_ERR_CONFIG, "aeroflux: pass slab %d must be -1, 0 or positive",
+                    (int)e->cfg.PassSlab);
+    if (e->cfg.PassSlab != 0) {
+        e->pass_slab = e->cfg.PassSlab < 0 ? kPlaneOrder : (int)e->cfg.PassSlab;
+        return AF_OK;
+    }
+    if (!mixed && e->tile64.k != 0) return AF_OK;
+    const Geom &G = e->G;
+    const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g, (int)mixed);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_pass_cache.find(key);
+        if (it != g_pass_cache.end()) {
+            e->pass_slab = it->second;
+            return AF_OK;
+        }
+    }
+    static const int kSlabs[] = {kPlaneOrder, 2, 4, 8, 16, 32};
+    SplitArgs<double> a64 = split_args(e);
+    SplitArgs<float> a32;
+    if (mixed) {
+        a32.red = e->mixed.e_red;
+        a32.black = e->mixed.e_black;
+        a32.rhs_red = e->mixed.r_red;
+        a32.rhs_black = e->mixed.r_black;
+        a32.NX = G.NX; a32.NY = G.NY; a32.NZ = G.NZ; a32.NXH = (G.NX + 1) / 2; a32.g = G.g;
+        a32.h2 = 0;
+    }
+    auto pass = [&](int sl) -> hipError_t {
+        if (mixed) {
+            a32.slab = sl;
+            return launch_split_pass<float>(a32, AF_COLOR_RED, 0, e->stream);
+        }
+        a64.slab = sl;
+        return launch_split_pass<double>(a64, AF_COLOR_RED, 0, e->stream);
+    };
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0), "create event");
+    HIP_TRY(hipEventCreate(&t1), "create event");
+    float best = 1e30f;
+    int bl = kPlaneOrder;
+    hipError_t err = hipSuccess;
+    for (int sl : kSlabs) {
+        if (err == hipSuccess) err = pass(sl);
+        if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
+        for (int r = 0; r < 6 && err == hipSuccess; r++) err = pass(sl);
+        if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(t1);
+        float ms = 0;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+        if (err == hipSuccess && ms < best) {
+            best = ms;
+            bl = sl;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (err != hipSuccess) return hip_fail(err, "tune colour-pass order");
+    e->pass_slab = bl;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_pass_cache[key] = bl;
+    return AF_OK;
+}
+
 int choose_schedule(af_engine *e)
 {
     const af_config &c = e->cfg;
@@ -792,6 +872,7 @@ int af_new(const af_config *cfg_in, af_engine **out)
         return bail(err, "initialise engine", AF_ERR_HIP);
     int rc = choose_schedule(e);
     if (rc == AF_OK) rc = choose_sweep_shape(e);
+    if (rc == AF_OK) rc = choose_pass_slab(e);
     if (rc != AF_OK) {
         const std::string msg = af::last_error();
         release(e);
@@ -1111,6 +1192,13 @@ int af_sweep_slab(const af_engine *e, int32_t *rows)
     return AF_OK;
 }
 
+int af_pass_slab(const af_engine *e, int32_t *rows)
+{
+    if (!e || !rows) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *rows = e->pass_slab >= (1 << 30) ? -1 : e->pass_slab;
+    return AF_OK;
+}
+
 int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles)
 {
     STAGE_BEGIN(e);
@@ -1139,10 +1227,23 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
     if (reps < 1 || !avg_ms) return fail(AF_ERR_ARG, "aeroflux: bad timing arguments");
     const Desc &d = e->cur;
     const DField P = dfield(e, d.f[AF_FIELD_P]), D = dfield(e, d.f[AF_FIELD_DIV]);
-    const bool split = e->tile64.k == 0 && e->cfg.Precision == AF_PRECISION_FP64;
+    const bool mixed = e->cfg.Precision == AF_PRECISION_MIXED;
+    const bool split = e->tile64.k == 0 && !mixed;
+    SplitArgs<float> a32;
+    if (mixed) {
+        a32.red = e->mixed.e_red;
+        a32.black = e->mixed.e_black;
+        a32.rhs_red = e->mixed.r_red;
+        a32.rhs_black = e->mixed.r_black;
+        a32.NX = e->G.NX; a32.NY = e->G.NY; a32.NZ = e->G.NZ; a32.NXH = (e->G.NX + 1) / 2;
+        a32.g = e->G.g;
+        a32.h2 = 0;
+        a32.slab = e->pass_slab;
+    }
     auto one = [&]() -> hipError_t {
         switch (kernel) {
         case 0:
+            if (mixed) return launch_split_pass<float>(a32, AF_COLOR_RED, 0, e->stream);
             if (split) return launch_split_pass<double>(split_args(e), AF_COLOR_RED, 0, e->stream);
             if (e->tile64.k > 0) {
                 TbArgs<double> a = tb_args(e, P, D);

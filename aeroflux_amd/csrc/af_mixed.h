@@ -28,6 +28,7 @@ struct MixedCtx {
     double target;     // max|div| target after the projection (0: one cycle)
     double res_to_div; // DT/(rho*H^2): max|div| = res_to_div * max|residual|
     bool rhs_ready = false;  // the split right-hand side is already in r_red/r_black
+    int slab = 1 << 30;      // workgroup y-slab order of the FP32 passes (SplitArgs::slab)
 };
 
 hipError_t init_mixed(MixedState &m, Geom G, hipStream_t s);

@@ -88,6 +88,7 @@ hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipS
     a.NX = c.G.NX; a.NY = c.G.NY; a.NZ = c.G.NZ; a.NXH = (c.G.NX + 1) / 2; a.g = c.G.g;
     a.h2 = 0;
     a.done = gated ? m.done : nullptr;
+    a.slab = c.slab;
     for (int p = 0; p < passes; p++) {
         hipError_t err = launch_split_pass<float>(a, p & 1, p == 0 ? 1 : (p == 1 ? 2 : 0), s);
         if (err != hipSuccess) return err;

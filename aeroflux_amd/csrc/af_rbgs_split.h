@@ -12,6 +12,8 @@ struct SplitArgs {
     int NX, NY, NZ, NXH, g;         // NXH = (NX + 1) / 2
     double h2;
     const int *done = nullptr;      // if set and *done != 0, the launch does nothing
+    int slab = 1 << 30;             // workgroup order: y-slabs of this many workgroup rows,
+                                    // each walked through z (>= the rows: plane order)
 };
 
 // One colour pass.  mode 0: normal; 1: first pass from an all-zero field;

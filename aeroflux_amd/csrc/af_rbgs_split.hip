@@ -30,7 +30,7 @@ __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color
 {
     if (a.done && *a.done) return;
     int bx, by, bz;
-    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    xcd_block3_slab(nb.x, nb.y, nb.z, a.slab, bx, by, bz);
     body::split_pass_column<T, MODE, ZR>(a, color, bx * BX + threadIdx.x, by * BY + threadIdx.y,
                                          bz * ZR);
 }

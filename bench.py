@@ -467,6 +467,8 @@ def main() -> int:
         "divergence_ms": e.time_kernel(2, 10),
         "gradient_ms": e.time_kernel(3, 10),
     }
+    tuned = {"sweep_shape": e.sweep_shape()[0] + 1, "sweep_slab_rows": e.sweep_slab(),
+             "pass_slab_workgroup_rows": e.pass_slab()}
     e.Close()
 
     out = {
@@ -504,6 +506,7 @@ def main() -> int:
             "avg_launch_ms": t_rb,
         },
         "kernels": kernels,
+        "tuned": tuned,
     }
 
     if not args.no_mixed:

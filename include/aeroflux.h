@@ -99,6 +99,10 @@ typedef struct af_config {
                                  shape k-1 (see af_sweep_shape) */
     int32_t SweepSlab;        /* advection block order: 0 -> tuned, -1 ->
                                  plane by plane, k -> y-slabs of k rows */
+    int32_t PassSlab;         /* colour-split pass order (FP64 split and
+                                 mixed solves): 0 -> tuned, -1 -> plane by
+                                 plane, k -> y-slabs of k workgroup rows
+                                 (4 grid rows each) */
 } af_config;
 
 /* Field, grid.go:12-21 (descriptor after any commitVelocity swaps). */
@@ -194,8 +198,9 @@ int af_project(af_engine *e);                           /* project.go:12-34 */
 /* Average device time (ms, HIP events on the engine's stream) of `reps`
  * back-to-back launches of one kernel of the step, launched exactly as the
  * step launches it.  kernel: 0 = one launch of the red-black kernel the
- * solve uses (one colour pass on the colour-split layout by default, one
- * pass on the arena layout for TileK = -1, TileK passes for TileK > 0),
+ * solve uses (one colour pass on the colour-split layout by default -- an
+ * FP32 pass for mixed precision -- one pass on the arena layout for
+ * TileK = -1, TileK passes for TileK > 0),
  * 1 = advection sweep (X axis), 2 = divergence, 3 = gradient subtract,
  * 4 = advection sweep (Y axis), 5 = advection sweep (Z axis).
  * No reference
@@ -221,6 +226,10 @@ int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *z
 /* Rows per y-slab of the advection block order in use (-1: plane by plane).
  * No reference counterpart. */
 int af_sweep_slab(const af_engine *e, int32_t *rows);
+
+/* Workgroup rows per y-slab of the colour-split passes' order in use (-1:
+ * plane by plane).  No reference counterpart. */
+int af_pass_slab(const af_engine *e, int32_t *rows);
 
 /* ------------------------------------------------------------- batching */
 /* Many engines of uneven sizes stepped together.  No reference counterpart:

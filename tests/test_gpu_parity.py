@@ -452,6 +452,34 @@ def test_step_bitwise_across_sweep_shapes(af, oracle_lib, shape):
     ge.Close()
 
 
+@pytest.mark.parametrize("slab", [-1, 1, 2, 3, 7])
+def test_step_bitwise_across_pass_orders(af, oracle_lib, slab):
+    """The colour passes' workgroup order (plane by plane, or y-slabs of 1-7
+    workgroup rows walked through z, uneven last slab) does not change a bit:
+    FP64 against the oracle, mixed against the plane order."""
+    cfg = dict(NX=40, NY=30, NZ=18, H=1 / 40, DT=0.01, Viscosity=1e-4, GaussSeidelIters=6)
+    f = ic.taylor_green(40, 30, 18, 1, 1 / 40, seed=23, amp=3.0, noise=0.05)
+    ge, oe = make_pair(af, oracle_lib, f, **cfg)
+    ge.Close()
+    ge = af.New(af.Config(**cfg, PassSlab=slab))
+    assert ge.pass_slab() == slab
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        ge.Mesh.SetData(ge.Mesh.field(fid), f[name])
+    for s in range(2):
+        ge.Step(); assert oe.step() == 0
+    assert_bitwise(ge, oe, range(8), f"pass slab {slab}")
+    ge.Close()
+    mixed = []
+    for sl in (-1, slab):
+        me = af.New(af.Config(**cfg, PassSlab=sl, Precision=afm.PRECISION_MIXED))
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            me.Mesh.SetData(me.Mesh.field(fid), f[name])
+        me.StepN(2)
+        mixed.append([me.Mesh.Data(me.Mesh.field(k)).tobytes() for k in range(8)])
+        me.Close()
+    assert mixed[0] == mixed[1], f"mixed pass slab {slab}"
+
+
 LDS_SHAPES = range(9, 13)  # the LDS-staged sweep variants (SweepShape 9..12)
 
 
