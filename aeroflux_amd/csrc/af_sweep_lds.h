@@ -89,65 +89,77 @@ struct LdsLoad {
 };
 
 // Staging of one plane of the three source fields, split over the
-// workgroup: thread t owns elements t, t+NT, ... of the flattened
-// (field, row, column) box.  Their plane-independent offsets are computed
-// once; per plane a thread issues all its global loads into registers
-// (`fetch`) and later writes them to the plane's ring slot (`store`), so the
-// next plane's loads are in flight while the current plane is computed.
+// workgroup: thread t owns elements t, t+NT, ... of one plane's (row,
+// column) box, in all three fields.  Field pointers, plane strides and
+// extents are workgroup-uniform (scalar registers); per element a thread
+// keeps only its offset in each field and its LDS offset.  Per plane a
+// thread issues all its global loads into registers (`fetch`) and later
+// writes them to the plane's ring slot (`store`), so the next plane's loads
+// are in flight while the current plane is computed.
 template <class B, int NT>
 struct Stager {
-    static constexpr int N = (3 * B::PLANE + NT - 1) / NT;
-    static_assert(NT <= 3 * B::PLANE, "a lane past the box repeats an element of it");
-    const __attribute__((address_space(1))) double *src[N];  // element's (x, y), plane 0
-    int sz[N];    // its field's plane stride
-    int zmax[N];  // its field's last plane
-    int off[N];   // LDS offset within a ring slot
-    double v[N];
+    static constexpr int NF = (B::PLANE + NT - 1) / NT;  // elements per field per thread
+    static_assert(NT <= B::PLANE, "a lane past the box repeats an element of it");
+    const __attribute__((address_space(1))) double *base[3];
+    int sz[3], zmax[3];  // plane stride and last plane of each field
+    int eoff[NF][3];     // element's offset in each field, plane 0
+    int loff[NF];        // element's LDS offset within a field's ring slot
+    double v[NF][3];
 
     // No lane branches: lanes past the box repeat an element of it (same
-    // load, same value to the same slot), and elements outside their field
+    // load, same value to the same slot), and elements outside a field
     // (columns, rows or planes beyond the padded field) load a clamped
     // in-field value into a slot no face reads -- LdsLoad reads only cells
     // of the field.
     //
-    // src(c) gives source field c's descriptor.  It must read memory that
-    // can be indexed at run time (kernel arguments, global memory): selecting
-    // among the members of a local copy would force that copy into scratch.
+    // src(c) gives source field c's descriptor (c is a compile-time
+    // constant here).
     template <class Src>
     __device__ __forceinline__ void init(const Src &src_of, int ox, int oy)
     {
         const int t = threadIdx.y * blockDim.x + threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int e0 = t + k * NT;
-            const int e = e0 < 3 * B::PLANE ? e0 : e0 - 3 * B::PLANE;
-            const int c = e / B::PLANE;
-            const int r = (e - c * B::PLANE) / B::XW;
-            const int col = e - c * B::PLANE - r * B::XW;
+        for (int c = 0; c < 3; c++) {
             const DField &f = src_of(c);
+            base[c] = body::gp((const double *)f.d);
+            sz[c] = f.sz;
+            zmax[c] = f.nz - 1;
+        }
+#pragma unroll
+        for (int j = 0; j < NF; j++) {
+            const int e0 = t + j * NT;
+            const int e = e0 < B::PLANE ? e0 : e0 - B::PLANE;
+            const int r = e / B::XW;
+            const int col = e - r * B::XW;
             const int x = ox + col, y = oy + r;
-            const int xc = x < 0 ? 0 : (x >= f.nx ? f.nx - 1 : x);
-            const int yc = y < 0 ? 0 : (y >= f.ny ? f.ny - 1 : y);
-            src[k] = body::gp((const double *)f.d) + xc + yc * f.sy;
-            sz[k] = f.sz;
-            zmax[k] = f.nz - 1;
-            off[k] = c * B::FIELD + r * B::XW + col;
+            loff[j] = e;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const DField &f = src_of(c);
+                const int xc = x < 0 ? 0 : (x >= f.nx ? f.nx - 1 : x);
+                const int yc = y < 0 ? 0 : (y >= f.ny ? f.ny - 1 : y);
+                eoff[j][c] = xc + yc * f.sy;
+            }
         }
     }
     __device__ __forceinline__ void fetch(int z)
     {
 #pragma unroll
-        for (int k = 0; k < N; k++) {
-            const int zc = z < 0 ? 0 : (z > zmax[k] ? zmax[k] : z);
-            v[k] = src[k][(size_t)zc * sz[k]];
+        for (int c = 0; c < 3; c++) {
+            const int zc = z < 0 ? 0 : (z > zmax[c] ? zmax[c] : z);
+            const auto p = base[c] + (size_t)zc * sz[c];
+#pragma unroll
+            for (int j = 0; j < NF; j++) v[j][c] = p[eoff[j][c]];
         }
     }
     __device__ __forceinline__ void store(lds_t *lds, int z) const
     {
         if (z < 0) return;
-        const int base = (z % B::R) * B::PLANE;
+        const int slot = (z % B::R) * B::PLANE;
 #pragma unroll
-        for (int k = 0; k < N; k++) lds[base + off[k]] = v[k];
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < NF; j++) lds[c * B::FIELD + slot + loff[j]] = v[j][c];
     }
 };
 

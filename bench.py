@@ -130,7 +130,7 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
       * a one-cycle mixed batch (the same 24 sweep pairs in FP32);
       * a checked mixed batch that iterates every projection until its FP64
         residual bounds max|div| by that engine's final max|div| on the FP64
-        path (20 FP32 sweep pairs, then checked cycles of 2, at most 4).
+        path (24 FP32 sweep pairs, then checked cycles of 2, at most 4).
     Each reports cell-updates/s and each engine's final max|div|."""
     from aeroflux_amd import workload
 
@@ -232,11 +232,11 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
     out["mixed"]["schedule"] = "24 FP32 sweep pairs in one FP64 defect-correction cycle"
     out["mixed_to_target"], _ = run(
         af.PRECISION_MIXED,
-        {"MixedChecked": 1, "MixedSweeps": 20, "MixedFixSweeps": 2, "MixedMaxCycles": 4},
-        lambda i: {"MixedSweeps": 20, "MixedFixSweeps": 2, "MixedMaxCycles": 4,
+        {"MixedChecked": 1, "MixedSweeps": 24, "MixedFixSweeps": 2, "MixedMaxCycles": 4},
+        lambda i: {"MixedSweeps": 24, "MixedFixSweeps": 2, "MixedMaxCycles": 4,
                    "MixedTarget": float(d_plain[i])})
     out["mixed_to_target"]["schedule"] = (
-        "20 FP32 sweep pairs, then FP64-residual-checked cycles of 2 pairs (at most 4) until "
+        "24 FP32 sweep pairs, then FP64-residual-checked cycles of 2 pairs (at most 4) until "
         "max|div| <= the engine's final max|div| on the plain FP64 path")
     out["speedup_mixed_vs_fp64_batch"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out
@@ -413,6 +413,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-engines", type=int, default=96)
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--target-sweeps", type=int, default=24,
+                    help="FP32 sweep pairs of the first cycle of the iterated-to-target run")
     ap.add_argument("--no-big", action="store_true")
     ap.add_argument("--big-n", type=int, default=512)
     ap.add_argument("--big-steps", type=int, default=100)
@@ -540,8 +542,8 @@ def main() -> int:
         # Iterated to the FP64 path's final max|div|: a short first cycle, then
         # correction cycles until the FP64 residual says the projection's
         # max|div| is at most that value.
-        tcfg = dict(cfg, Precision=af.PRECISION_MIXED, MixedSweeps=20, MixedFixSweeps=2,
-                    MixedMaxCycles=4, MixedTarget=d64)
+        tcfg = dict(cfg, Precision=af.PRECISION_MIXED, MixedSweeps=args.target_sweeps,
+                    MixedFixSweeps=2, MixedMaxCycles=4, MixedTarget=d64)
         t, tel = run_gpu(af, tcfg, f, args.warmup, args.steps)
         dt_ = t.MaxDivergence()
         terrs = {}
@@ -559,8 +561,9 @@ def main() -> int:
             "speedup_vs_fp64": tval / value,
             "target_max_divergence": d64,
             "final_max_divergence": dt_,
-            "schedule": "20 FP32 sweep pairs, then FP64-residual-checked cycles of 2 pairs "
-                        "(at most 4 cycles) until max|div| <= the FP64 run's final value",
+            "schedule": f"{args.target_sweeps} FP32 sweep pairs, then FP64-residual-checked "
+                        "cycles of 2 pairs (at most 4 cycles) until max|div| <= the FP64 run's "
+                        "final value",
             "cycles_last_solve": tcyc,
             "max_rel_error_vs_fp64": terrs,
         }
