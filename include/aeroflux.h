@@ -147,9 +147,12 @@ int af_get_config(const af_engine *e, af_config *out);
  * as after a contained fault in the reference). */
 int af_step(af_engine *e);
 
-/* n Steps back to back with one host synchronisation at the end.  On a
- * fault, *first_fault_step (if non-null) receives the index of the first
- * faulting step.  No reference counterpart (a loop of Step calls). */
+/* n Steps back to back.  With first_fault_step null: one host
+ * synchronisation at the end, AF_ERR_FAULT if any step faulted.  Non-null:
+ * the fault flag is read after every step (one synchronisation per step, as
+ * n Step calls), the call stops at the first faulting step and
+ * *first_fault_step receives its index.  No reference counterpart (a loop
+ * of Step calls). */
 int af_step_n(af_engine *e, int64_t n, int64_t *first_fault_step);
 
 /* ---------------------------------------------------------- diagnostics */
