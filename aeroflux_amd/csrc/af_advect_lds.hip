@@ -46,13 +46,13 @@ __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int 
     const int ntile = nb.x * nb.y * nb.z;
     bool bad = false;
     if ((int)blockIdx.x >= ntile) {
-        slds::wall_face<AX>(a, ((int)blockIdx.x - ntile) * (BX * BY) + threadIdx.y * BX + threadIdx.x,
-                            bad);
+        slds::wall_face<AX, false>(
+            a, ((int)blockIdx.x - ntile) * (BX * BY) + threadIdx.y * BX + threadIdx.x, bad);
     } else {
         int bx, by, bz;
         xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
         auto src_of = [&](int c) -> const DField & { return c == 0 ? a.U : (c == 1 ? a.V : a.W); };
-        slds::tile<AX, BY, D, ZC>(a, src_of, (slds::lds_t *)lds_mem, bx, by, bz, bad);
+        slds::tile<AX, BY, D, ZC, false>(a, src_of, (slds::lds_t *)lds_mem, bx, by, bz, bad);
     }
     if (bad) *body::gp(a.flag) = 1;
 }

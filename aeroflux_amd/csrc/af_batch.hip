@@ -243,6 +243,7 @@ __global__ __launch_bounds__(BX *BYT) void kb_sweep(BCtl c, int sw)
         a.invH = e.invH;
         a.lam = e.lam_half;
         a.flag = c.flags + s;
+        a.bc = 1;  // batched sweeps always apply the velocity BC to their output
         const int row = e.G.NX + 1;
         int x, y;
         const int zb = plane_point(t, row, row * (e.G.NY + 1), BX * BYT, x, y);
@@ -286,18 +287,20 @@ __global__ __launch_bounds__(BX *LBY) void kb_sweep_lds(BCtl c, int sw)
     sa.invH = e.invH;
     sa.lam = e.lam_half;
     sa.flag = c.flags + s;
+    sa.bc = 1;
     const int3 nb = slds::tiles<LBY, ZC>(e.G);
     const int ntile = nb.x * nb.y * nb.z;
     bool bad = false;
     if (t >= ntile) {
-        slds::wall_face<AX>(sa, (t - ntile) * (BX * LBY) + threadIdx.y * BX + threadIdx.x, bad);
+        slds::wall_face<AX, true>(sa, (t - ntile) * (BX * LBY) + threadIdx.y * BX + threadIdx.x,
+                                  bad);
     } else {
         const int rest = t / nb.x;
         // Descriptors by run-time index come from the slot table in global
         // memory (a local copy indexed at run time would live in scratch).
         auto src_of = [&](int q) -> const DField & { return velf(e, sw, q); };
-        slds::tile<AX, LBY, 2, ZC>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
-                                   rest % nb.y, rest / nb.y, bad);
+        slds::tile<AX, LBY, 2, ZC, true>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
+                                         rest % nb.y, rest / nb.y, bad);
     }
     if (bad) *body::gp(sa.flag) = 1;
 }
@@ -814,10 +817,11 @@ hipError_t enq_solve(const af_batch *b)
 }
 
 // project, project.go:12-34.
+// The velocity already holds the BC the projection starts with: the sweep
+// before it applied it to its output.
 hipError_t enq_project(const af_batch *b, int sw)
 {
-    hipError_t err = enq_vbc(b, sw);
-    if (err == hipSuccess) err = enq_div(b, sw);
+    hipError_t err = enq_div(b, sw);
     if (err == hipSuccess) err = enq_solve(b);
     if (err == hipSuccess) err = enq_grad(b, sw);
     if (err == hipSuccess) err = enq_vbc(b, sw);
@@ -830,19 +834,18 @@ hipError_t enq_step(const af_batch *b)
 {
     static const int fwd[3] = {AF_AXIS_X, AF_AXIS_Y, AF_AXIS_Z};
     static const int rev[3] = {AF_AXIS_Z, AF_AXIS_Y, AF_AXIS_X};
-    hipError_t err = hipSuccess;
+    // Batched sweeps apply the velocity BC to their own output, so only the
+    // step's first BC and the two after the gradients run as launches (as in
+    // the single engine's step).
+    hipError_t err = enq_vbc(b, 0);
     int sw = 0;
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
-        err = enq_vbc(b, sw);
-        if (err == hipSuccess) err = enq_sweep(b, b->shape, b->nwg_sweep, fwd[k], sw);
+        err = enq_sweep(b, b->shape, b->nwg_sweep, fwd[k], sw);
         sw ^= 1;
     }
     if (err == hipSuccess) err = enq_project(b, sw);
-    // The projection's closing velocity BC already holds for the first
-    // reverse sweep (the BC is idempotent), as in the single engine's step.
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
-        if (k > 0) err = enq_vbc(b, sw);
-        if (err == hipSuccess) err = enq_sweep(b, b->shape, b->nwg_sweep, rev[k], sw);
+        err = enq_sweep(b, b->shape, b->nwg_sweep, rev[k], sw);
         sw ^= 1;
     }
     if (err == hipSuccess) err = enq_project(b, sw);

@@ -62,31 +62,41 @@ __host__ __device__ inline int64_t shell_count(int Lx, int Ly, int Lz)
     return 2LL * Lx * Ly + 2LL * Lx * (Lz - 2) + 2LL * (Ly - 2) * (Lz - 2);
 }
 
-__device__ __forceinline__ void shell_decode(int64_t k, int Lx, int Ly, int Lz, int &x, int &y,
+__device__ __forceinline__ void shell_decode(int64_t k64, int Lx, int Ly, int Lz, int &x, int &y,
                                              int &z)
 {
-    const int64_t nz = 2LL * Lx * Ly;
+    // Shell sizes are far below 2^31 (fields are indexed with 32-bit
+    // integers), so 32-bit unsigned division suffices; 64-bit division is a
+    // long instruction sequence that dominated this kernel.
+    unsigned k = (unsigned)k64;
+    const unsigned ux = (unsigned)Lx, uy = (unsigned)Ly;
+    const unsigned nz = 2u * ux * uy;
     if (k < nz) {
-        x = (int)(k % Lx);
-        const int64_t r = k / Lx;
-        y = (int)(r % Ly);
-        z = (r / Ly) == 0 ? 0 : Lz - 1;
+        const unsigned r = k / ux;
+        x = (int)(k - r * ux);
+        const unsigned q = r / uy;
+        y = (int)(r - q * uy);
+        z = q == 0 ? 0 : Lz - 1;
         return;
     }
     k -= nz;
-    const int64_t ny = 2LL * Lx * (Lz - 2);
+    const unsigned uz2 = (unsigned)(Lz - 2);
+    const unsigned ny = 2u * ux * uz2;
     if (k < ny) {
-        x = (int)(k % Lx);
-        const int64_t r = k / Lx;
-        z = 1 + (int)(r % (Lz - 2));
-        y = (r / (Lz - 2)) == 0 ? 0 : Ly - 1;
+        const unsigned r = k / ux;
+        x = (int)(k - r * ux);
+        const unsigned q = r / uz2;
+        z = 1 + (int)(r - q * uz2);
+        y = q == 0 ? 0 : Ly - 1;
         return;
     }
     k -= ny;
-    y = 1 + (int)(k % (Ly - 2));
-    const int64_t r = k / (Ly - 2);
-    z = 1 + (int)(r % (Lz - 2));
-    x = (r / (Lz - 2)) == 0 ? 0 : Lx - 1;
+    const unsigned uy2 = (unsigned)(Ly - 2);
+    const unsigned r = k / uy2;
+    y = 1 + (int)(k - r * uy2);
+    const unsigned q = r / uz2;
+    z = 1 + (int)(r - q * uz2);
+    x = q == 0 ? 0 : Lx - 1;
 }
 
 // Velocity-BC shell sizes of U, V, W (counts of cells written).
@@ -322,6 +332,55 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, 
     return adv + diffusion<AX, C>(a, ld, x, y, z);
 }
 
+// Stores the swept value v of component C at face (x, y, z) of T and (with
+// a.bc set in store_face) applies applyVelocityBC (stencil.go:246-315) to every
+// cell that takes its value from this face (see the velocity BC above): a
+// wall face -- normal coordinate g or g+N -- stores +0, and each ghost cell
+// whose base is this face stores the stored value negated once per ghost
+// coordinate.  A sweep writes every base cell of the BC box exactly once
+// (its faces are the box's cells with interior tangential coordinates), so
+// with a.bc its output holds what the BC applied after it would leave, and
+// that BC launch is skipped.  Without a.bc it is the plain store of
+// advectDiffuseAxis.
+template <int C>
+__device__ __forceinline__ void store_face_bc(const Geom &G, const DField &T, int x, int y, int z,
+                                              double v)
+{
+    const auto t = gp(T.d);
+    const int i = idx(T, x, y, z);
+    const int g = G.g;
+    const int c[3] = {x, y, z};
+    const int N[3] = {G.NX, G.NY, G.NZ};
+    if (c[C] == g || c[C] == g + N[C]) v = 0.0;
+    t[i] = v;
+    constexpr int A = C == 0 ? 1 : 0;   // the two tangential axes
+    constexpr int Bt = C == 2 ? 1 : 2;
+    const int sa = A == 0 ? 1 : T.sy;
+    const int sb = Bt == 1 ? T.sy : T.sz;
+    const bool alo = c[A] == g, ahi = c[A] == g + N[A] - 1;
+    const bool blo = c[Bt] == g, bhi = c[Bt] == g + N[Bt] - 1;
+    if (!(alo || ahi || blo || bhi)) return;
+    const double nv = -v;  // one ghost coordinate; two give v back (negation is exact)
+    if (alo) t[i - sa] = nv;
+    if (ahi) t[i + sa] = nv;
+    if (blo) t[i - sb] = nv;
+    if (bhi) t[i + sb] = nv;
+    if (alo && blo) t[i - sa - sb] = v;
+    if (alo && bhi) t[i - sa + sb] = v;
+    if (ahi && blo) t[i + sa - sb] = v;
+    if (ahi && bhi) t[i + sa + sb] = v;
+}
+
+template <int C>
+__device__ __forceinline__ void store_face(const SweepArgs &a, const DField &T, int x, int y,
+                                           int z, double v)
+{
+    if (a.bc)
+        store_face_bc<C>(a.G, T, x, y, z, v);
+    else
+        gp(T.d)[idx(T, x, y, z)] = v;
+}
+
 // One thread of advectDiffuseAxis (stencil.go:21-25): the U, V and W faces
 // at (x, y, z0 .. z0+ZR-1) that exist (x <= g+NX, y <= g+NY, z <= g+NZ).
 template <int AX, int ZR>
@@ -349,9 +408,9 @@ __device__ __forceinline__ void sweep_column(const SweepArgs &a, int x, int y, i
 #pragma unroll
     for (int r = 0; r < ZR; r++) {
         const int z = z0 + r;
-        if (du[r]) gp(a.T0.d)[idx(a.T0, x, y, z)] = vu[r];
-        if (dv[r]) gp(a.T1.d)[idx(a.T1, x, y, z)] = vv[r];
-        if (dw[r]) gp(a.T2.d)[idx(a.T2, x, y, z)] = vw[r];
+        if (du[r]) store_face<0>(a, a.T0, x, y, z, vu[r]);
+        if (dv[r]) store_face<1>(a, a.T1, x, y, z, vv[r]);
+        if (dw[r]) store_face<2>(a, a.T2, x, y, z, vw[r]);
     }
     if (bad) *gp(a.flag) = 1;
 }
@@ -395,26 +454,38 @@ __device__ __forceinline__ void pressure_bc_cell(const DField &P, const Geom &G,
     const int g = G.g;
     int x, y, z;
     const int Lx = P.nx, Ly = P.ny, Lz = P.nz, t = g;
-    int64_t r = k;
-    const int64_t fz = 2LL * t * Lx * Ly;
+    // 32-bit unsigned division: the shell is far smaller than 2^31 cells.
+    const unsigned ux = (unsigned)Lx, uy = (unsigned)Ly;
+    const unsigned uyi = (unsigned)(Ly - 2 * t), uzi = (unsigned)(Lz - 2 * t);
+    unsigned r = (unsigned)k;
+    const unsigned fz = 2u * t * ux * uy;
     if (r < fz) {
-        x = (int)(r % Lx); r /= Lx;
-        y = (int)(r % Ly); r /= Ly;
-        const int s = (int)r;
+        unsigned q = r / ux;
+        x = (int)(r - q * ux);
+        r = q;
+        q = r / uy;
+        y = (int)(r - q * uy);
+        const int s = (int)q;
         z = s < t ? s : Lz - 2 * t + s;
     } else {
         r -= fz;
-        const int64_t fy = 2LL * t * Lx * (Lz - 2 * t);
+        const unsigned fy = 2u * t * ux * uzi;
         if (r < fy) {
-            x = (int)(r % Lx); r /= Lx;
-            z = t + (int)(r % (Lz - 2 * t)); r /= (Lz - 2 * t);
-            const int s = (int)r;
+            unsigned q = r / ux;
+            x = (int)(r - q * ux);
+            r = q;
+            q = r / uzi;
+            z = t + (int)(r - q * uzi);
+            const int s = (int)q;
             y = s < t ? s : Ly - 2 * t + s;
         } else {
             r -= fy;
-            y = t + (int)(r % (Ly - 2 * t)); r /= (Ly - 2 * t);
-            z = t + (int)(r % (Lz - 2 * t)); r /= (Lz - 2 * t);
-            const int s = (int)r;
+            unsigned q = r / uyi;
+            y = t + (int)(r - q * uyi);
+            r = q;
+            q = r / uzi;
+            z = t + (int)(r - q * uzi);
+            const int s = (int)q;
             x = s < t ? s : Lx - 2 * t + s;
         }
     }

@@ -374,7 +374,10 @@ hipError_t enq_step(af_engine *e, Desc &d)
     // The projection ends with applyVelocityBC and nothing touches the
     // velocity before the reverse sweeps' first applyVelocityBC.  The BC
     // writes only cells it never reads, so applying it twice leaves the
-    // values of applying it once: the second application is skipped.
+    // values of applying it once: the second application is skipped.  (The
+    // sweeps could apply the BC to their own output, as the batched sweeps
+    // do, but the extra registers cost the single engine's LDS-staged sweep
+    // more occupancy than the BC launches cost.)
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
         if (k > 0) err = enq_velocity_bc(e, d);
         if (err == hipSuccess) err = enq_sweep(e, d, rev[k], half);

@@ -28,6 +28,7 @@ struct SweepArgs {
     int *flag;             // set to 1 on a NaN departure coordinate
     int shape = 4;         // workgroup shape, 0 .. sweep_shape_count()-1
     int slab = 1 << 30;    // rows per y-slab of the block order (>= NY+1: plane order)
+    int bc = 0;            // also apply applyVelocityBC to the output (body::store_face)
 };
 
 // Number of sweep shapes: gathering shapes (rows per block x z per thread)

@@ -179,7 +179,7 @@ __host__ __device__ inline int wall_blocks(const Geom &G)
 
 // Wall column x = g+NX: U faces only (V and W faces need x < g+NX), from
 // global memory; face k of NY*NZ.
-template <int AX>
+template <int AX, bool BC>
 __device__ __forceinline__ void wall_face(const SweepArgs &a, int k, bool &bad)
 {
     const int g = a.G.g;
@@ -187,7 +187,10 @@ __device__ __forceinline__ void wall_face(const SweepArgs &a, int k, bool &bad)
     const int xe = g + a.G.NX, y = g + k % a.G.NY, z = g + k / a.G.NY;
     const body::GlobalLoad ld{a};
     const double v = body::advect_value<AX, 0>(a, ld, xe, y, z, bad);
-    body::gp(a.T0.d)[body::idx(a.T0, xe, y, z)] = v;
+    if (BC)
+        body::store_face_bc<0>(a.G, a.T0, xe, y, z, v);
+    else
+        body::gp(a.T0.d)[body::idx(a.T0, xe, y, z)] = v;
 }
 
 // A face of the current plane formed from LDS (body::advect_value's parts,
@@ -224,7 +227,7 @@ __device__ __forceinline__ double far_taps(const DField &s, const body::Departur
 // the faces of x in [g+64bx, g+64bx+64) (x < g+NX), y = g+BY*by+ty
 // (y <= g+NY), z over the chunk.  Ends with a barrier, so the caller may
 // reuse `lds` at once.
-template <int AX, int BY, int DD, int ZC, class Src>
+template <int AX, int BY, int DD, int ZC, bool BC, class Src>
 __device__ __forceinline__ void tile(const SweepArgs &a, const Src &src_of, lds_t *lds, int bx,
                                      int by, int bz, bool &bad)
 {
@@ -251,9 +254,15 @@ __device__ __forceinline__ void tile(const SweepArgs &a, const Src &src_of, lds_
     bool du = false, dv = false, dw = false;
     int zp = zs;
     auto put = [&]() {
-        if (du) body::gp(a.T0.d)[body::idx(a.T0, x, y, zp)] = vu;
-        if (dv) body::gp(a.T1.d)[body::idx(a.T1, x, y, zp)] = vv;
-        if (dw) body::gp(a.T2.d)[body::idx(a.T2, x, y, zp)] = vw;
+        if (BC) {
+            if (du) body::store_face_bc<0>(a.G, a.T0, x, y, zp, vu);
+            if (dv) body::store_face_bc<1>(a.G, a.T1, x, y, zp, vv);
+            if (dw) body::store_face_bc<2>(a.G, a.T2, x, y, zp, vw);
+        } else {
+            if (du) body::gp(a.T0.d)[body::idx(a.T0, x, y, zp)] = vu;
+            if (dv) body::gp(a.T1.d)[body::idx(a.T1, x, y, zp)] = vv;
+            if (dw) body::gp(a.T2.d)[body::idx(a.T2, x, y, zp)] = vw;
+        }
     };
     for (int z = zs; z < zend; z++) {
         st.store(lds, z - B::ZLO + B::R - 1);
