@@ -564,7 +564,6 @@ struct af_batch {
     int nwg_sweep = 2048, nwg_pass = 2048, nwg_other = 2048;
     int lds_grid = 8;      // workgroups of the LDS-staged sweep launches in the graph
     int lds_need = 8;      // workgroups the current table's K_LDS list needs
-    int staged_slots = 0;  // resident slots on the LDS-staged sweep
     bool tuned = false;
     int nact = 0;
     int64_t cells = 0;
@@ -643,9 +642,6 @@ int build_table(af_batch *b, int plan, int ring = 0)
     const int lds_total = t[2 + (int64_t)stride * (1 + K_LDS) + n];
     b->lds_need = 8 * ((lds_total + 7) / 8);
     if (b->lds_need < 8) b->lds_need = 8;
-    b->staged_slots = 0;
-    if (plan_lds(plan) >= 0)
-        for (int a = 0; a < n; a++) b->staged_slots += lds_slot(slot_geom(b->slots[act[a]]));
     return AF_OK;
 }
 
