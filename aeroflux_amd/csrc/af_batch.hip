@@ -382,7 +382,8 @@ __global__ __launch_bounds__(256) void kb_pbc(BCtl c)
     });
 }
 
-// subtractGradient, project.go:172-205.
+// subtractGradient, project.go:172-205, with the velocity BC that follows it
+// in project applied to the updated faces.
 __global__ __launch_bounds__(BX *BY) void kb_grad(BCtl c, int sw)
 {
     for_items<K_GRAD>(c, [&](int s, int t) {
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(BX *BY) void kb_grad(BCtl c, int sw)
         const Geom &G = e.G;
         int xi, yi;
         const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
-        body::gradient_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2), e.f[AF_FIELD_P], G,
+        body::gradient_cell<true>(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2), e.f[AF_FIELD_P], G,
                             e.grad_scale, G.g + xi, G.g + yi, G.g + zi);
     });
 }
@@ -813,14 +814,14 @@ hipError_t enq_solve(const af_batch *b)
 }
 
 // project, project.go:12-34.
-// The velocity already holds the BC the projection starts with: the sweep
-// before it applied it to its output.
+// project, project.go:12-34.  The velocity already holds the BC the
+// projection starts with (the sweep before it applied it), and the gradient
+// subtraction applies the closing one.
 hipError_t enq_project(const af_batch *b, int sw)
 {
     hipError_t err = enq_div(b, sw);
     if (err == hipSuccess) err = enq_solve(b);
-    if (err == hipSuccess) err = enq_grad(b, sw);
-    if (err == hipSuccess) err = enq_vbc(b, sw);
+    if (err == hipSuccess) err = enq_grad(b, sw);  // applies the closing velocity BC
     return err;
 }
 
@@ -830,9 +831,8 @@ hipError_t enq_step(const af_batch *b)
 {
     static const int fwd[3] = {AF_AXIS_X, AF_AXIS_Y, AF_AXIS_Z};
     static const int rev[3] = {AF_AXIS_Z, AF_AXIS_Y, AF_AXIS_X};
-    // Batched sweeps apply the velocity BC to their own output, so only the
-    // step's first BC and the two after the gradients run as launches (as in
-    // the single engine's step).
+    // Batched sweeps and gradient subtractions apply the velocity BC to
+    // their own output, so only the step's first BC runs as a launch.
     hipError_t err = enq_vbc(b, 0);
     int sw = 0;
     for (int k = 0; k < 3 && err == hipSuccess; k++) {

@@ -505,7 +505,21 @@ __device__ __forceinline__ void pressure_bc_cell(const DField &P, const Geom &G,
 
 // subtractGradient, project.go:172-205.  The thread of interior column
 // (x, y) at z updates U, V (z < g+NZ) and W; the last column, row and z
-// position also update the wall faces U(g+NX), V(g+NY), W(g+NZ).
+// position also update the wall faces U(g+NX), V(g+NY), W(g+NZ).  With BC
+// the updated faces go through store_face_bc: the velocity BC that follows
+// the gradient in project (project.go:33) is then already applied (the
+// faces a thread updates are exactly the BC's base cells).
+template <bool BC, int C>
+__device__ __forceinline__ void put_grad(const Geom &G, const DField &T, int x, int y, int z,
+                                         double v)
+{
+    if (BC)
+        store_face_bc<C>(G, T, x, y, z, v);
+    else
+        gp(T.d)[idx(T, x, y, z)] = v;
+}
+
+template <bool BC>
 __device__ __forceinline__ void gradient_cell(const DField &U, const DField &V, const DField &W,
                                               const DField &P, const Geom &G, double scale, int x,
                                               int y, int z)
@@ -517,14 +531,16 @@ __device__ __forceinline__ void gradient_cell(const DField &U, const DField &V, 
     const int ip = idx(P, x, y, z);
     if (z < g + G.NZ) {
         const int iu = idx(U, x, y, z);
-        u[iu] -= scale * (p[ip] - p[ip - 1]);
-        if (x == g + G.NX - 1) u[iu + 1] -= scale * (p[ip + 1] - p[ip]);
+        put_grad<BC, 0>(G, U, x, y, z, u[iu] - scale * (p[ip] - p[ip - 1]));
+        if (x == g + G.NX - 1)
+            put_grad<BC, 0>(G, U, x + 1, y, z, u[iu + 1] - scale * (p[ip + 1] - p[ip]));
         const int iv = idx(V, x, y, z);
-        v[iv] -= scale * (p[ip] - p[ip - P.sy]);
-        if (y == g + G.NY - 1) v[iv + V.sy] -= scale * (p[ip + P.sy] - p[ip]);
+        put_grad<BC, 1>(G, V, x, y, z, v[iv] - scale * (p[ip] - p[ip - P.sy]));
+        if (y == g + G.NY - 1)
+            put_grad<BC, 1>(G, V, x, y + 1, z, v[iv + V.sy] - scale * (p[ip + P.sy] - p[ip]));
     }
     const int iw = idx(W, x, y, z);
-    w[iw] -= scale * (p[ip] - p[ip - P.sz]);
+    put_grad<BC, 2>(G, W, x, y, z, w[iw] - scale * (p[ip] - p[ip - P.sz]));
 }
 
 // ---------------------------------------------------- colour-split layout

@@ -224,11 +224,13 @@ hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false)
                              e->div_scale, e->stream, split, red, black, e->h2);
 }
 
-hipError_t enq_gradient(af_engine *e, const Desc &d)
+// subtractGradient; with `velocity_bc` also the velocity BC that follows it
+// in project.
+hipError_t enq_gradient(af_engine *e, const Desc &d, bool velocity_bc = false)
 {
     return launch_gradient(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
                            dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_P]), e->G,
-                           e->grad_scale, e->stream);
+                           e->grad_scale, e->stream, velocity_bc);
 }
 
 // solvePressureRedBlack (project.go:73-100), FP64, production form: the
@@ -345,15 +347,16 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
     return enq_solve_fp64(e, d, rhs_ready);
 }
 
-// project, project.go:12-34
+// project, project.go:12-34.  The gradient subtraction applies the closing
+// velocity BC itself (body::gradient_cell), which is the same values as the
+// separate launch.
 hipError_t enq_project(af_engine *e, const Desc &d)
 {
     const bool fuse = fused_rhs(e);
     hipError_t err = enq_velocity_bc(e, d);
     if (err == hipSuccess) err = enq_divergence(e, d, fuse);
     if (err == hipSuccess) err = enq_solve(e, d, fuse);
-    if (err == hipSuccess) err = enq_gradient(e, d);
-    if (err == hipSuccess) err = enq_velocity_bc(e, d);
+    if (err == hipSuccess) err = enq_gradient(e, d, true);
     return err;
 }
 

@@ -56,8 +56,10 @@ hipError_t launch_rbgs_first(DField P, DField D, Geom G, double h2, hipStream_t 
 // when zero_outer is set.
 hipError_t launch_pressure_bc(DField P, Geom G, bool zero_outer, hipStream_t s);
 hipError_t launch_zero(double *d, int64_t n, hipStream_t s);
+// velocity_bc: also apply applyVelocityBC to the updated velocity (the BC
+// that follows subtractGradient in project), so that launch can be skipped.
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
-                           hipStream_t s);
+                           hipStream_t s, bool velocity_bc = false);
 hipError_t launch_set_int(int *p, int v, hipStream_t s);
 
 // Reductions: two launches, fixed tree, no atomics.  `partials` must hold

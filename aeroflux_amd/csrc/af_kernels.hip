@@ -135,15 +135,17 @@ __global__ __launch_bounds__(256) void k_zero(double *d, int64_t n)
     if (i < n) d[i] = 0;
 }
 
-// subtractGradient, project.go:172-205, one thread per interior column and z.
+// subtractGradient, project.go:172-205, one thread per interior column and z
+// (BC: with the velocity BC that follows it applied, body::gradient_cell).
+template <bool BC>
 __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField W, DField P,
                                                      Geom G, double scale, int3 nb)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    body::gradient_cell(U, V, W, P, G, scale, g + bx * BX + threadIdx.x,
-                        g + by * BY + threadIdx.y, g + bz);
+    body::gradient_cell<BC>(U, V, W, P, G, scale, g + bx * BX + threadIdx.x,
+                            g + by * BY + threadIdx.y, g + bz);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -316,10 +318,15 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s)
 }
 
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
-                           hipStream_t s)
+                           hipStream_t s, bool velocity_bc)
 {
     const int3 nb = nblocks(G.NX, G.NY, G.NZ + 1);
-    hipLaunchKernelGGL(k_gradient, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale, nb);
+    if (velocity_bc)
+        hipLaunchKernelGGL(k_gradient<true>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
+                           nb);
+    else
+        hipLaunchKernelGGL(k_gradient<false>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
+                           nb);
     return hipGetLastError();
 }
 

@@ -191,7 +191,7 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
         t_res = b.time_stage(9, 10) if prec == af.PRECISION_MIXED else 0.0
         t_div, t_grad = b.time_stage(2, 10), b.time_stage(3, 10)
         npass = 2 * (bc_extra.get("MixedSweeps", 24) if prec == af.PRECISION_MIXED else 24)
-        model = 3 * t_vbc + 2 * sum(sw) + 2 * (t_div + npass * t_pass + t_merge + t_pbc + t_grad)
+        model = t_vbc + 2 * sum(sw) + 2 * (t_div + npass * t_pass + t_merge + t_pbc + t_grad)
         info = b.info()
         pass_bytes = (6 if prec == af.PRECISION_MIXED else 12) * cells
         res = {
@@ -209,9 +209,9 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
                        "divergence_ms": t_div, "gradient_ms": t_grad, "merge_ms": t_merge,
                        "pressure_bc_ms": t_pbc, "residual_ms": t_res,
                        "one_cycle_step_model_ms": model,
-                       "model": "3 velocity BCs + 2 x (X+Y+Z sweeps, which apply the BC to "
+                       "model": "1 velocity BC + 2 x (X+Y+Z sweeps, which apply the BC to "
                                 "their output) + 2 x (divergence + "
-                                f"{npass} passes + merge + pressure BC + gradient)"},
+                                f"{npass} passes + merge + pressure BC + gradient, which applies the BC)"},
             "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
                       "staged_slots": info["StagedSlots"],
                       "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
