@@ -43,14 +43,16 @@ constexpr int BX = body::BX;
 constexpr int BY = 4;       // rows per workgroup of the 64x4 stages
 constexpr int PZ = 4;       // z-planes per thread of a colour pass
 constexpr int CHUNK = 256;  // cells per item of the shell stages
+constexpr int CZ = 4;       // z-planes per item of K_CELL (divergence, merge)
 constexpr int XCDS = 8;
 
 // Stage kinds: each has its own tile decomposition and prefix table.
 enum Kind {
     K_VBC,    // velocity-BC shell cells, CHUNK per item
     K_SWEEP,  // advection tiles (64 x rows x z-per-thread) of the union box
-    K_CELL,   // 64 x 4 tiles of one interior z-plane
-    K_GRAD,   // as K_CELL over NZ + 1 planes
+    K_CELL,   // 256-point plane tiles of CZ interior z-planes
+    K_GRAD,   // 256-point tiles of one plane, over NZ + 1 planes (measured: more
+              // planes per item made the gradient slower)
     K_PASS,   // colour-pass tiles of the split layout (64 x 4 x PZ)
     K_PBC,    // pressure ghost-shell cells, CHUNK per item
     K_COL,    // 64 x 4 interior columns (z loop inside)
@@ -141,7 +143,7 @@ inline int64_t kind_items(int kind, const Geom &G, const DField &P, int plan)
         const int r = kShapeRows[shape], z = kShapeZ[shape];
         return plane_tiles((int64_t)(G.NX + 1) * (G.NY + 1), BX * r) * ((G.NZ + z) / z);
     }
-    case K_CELL: return plane_tiles((int64_t)G.NX * G.NY, BX * BY) * G.NZ;
+    case K_CELL: return plane_tiles((int64_t)G.NX * G.NY, BX * BY) * ((G.NZ + CZ - 1) / CZ);
     case K_GRAD: return plane_tiles((int64_t)G.NX * G.NY, BX * BY) * (G.NZ + 1);
     case K_PASS: return plane_tiles(nxh * G.NY, BX * BY) * ((G.NZ + PZ - 1) / PZ);
     case K_PBC:
@@ -315,18 +317,24 @@ __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
         int xi, yi;
-        const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
-        double v;
-        if (!body::divergence_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2),
-                                   e.f[AF_FIELD_DIV], G, e.div_scale, xi + G.g, yi + G.g,
-                                   zi + G.g, v))
-            return;
-        const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
-        const bool black = body::parity(xi, yi, zi, G.g);
-        if (MIXED)
-            body::gp(static_cast<float *>(black ? e.d_black : e.d_red))[i] = (float)(-(e.h2 * v));
-        else
-            body::gp(static_cast<double *>(black ? e.d_black : e.d_red))[i] = v;
+        const int zb = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
+        for (int r = 0; r < CZ; r++) {
+            const int zi = zb * CZ + r;
+            if (zi >= G.NZ) break;
+            double v;
+            // (false: x or y outside the interior, the same on every plane)
+            if (!body::divergence_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2),
+                                       e.f[AF_FIELD_DIV], G, e.div_scale, xi + G.g, yi + G.g,
+                                       zi + G.g, v))
+                break;
+            const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
+            const bool black = body::parity(xi, yi, zi, G.g);
+            if (MIXED)
+                body::gp(static_cast<float *>(black ? e.d_black : e.d_red))[i] =
+                    (float)(-(e.h2 * v));
+            else
+                body::gp(static_cast<double *>(black ? e.d_black : e.d_red))[i] = v;
+        }
     });
 }
 
@@ -362,10 +370,14 @@ __global__ __launch_bounds__(BX *BY) void kb_from_split(BCtl c, int gated)
         const Geom &G = e.G;
         const DField &P = e.f[AF_FIELD_P];
         int xi, yi;
-        const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
-        body::from_split_cell<TS, ADD>(P.d, P.sy, P.sz, G.g, G.NX, G.NY, (G.NX + 1) / 2,
-                                       static_cast<const TS *>(e.s_red),
-                                       static_cast<const TS *>(e.s_black), xi, yi, zi);
+        const int zb = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
+        for (int r = 0; r < CZ; r++) {
+            const int zi = zb * CZ + r;
+            if (zi >= G.NZ) break;
+            body::from_split_cell<TS, ADD>(P.d, P.sy, P.sz, G.g, G.NX, G.NY, (G.NX + 1) / 2,
+                                           static_cast<const TS *>(e.s_red),
+                                           static_cast<const TS *>(e.s_black), xi, yi, zi);
+        }
     });
 }
 
@@ -391,8 +403,8 @@ __global__ __launch_bounds__(BX *BY) void kb_grad(BCtl c, int sw)
         const Geom &G = e.G;
         int xi, yi;
         const int zi = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
-        body::gradient_cell<true>(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2), e.f[AF_FIELD_P], G,
-                            e.grad_scale, G.g + xi, G.g + yi, G.g + zi);
+        body::gradient_cell<true>(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2),
+                                  e.f[AF_FIELD_P], G, e.grad_scale, G.g + xi, G.g + yi, G.g + zi);
     });
 }
 
