@@ -41,7 +41,7 @@ namespace {
 
 constexpr int BX = body::BX;
 constexpr int BY = 4;       // rows per workgroup of the 64x4 stages
-constexpr int PZ = 4;       // z-planes per thread of a colour pass
+constexpr int PZ = 8;       // z-planes per thread of a colour pass
 constexpr int CHUNK = 256;  // cells per item of the shell stages
 constexpr int CZ = 4;       // z-planes per item of K_CELL (divergence, merge)
 constexpr int XCDS = 8;
