@@ -138,6 +138,8 @@ _SIGNATURES = [
     ("af_max_velocity", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("af_cfl", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("af_enable_sparse_solver", C.c_int, [C.c_void_p]),
+    ("af_enable_direct_solver", C.c_int, [C.c_void_p]),
+    ("af_direct_solver_enabled", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     ("af_field_desc", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_CField)]),
     ("af_field_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
     ("af_field_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
@@ -398,6 +400,18 @@ class Engine:
 
     def EnableSparseSolver(self) -> None:  # sparse_fallback.go:17-19
         _check(load_library().af_enable_sparse_solver(self._handle()))
+
+    def EnableDirectSolver(self) -> None:
+        """Direct pressure solve on the GPU (the pinned Neumann system of the
+        reference's Darwin EnableSparseSolver, sparse_darwin.go:19-172),
+        used by every later projection.  No counterpart in the Linux
+        reference build."""
+        _check(load_library().af_enable_direct_solver(self._handle()))
+
+    def direct_solver_enabled(self) -> bool:
+        on = C.c_int32()
+        _check(load_library().af_direct_solver_enabled(self._handle(), C.byref(on)))
+        return bool(on.value)
 
     def Close(self) -> None:
         """Close, aeroflux.go:419-439; idempotent."""

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libaeroflux.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["af_kernels.hip", "af_advect.hip", "af_advect_lds.hip", "af_rbgs_tb.hip", "af_rbgs_split.hip", "af_mixed.hip",
-           "af_engine.hip", "af_batch.hip", "af_ic.hip"]
+           "af_engine.hip", "af_batch.hip", "af_ic.hip", "af_direct.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -57,7 +57,7 @@ def build(verbose: bool = False) -> str:
         _run(cmd)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lrocblas"]
     if verbose:
         print(" ".join(cmd))
     _run(cmd)

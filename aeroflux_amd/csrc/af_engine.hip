@@ -22,6 +22,7 @@
 #include "af_ic.h"
 #include "af_internal.h"
 #include "af_mixed.h"
+#include "af_direct.h"
 #include "af_rbgs_split.h"
 #include "af_rbgs_tb.h"
 
@@ -138,6 +139,7 @@ struct af_engine {
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
     ICScratch ic;               // initial-condition generator scratch (lazy)
+    DirectState direct;         // direct pressure solve, once enabled
 };
 
 namespace {
@@ -202,6 +204,7 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
 // right-hand side (the colour-split FP64 solve and the mixed solve read it).
 bool fused_rhs(const af_engine *e)
 {
+    if (e->direct.ready()) return false;
     return e->cfg.Precision == AF_PRECISION_MIXED || e->tile64.k == 0;
 }
 
@@ -329,6 +332,11 @@ hipError_t enq_solve_fp64(af_engine *e, const Desc &d, bool rhs_ready)
 // right-hand side (enq_divergence with fuse).
 hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 {
+    if (e->direct.ready()) {
+        int bs = 0;
+        return enqueue_direct_solve(e->direct, dfield(e, d.f[AF_FIELD_P]),
+                                    dfield(e, d.f[AF_FIELD_DIV]), e->G, e->stream, &bs);
+    }
     if (e->cfg.Precision == AF_PRECISION_MIXED) {
         MixedCtx c;
         c.P = dfield(e, d.f[AF_FIELD_P]);
@@ -442,6 +450,7 @@ void release(af_engine *e)
             g = nullptr;
         }
     release_mixed(e->mixed);
+    release_direct(e->direct);
     e->ic.release();
     if (e->arena) (void)hipFree(e->arena);
     if (e->pbuf) (void)hipFree(e->pbuf);
@@ -1017,6 +1026,35 @@ int af_enable_sparse_solver(af_engine *e)
     return fail(AF_ERR_UNSUPPORTED,
                 "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
                 "Red-Black iterative solver is used instead");
+}
+
+int af_enable_direct_solver(af_engine *e)
+{
+    if (int rc = check_open(e)) return rc;
+    if (e->direct.ready()) return AF_OK;
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    HIP_TRY(hipStreamSynchronize(e->stream), "enable direct solver");
+    int bs = 0;
+    const hipError_t err = init_direct(e->direct, e->G, e->cfg.H, e->stream, &bs);
+    if (err != hipSuccess) {
+        release_direct(e->direct);
+        if (bs != 0) return fail(AF_ERR_HIP, "aeroflux: direct solver: BLAS status %d", bs);
+        return hip_fail(err, "direct solver");
+    }
+    // The step graphs were captured with the red-black solve.
+    for (auto &g : e->graphs)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    return AF_OK;
+}
+
+int af_direct_solver_enabled(const af_engine *e, int32_t *on)
+{
+    if (!e || !on) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *on = e->direct.ready() ? 1 : 0;
+    return AF_OK;
 }
 
 int af_field_desc(const af_engine *e, int id, af_field *out)

@@ -166,6 +166,17 @@ int af_cfl(af_engine *e, double *out);              /* stencil.go:336-338 */
  * reference's non-Darwin behaviour. */
 int af_enable_sparse_solver(af_engine *e);
 
+/* Direct pressure solve on the GPU, for the system the reference's Darwin
+ * build factors in EnableSparseSolver (sparse_darwin.go:19-125) and solves
+ * in solvePressureDirect (:127-172): the interior 7-point Neumann Laplacian
+ * with the first cell's diagonal raised by 1/H^2, right-hand side -Div,
+ * followed by applyPressureBC.  Solved spectrally (orthonormal DCT-II along
+ * each axis as FP64 GEMMs), exact up to rounding.  Once enabled every
+ * projection uses it in place of the red-black sweeps (project.go:19-27).
+ * No counterpart in the reference's Linux build. */
+int af_enable_direct_solver(af_engine *e);
+int af_direct_solver_enabled(const af_engine *e, int32_t *on);
+
 /* --------------------------------------------------------- field access */
 
 /* Field descriptor (MAC.P/U/..., grid.go:60-67) after descriptor swaps. */

@@ -1,0 +1,139 @@
+"""Direct pressure solve (af_enable_direct_solver).
+
+The reference's Darwin build assembles the interior 7-point Neumann
+Laplacian with the first cell pinned and factors it (EnableSparseSolver,
+sparse_darwin.go:19-125); solvePressureDirect (:127-172) solves it for -Div
+and applies the pressure BC.  Here the same matrix is assembled in Python the
+way the reference assembles it and solved with scipy's sparse LU; the GPU's
+spectral solve must match it.  Properties: a projection with the direct
+solve leaves the velocity divergence-free, and the step graph (replayed)
+equals the same stages run one by one.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import aeroflux_amd as afm
+from aeroflux_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def pinned_matrix(nx, ny, nz, h):
+    """sparse_darwin.go:41-95: neighbour count on the diagonal (+1 for cell
+    0), -1 for each neighbour, all divided by h*h; CellIndex = i + j*NX +
+    k*NX*NY (grid.go:183-185)."""
+    h2 = h * h
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    col = i + j * nx + k * nx * ny
+    diag = ((i > 0).astype(float) + (i < nx - 1) + (j > 0) + (j < ny - 1) + (k > 0) + (k < nz - 1)
+            + (col == 0))
+    rows, cols, vals = [col], [col], [diag / h2]
+    for di, dj, dk, ok in ((1, 0, 0, i < nx - 1), (0, 1, 0, j < ny - 1), (0, 0, 1, k < nz - 1)):
+        a = col[ok]
+        b = (i[ok] + di) + (j[ok] + dj) * nx + (k[ok] + dk) * nx * ny
+        off = np.full(a.size, -1.0 / h2)
+        rows += [a, b]
+        cols += [b, a]
+        vals += [off, off]
+    n = nx * ny * nz
+    return sp.csc_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(n, n))
+
+
+GRIDS = [(10, 12, 8, 1), (16, 16, 16, 1), (7, 5, 9, 2), (24, 20, 6, 1)]
+
+
+@pytest.mark.parametrize("shp", GRIDS)
+def test_direct_solve_matches_pinned_system(af, shp):
+    nx, ny, nz, g = shp
+    h = 1.0 / max(nx, ny, nz)
+    e = af.New(af.Config(NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, DT=0.01, Viscosity=1e-4))
+    e.EnableDirectSolver()
+    assert e.direct_solver_enabled()
+    rng = np.random.default_rng(nx * 100 + ny)
+    div = np.zeros(e.Mesh.field(afm.FIELD_DIV).shape)
+    interior = rng.standard_normal((nz, ny, nx)) * 50.0
+    div[g:g + nz, g:g + ny, g:g + nx] = interior
+    e.Mesh.SetData(e.Mesh.field(afm.FIELD_DIV), div)
+    e.solve_pressure()
+    p = e.Mesh.Data(e.Mesh.field(afm.FIELD_P))
+    e.Close()
+    ref = spl.spsolve(pinned_matrix(nx, ny, nz, h), -interior.ravel())
+    got = p[g:g + nz, g:g + ny, g:g + nx].ravel()
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() <= 1e-10 * scale, (np.abs(got - ref).max(), scale)
+    # applyPressureBC (project.go:139-163): first ghost layer mirrors the
+    # interior cell with clamped coordinates.
+    pin = p[g:g + nz, g:g + ny, g:g + nx]
+    assert np.array_equal(p[g:g + nz, g:g + ny, g - 1], pin[:, :, 0])
+    assert np.array_equal(p[g:g + nz, g:g + ny, g + nx], pin[:, :, -1])
+    assert np.array_equal(p[g - 1, g:g + ny, g:g + nx], pin[0])
+    assert np.array_equal(p[g:g + nz, g + ny, g:g + nx], pin[:, -1, :])
+
+
+def test_direct_projection_is_divergence_free(af):
+    n = 32
+    cfg = dict(NX=n, NY=n, NZ=n, H=1.0 / n, DT=0.002, Viscosity=1e-4)
+    f = ic.taylor_green(n, n, n, 1, 1.0 / n, seed=4, amp=2.0, noise=0.3)
+    e = af.New(af.Config(**cfg))
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), f[name])
+    e.apply_velocity_bc()
+    before = e.MaxDivergence()
+    e.EnableDirectSolver()
+    e.project()
+    after = e.MaxDivergence()
+    e.Close()
+    assert before > 1.0
+    assert after <= 1e-9 * before, (before, after)
+
+
+def stage_step(e, dt):
+    """Step (aeroflux.go:348-409) through the stage calls, no graph."""
+    for ax in (0, 1, 2):
+        e.apply_velocity_bc()
+        e.sweep_axis(ax, 0.5 * dt)
+        e.commit_velocity()
+    e.project()
+    for ax in (2, 1, 0):
+        e.apply_velocity_bc()
+        e.sweep_axis(ax, 0.5 * dt)
+        e.commit_velocity()
+    e.project()
+
+
+def test_direct_step_graph_equals_stages(af):
+    """The direct solve inside the captured step graph, replayed step after
+    step, gives the bits of the same stages enqueued one by one."""
+    n = 24
+    cfg = dict(NX=n, NY=20, NZ=16, H=1.0 / n, DT=0.004, Viscosity=1e-4)
+    f = ic.taylor_green(n, 20, 16, 1, 1.0 / n, seed=6, amp=1.5, noise=0.1)
+    out = []
+    for graph in (True, False):
+        e = af.New(af.Config(**cfg))
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            e.Mesh.SetData(e.Mesh.field(fid), f[name])
+        e.EnableDirectSolver()
+        states = []
+        for _ in range(3):
+            if graph:
+                e.Step()
+            else:
+                stage_step(e, cfg["DT"])
+            states.append([e.Mesh.Data(e.Mesh.field(k)).tobytes() for k in range(8)])
+        e.Close()
+        out.append(states)
+    for s, (a, b) in enumerate(zip(*out)):
+        assert a == b, f"step {s}"
+
+
+def test_sparse_solver_keeps_reference_linux_error(af):
+    e = af.New(af.Config(NX=8, NY=8, NZ=8))
+    with pytest.raises(af.AeroFluxError) as ei:
+        e.EnableSparseSolver()
+    assert "requires darwin" in str(ei.value)
+    assert not e.direct_solver_enabled()
+    e.Close()
