@@ -1300,10 +1300,11 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
         case 5: return enq_sweep(e, d, AF_AXIS_Z, e->cfg.DT * 0.5);
         case 2: return enq_divergence(e, d);
         case 3: return enq_gradient(e, d);
+        case 6: return enq_solve(e, d);
         default: return hipErrorInvalidValue;
         }
     };
-    if (kernel < 0 || kernel > 5) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
+    if (kernel < 0 || kernel > 6) return fail(AF_ERR_ARG, "aeroflux: bad kernel id %d", kernel);
     HIP_TRY(one(), "warm-up launch");
     hipEvent_t a, b;
     HIP_TRY(hipEventCreate(&a), "create event");

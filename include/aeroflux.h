@@ -216,7 +216,9 @@ int af_project(af_engine *e);                           /* project.go:12-34 */
  * FP32 pass for mixed precision -- one pass on the arena layout for
  * TileK = -1, TileK passes for TileK > 0),
  * 1 = advection sweep (X axis), 2 = divergence, 3 = gradient subtract,
- * 4 = advection sweep (Y axis), 5 = advection sweep (Z axis).
+ * 4 = advection sweep (Y axis), 5 = advection sweep (Z axis),
+ * 6 = one whole pressure solve from Div, as the projection runs it (the
+ * direct solve once af_enable_direct_solver was called).
  * No reference
  * counterpart (measurement only); it overwrites P and the Tmp fields. */
 int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
