@@ -95,6 +95,33 @@ class _CBatchInfo(C.Structure):
     ]
 
 
+class SnapshotHeader(C.Structure):
+    """af_snapshot_header: the 128 bytes at the start of a snapshot."""
+    _fields_ = [
+        ("Magic", C.c_char * 8), ("Version", C.c_uint32), ("HeaderBytes", C.c_uint32),
+        ("NX", C.c_int64), ("NY", C.c_int64), ("NZ", C.c_int64), ("Ghost", C.c_int64),
+        ("H", C.c_double), ("DT", C.c_double), ("Viscosity", C.c_double),
+        ("Density", C.c_double), ("PayloadBytes", C.c_int64), ("Checksum", C.c_uint64),
+        ("Reserved", C.c_uint8 * 32),
+    ]
+
+
+def _raw_buffer(buf) -> tuple[int, int]:
+    """(address, bytes) of a snapshot buffer: a numpy array, bytes or
+    bytearray in host memory, or an (address, bytes) pair for memory from
+    the HIP runtime the library links (device or pinned host)."""
+    if isinstance(buf, tuple) and len(buf) == 2:
+        return int(buf[0]), int(buf[1])
+    if isinstance(buf, np.ndarray):
+        if not buf.flags.c_contiguous:
+            raise AeroFluxError(AF_ERR_ARG, "aeroflux: snapshot array must be contiguous")
+        return int(buf.ctypes.data), int(buf.nbytes)
+    if isinstance(buf, (bytes, bytearray, memoryview)):
+        arr = np.frombuffer(buf, dtype=np.uint8)
+        return int(arr.ctypes.data), int(arr.nbytes)
+    raise AeroFluxError(AF_ERR_ARG, f"aeroflux: unsupported snapshot buffer {type(buf)!r}")
+
+
 class _CServeJob(C.Structure):
     _fields_ = [
         ("NX", C.c_int64), ("NY", C.c_int64), ("NZ", C.c_int64), ("Steps", C.c_int64),
@@ -145,6 +172,9 @@ _SIGNATURES = [
     ("af_field_write", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
     ("af_field_device_ptr", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
     ("af_arena_bytes", C.c_int64, [C.c_void_p]),
+    ("af_snapshot_bytes", C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("af_snapshot", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
+    ("af_restore", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64]),
     ("af_apply_velocity_bc", C.c_int, [C.c_void_p]),
     ("af_sweep_axis", C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     ("af_commit_velocity", C.c_int, [C.c_void_p]),
@@ -179,6 +209,9 @@ _SIGNATURES = [
     ("af_batch_kinetic_energy", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     ("af_batch_mixed_stats", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int32)]),
+    ("af_batch_snapshot_bytes", C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]),
+    ("af_batch_snapshot", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
+    ("af_batch_restore", C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     ("af_batch_tune", C.c_int, [C.c_void_p]),
     ("af_batch_time_stage", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     ("af_init_band_limited", C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.c_double,
@@ -400,6 +433,30 @@ class Engine:
 
     def EnableSparseSolver(self) -> None:  # sparse_fallback.go:17-19
         _check(load_library().af_enable_sparse_solver(self._handle()))
+
+    def snapshot_bytes(self) -> int:
+        n = C.c_int64()
+        _check(load_library().af_snapshot_bytes(self._handle(), C.byref(n)))
+        return int(n.value)
+
+    def Snapshot(self, out=None):
+        """Snapshot of the engine's whole state (af_snapshot): a header and
+        the eight padded fields.  Returns a new host uint8 array, or writes
+        into `out` (a host numpy array, or an (address, bytes) pair of
+        device or pinned memory, at least snapshot_bytes() long) and
+        returns it."""
+        if out is None:
+            out = np.empty(self.snapshot_bytes(), dtype=np.uint8)
+        ptr, n = _raw_buffer(out)
+        _check(load_library().af_snapshot(self._handle(), C.c_void_p(ptr), n))
+        return out
+
+    def Restore(self, snap) -> None:
+        """Restores a snapshot taken from an engine or batch slot of the same
+        grid (af_restore)."""
+        ptr, n = _raw_buffer(snap)
+        keep = snap  # noqa: F841 -- the buffer must outlive the call
+        _check(load_library().af_restore(self._handle(), C.c_void_p(ptr), n))
 
     def EnableDirectSolver(self) -> None:
         """Direct pressure solve on the GPU (the pinned Neumann system of the
@@ -651,6 +708,24 @@ class Batch:
                           slope: float = -5.0 / 3.0, rms: float = 1.0) -> None:
         _check(load_library().af_batch_init_band_limited(self._handle(), slot, seed, kmax, slope,
                                                          rms))
+
+    def snapshot_bytes(self, slot: int) -> int:
+        n = C.c_int64()
+        _check(load_library().af_batch_snapshot_bytes(self._handle(), slot, C.byref(n)))
+        return int(n.value)
+
+    def Snapshot(self, slot: int, out=None):
+        """Snapshot of one slot, in the engine's format (af_batch_snapshot)."""
+        if out is None:
+            out = np.empty(self.snapshot_bytes(slot), dtype=np.uint8)
+        ptr, n = _raw_buffer(out)
+        _check(load_library().af_batch_snapshot(self._handle(), slot, C.c_void_p(ptr), n))
+        return out
+
+    def Restore(self, slot: int, snap) -> None:
+        ptr, n = _raw_buffer(snap)
+        keep = snap  # noqa: F841 -- the buffer must outlive the call
+        _check(load_library().af_batch_restore(self._handle(), slot, C.c_void_p(ptr), n))
 
     def Tune(self) -> None:
         _check(load_library().af_batch_tune(self._handle()))

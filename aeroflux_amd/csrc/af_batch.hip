@@ -33,6 +33,7 @@
 #include "af_sweep_lds.h"
 #include "af_ic.h"
 #include "af_internal.h"
+#include "af_snapshot.h"
 
 #pragma clang fp contract(off)
 
@@ -1387,6 +1388,38 @@ int af_batch_field_read(af_batch *b, int32_t slot, int id, double *host, int64_t
 int af_batch_field_write(af_batch *b, int32_t slot, int id, const double *host, int64_t count)
 {
     return batch_copy(b, slot, id, const_cast<double *>(host), count, true);
+}
+
+int af_batch_snapshot_bytes(const af_batch *b, int32_t slot, int64_t *bytes)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    if (!bytes) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *bytes = snapshot_bytes(b->slots[slot].L.f);
+    return AF_OK;
+}
+
+// The slot's reduction output word serves as the checksum accumulator; the
+// stream is drained first, so no step is using it.
+static int batch_snapshot_io(af_batch *b, int32_t slot, void *buf, int64_t bytes, bool save)
+{
+    if (int rc = check_slot(b, slot)) return rc;
+    HIP_TRY(hipSetDevice(b->device), "set device");
+    HIP_TRY(hipStreamSynchronize(b->stream), "batch snapshot");
+    const HSlot &h = b->slots[slot];
+    unsigned char *base = reinterpret_cast<unsigned char *>(b->pool.base + h.off);
+    auto *acc = reinterpret_cast<unsigned long long *>(base + h.L.out_off);
+    if (save) return snapshot_save(h.L.f, base, h.cfg, buf, bytes, acc, b->stream);
+    return snapshot_load(h.L.f, base, h.cfg, buf, bytes, acc, b->stream);
+}
+
+int af_batch_snapshot(af_batch *b, int32_t slot, void *dst, int64_t bytes)
+{
+    return batch_snapshot_io(b, slot, dst, bytes, true);
+}
+
+int af_batch_restore(af_batch *b, int32_t slot, const void *src, int64_t bytes)
+{
+    return batch_snapshot_io(b, slot, const_cast<void *>(src), bytes, false);
 }
 
 int af_batch_apply_velocity_bc(af_batch *b, int32_t slot)

@@ -23,6 +23,7 @@
 #include "af_internal.h"
 #include "af_mixed.h"
 #include "af_direct.h"
+#include "af_snapshot.h"
 #include "af_rbgs_split.h"
 #include "af_rbgs_tb.h"
 
@@ -1107,6 +1108,30 @@ int af_field_device_ptr(const af_engine *e, int id, uint64_t *out)
 }
 
 int64_t af_arena_bytes(const af_engine *e) { return e ? e->bytes : 0; }
+
+int af_snapshot_bytes(const af_engine *e, int64_t *bytes)
+{
+    if (int rc = check_open(e)) return rc;
+    if (!bytes) return fail(AF_ERR_ARG, "aeroflux: null argument");
+    *bytes = snapshot_bytes(e->cur.f);
+    return AF_OK;
+}
+
+int af_snapshot(af_engine *e, void *dst, int64_t bytes)
+{
+    if (int rc = check_open(e)) return rc;
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    return snapshot_save(e->cur.f, e->arena, e->cfg, dst, bytes,
+                         reinterpret_cast<unsigned long long *>(e->red_out), e->stream);
+}
+
+int af_restore(af_engine *e, const void *src, int64_t bytes)
+{
+    if (int rc = check_open(e)) return rc;
+    HIP_TRY(hipSetDevice(e->device), "set device");
+    return snapshot_load(e->cur.f, e->arena, e->cfg, src, bytes,
+                         reinterpret_cast<unsigned long long *>(e->red_out), e->stream);
+}
 
 // ---- stage entry points ------------------------------------------------
 

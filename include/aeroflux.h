@@ -193,6 +193,40 @@ int af_field_device_ptr(const af_engine *e, int field_id, uint64_t *out);
 /* Total arena bytes (MAC.Bytes, grid.go:128). */
 int64_t af_arena_bytes(const af_engine *e);
 
+/* ------------------------------------------------- snapshot and restore */
+/* No counterpart in the reference (it has no engine snapshot format).  A
+ * snapshot is this 128-byte header followed by the eight fields in field-id
+ * order (P, U, V, W, Div, Tmp0, Tmp1, Tmp2), each Field.Count doubles of
+ * the padded field, with nothing between them; little-endian.  It holds the
+ * whole state a step reads, so restoring it into an engine (or a batch slot)
+ * with the same grid and stepping on gives the same bits as never having
+ * stopped.  Checksum = sum mod 2^64 over the payload's 64-bit words w_i
+ * (i from 0 across all fields) of m(w_i ^ (i * 0x9E3779B97F4A7C15)), m the
+ * SplitMix64 output function (z ^= z >> 30; z *= 0xBF58476D1CE4E5B9;
+ * z ^= z >> 27; z *= 0x94D049BB133111EB; z ^= z >> 31). */
+typedef struct af_snapshot_header {
+    char Magic[8];             /* 'A','F','S','N','A','P',0,1 */
+    uint32_t Version;          /* 1 */
+    uint32_t HeaderBytes;      /* 128 */
+    int64_t NX, NY, NZ, Ghost; /* must match the engine restored into */
+    double H, DT, Viscosity, Density; /* recorded only */
+    int64_t PayloadBytes;      /* 8 * sum of the eight Field.Count */
+    uint64_t Checksum;
+    uint8_t Reserved[32];      /* zero */
+} af_snapshot_header;
+
+/* Bytes a snapshot of this engine takes (header + payload). */
+int af_snapshot_bytes(const af_engine *e, int64_t *bytes);
+/* Writes a snapshot into dst (host or device memory of at least
+ * af_snapshot_bytes bytes; pinned host memory copies fastest). */
+int af_snapshot(af_engine *e, void *dst, int64_t bytes);
+/* Restores a snapshot.  The header is checked (magic, version, grid, sizes)
+ * before the arena is touched: AF_ERR_ARG leaves the engine as it was.  A
+ * payload whose checksum does not match is rejected with AF_ERR_ARG after
+ * zeroing the eight fields.  Config values other than the grid (DT,
+ * Viscosity, Precision ...) are the engine's own, not the snapshot's. */
+int af_restore(af_engine *e, const void *src, int64_t bytes);
+
 /* ----------------------------------------- stage entry points (parity) */
 /* These mirror the engine's unexported methods that the reference's own
  * in-package tests call (engine_test.go:121, :334). */
@@ -334,6 +368,12 @@ int af_batch_apply_velocity_bc(af_batch *b, int32_t slot);          /* stencil.g
 int af_batch_max_divergence(af_batch *b, int32_t slot, double *out); /* project.go:210-232 */
 int af_batch_kinetic_energy(af_batch *b, int32_t slot, double *out); /* project.go:238-258 */
 int af_batch_mixed_stats(af_batch *b, int32_t slot, double *last_div, int32_t *cycles);
+/* Snapshot and restore of one slot, in the engine's format (above): a
+ * slot's snapshot restores into an engine of the same grid and the other
+ * way round. */
+int af_batch_snapshot_bytes(const af_batch *b, int32_t slot, int64_t *bytes);
+int af_batch_snapshot(af_batch *b, int32_t slot, void *dst, int64_t bytes);
+int af_batch_restore(af_batch *b, int32_t slot, const void *src, int64_t bytes);
 
 /* Runs the autotuner now on the resident slots (it otherwise runs at the
  * first step): advection shape and persistent workgroup counts, each timed

@@ -110,3 +110,32 @@ def test_sparse_solver_is_rejected(lib):
     # sparse_fallback.go:17-19 -- checked before any device work
     with pytest.raises(af.AeroFluxError, match="sparse direct solver"):
         af.New(af.Config(NX=8, NY=8, NZ=8, UseSparseSolver=True))
+
+
+def test_snapshot_header_layout_matches_c(tmp_path):
+    """af_snapshot_header as gcc lays it out equals the ctypes mirror
+    (128 bytes, every field at the same offset)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    names = [n for n, _ in af.SnapshotHeader._fields_]
+    src = tmp_path / "hdr.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "aeroflux.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu\\n", sizeof(af_snapshot_header));\n'
+        + "".join(f'  printf("%zu\\n", offsetof(af_snapshot_header, {n}));\n' for n in names)
+        + "  return 0;\n}\n")
+    exe = tmp_path / "hdr"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    assert out[0] == 128 == C_sizeof(af.SnapshotHeader)
+    assert out[1:] == [getattr(af.SnapshotHeader, n).offset for n in names]
+
+
+def C_sizeof(t):
+    import ctypes
+    return ctypes.sizeof(t)
