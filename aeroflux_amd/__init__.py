@@ -48,6 +48,8 @@ FIELD_NAMES = ("P", "U", "V", "W", "Div", "Tmp0", "Tmp1", "Tmp2")
 AXIS_X, AXIS_Y, AXIS_Z = range(3)
 COLOR_RED, COLOR_BLACK = range(2)
 PRECISION_FP64, PRECISION_MIXED = range(2)
+STREAM_READ, STREAM_COPY, STREAM_2R1W, STREAM_4R3W = range(4)
+SERVE_FCFS, SERVE_LEAST_WORK_PREEMPT = 0, 1
 
 
 class AeroFluxError(RuntimeError):
@@ -83,6 +85,7 @@ class _CBatchConfig(C.Structure):
         ("GaussSeidelIters", C.c_int64), ("MixedMaxCycles", C.c_int32),
         ("MixedSweeps", C.c_int32), ("MixedFixSweeps", C.c_int32), ("MixedChecked", C.c_int32),
         ("SweepShape", C.c_int32), ("Workgroups", C.c_int32), ("SweepStaged", C.c_int32),
+        ("ServePolicy", C.c_int32), ("PreemptRatio", C.c_double), ("SnapshotBudget", C.c_int64),
     ]
 
 
@@ -140,7 +143,7 @@ class _CServeResult(C.Structure):
 class _CServeStats(C.Structure):
     _fields_ = [
         ("Wall", C.c_double), ("Steps", C.c_int64), ("CellUpdates", C.c_int64),
-        ("MaxResident", C.c_int32), ("Pad", C.c_int32), ("PeakBytes", C.c_int64),
+        ("MaxResident", C.c_int32), ("Preemptions", C.c_int32), ("PeakBytes", C.c_int64),
     ]
 
 
@@ -185,6 +188,7 @@ _SIGNATURES = [
     ("af_subtract_gradient", C.c_int, [C.c_void_p]),
     ("af_project", C.c_int, [C.c_void_p]),
     ("af_time_kernel", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    ("af_measure_stream", C.c_int, [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     ("af_mixed_stats", C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
     ("af_solver_tile", C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]),
@@ -593,12 +597,26 @@ class BatchConfig:
     SweepShape: int = 0
     Workgroups: int = 0
     SweepStaged: int = 0
+    ServePolicy: int = SERVE_FCFS
+    PreemptRatio: float = 0.0
+    SnapshotBudget: int = 0
 
     def _to_c(self) -> _CBatchConfig:
         c = _CBatchConfig()
         for f in dc_fields(self):
-            setattr(c, f.name, int(getattr(self, f.name)))
+            v = getattr(self, f.name)
+            setattr(c, f.name, float(v) if f.name == "PreemptRatio" else int(v))
         return c
+
+
+
+
+def MeasureStream(kind: int, n: int, reps: int = 20) -> float:
+    """Attainable streaming bandwidth in GB/s of one access pattern over
+    arrays of n doubles (af_measure_stream)."""
+    out = C.c_double()
+    _check(load_library().af_measure_stream(kind, n, reps, C.byref(out)))
+    return float(out.value)
 
 
 def SlotBytes(cfg: Config, precision: int) -> int:
@@ -784,10 +802,10 @@ def Serve(batch: Batch, jobs: list[dict], kmax: int = 8, slope: float = -5.0 / 3
         cb = C.cast(None, _ServeProgress)
     else:
         def report(p, done, _user):
-            progress({name: getattr(p.contents, name) for name, _ in _CServeStats._fields_
-                      if name != "Pad"}, int(done))
+            progress({name: getattr(p.contents, name) for name, _ in _CServeStats._fields_},
+                     int(done))
         cb = _ServeProgress(report)
     _check(load_library().af_serve(batch._handle(), arr, n, kmax, slope, rms, int(realtime),
                                    res, C.byref(st), cb, float(progress_interval), None))
     out = [{name: getattr(res[i], name) for name, _ in _CServeResult._fields_} for i in range(n)]
-    return out, {name: getattr(st, name) for name, _ in _CServeStats._fields_ if name != "Pad"}
+    return out, {name: getattr(st, name) for name, _ in _CServeStats._fields_}

@@ -24,7 +24,9 @@
 #include <thread>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <new>
+#include <set>
 #include <vector>
 
 #include "../../include/aeroflux.h"
@@ -1067,6 +1069,12 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
     if (c.SweepStaged < -1 || c.SweepStaged > kLdsVariants)
         return fail(AF_ERR_CONFIG, "aeroflux: staged sweep variant %d out of range -1..%d",
                     (int)c.SweepStaged, kLdsVariants);
+    if (c.PreemptRatio == 0) c.PreemptRatio = 4.0;
+    if (c.SnapshotBudget == 0) c.SnapshotBudget = (int64_t)32 << 30;
+    if (c.ServePolicy < 0 || c.ServePolicy > 1 || !(c.PreemptRatio >= 1.0) ||
+        std::isinf(c.PreemptRatio) || c.SnapshotBudget < 0)
+        return fail(AF_ERR_CONFIG,
+                    "aeroflux: serve policy must be 0 or 1, PreemptRatio >= 1, SnapshotBudget >= 0");
 
     af_batch *b = new (std::nothrow) af_batch();
     if (!b) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
@@ -1619,7 +1627,29 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
     };
     std::vector<Res> resident;
     std::deque<std::pair<int64_t, std::vector<int64_t>>> pending;
-    std::deque<int64_t> queue;
+    // Waiting jobs, ordered by key: arrival index (first come first served)
+    // or remaining work, cells x steps left, ties by arrival (policy 1).
+    const bool by_work = b->cfg.ServePolicy == 1;
+    std::set<std::pair<double, int64_t>> queue;
+    auto work_of = [&](int64_t j, int64_t left) {
+        return (double)(cfgs[j].NX * cfgs[j].NY * cfgs[j].NZ) * (double)left;
+    };
+    auto enqueue = [&](int64_t j, int64_t left) {
+        queue.insert({by_work ? work_of(j, left) : (double)j, j});
+    };
+    // Preempted jobs: host snapshot of the slot's arena and its fault flag.
+    struct Held {
+        std::unique_ptr<unsigned char[]> buf;
+        int64_t bytes = 0;
+        int64_t left = 0;
+        int flag = 0;
+        bool once = false;  // preempted already
+    };
+    std::vector<Held> held(by_work ? (size_t)n : 0);
+    int64_t held_bytes = 0;
+    auto left_now = [&](int64_t j) {
+        return by_work && held[(size_t)j].buf ? held[(size_t)j].left : jobs[j].Steps;
+    };
     int64_t next = 0, step = 0, completed = -1, jobs_done = 0;
     double last_report = 0;
     af_serve_stats st{};
@@ -1649,49 +1679,127 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         out[i].Slot = -1;
         out[i].Status = status[i];
     }
+    // Policy 1: snapshot the resident job with the most remaining work to
+    // host memory, free its slot and requeue it, if it has more than
+    // PreemptRatio times the work of queued job j and the snapshot fits the
+    // host budget.  Every step in flight completes first (the snapshot is
+    // stream-ordered and synchronises), so the snapshot is the state after
+    // the job's enqueued steps and `left` steps remain.
+    auto preempt_for = [&](int64_t j, bool *done) -> int {
+        *done = false;
+        if (!by_work || resident.empty()) return AF_OK;
+        // A job is preempted at most once: after it resumes it runs to the
+        // end, which bounds the snapshot traffic a steady stream of short
+        // arrivals can cause.
+        size_t v = resident.size();
+        for (size_t r = 0; r < resident.size(); r++)
+            if (!held[(size_t)resident[r].job].once &&
+                (v == resident.size() || work_of(resident[r].job, resident[r].left) >
+                                             work_of(resident[v].job, resident[v].left)))
+                v = r;
+        if (v == resident.size()) return AF_OK;
+        const Res x = resident[v];
+        // Strictly more: within one admission pass (no step in between) two
+        // jobs can then never evict each other in turn.
+        if (!(work_of(x.job, x.left) > b->cfg.PreemptRatio * work_of(j, left_now(j))))
+            return AF_OK;
+        HSlot &h = b->slots[x.slot];
+        const int64_t nb = snapshot_bytes(h.L.f);
+        if (held_bytes + nb > b->cfg.SnapshotBudget) return AF_OK;
+        std::unique_ptr<unsigned char[]> buf(new (std::nothrow) unsigned char[(size_t)nb]);
+        if (!buf) return AF_OK;
+        unsigned char *base = reinterpret_cast<unsigned char *>(b->pool.base + h.off);
+        auto *acc = reinterpret_cast<unsigned long long *>(base + h.L.out_off);
+        if (int e = snapshot_save(h.L.f, base, h.cfg, buf.get(), nb, acc, b->stream)) return e;
+        int flag = 0;
+        HIP_TRY(hipMemcpy(&flag, b->d_ctl + x.slot, sizeof(int), hipMemcpyDeviceToHost),
+                "preempt");
+        while (!pending.empty()) HIP_TRY(finish_oldest(), "preempt");
+        free_slot(b, x.slot);
+        Held &k = held[(size_t)x.job];
+        k.buf = std::move(buf);
+        k.bytes = nb;
+        k.left = x.left;
+        k.flag = flag;
+        k.once = true;
+        held_bytes += nb;
+        resident[v] = resident.back();
+        resident.pop_back();
+        enqueue(x.job, x.left);
+        st.Preemptions++;
+        *done = true;
+        return AF_OK;
+    };
     while (err == hipSuccess && rc == AF_OK) {
         const double t = now();
-        while (next < n && (!realtime || jobs[next].Arrival <= t)) queue.push_back(next++);
-        // Admission, first come first served, while the pool can place the
-        // head job.
-        while (!queue.empty() && err == hipSuccess) {
-            const int64_t j = queue.front();
+        while (next < n && (!realtime || jobs[next].Arrival <= t)) {
+            enqueue(next, jobs[next].Steps);
+            next++;
+        }
+        // Admission in queue order while the pool can place the head job
+        // (policy 1 may preempt to make room).
+        while (!queue.empty() && err == hipSuccess && rc == AF_OK) {
+            const int64_t j = queue.begin()->second;
             if (status[j] == AF_ERR_OOM) {
                 out[j].Admitted = out[j].Done = t;
-                queue.pop_front();
+                queue.erase(queue.begin());
                 continue;
             }
             const int cand = pick_slot(b);
-            if (cand < 0) break;
-            // The slot's pinned descriptor is rewritten below; an upload of
-            // it for an earlier job must have run (it has once that job's
-            // first step is complete).
-            while (b->slots[cand].desc_step > completed && !pending.empty() &&
-                   (err = finish_oldest()) == hipSuccess) {
+            if (cand >= 0) {
+                // The slot's pinned descriptor is rewritten below; an upload
+                // of it for an earlier job must have run (it has once that
+                // job's first step is complete).
+                while (b->slots[cand].desc_step > completed && !pending.empty() &&
+                       (err = finish_oldest()) == hipSuccess) {
+                }
+                if (err != hipSuccess) break;
             }
-            if (err != hipSuccess) break;
             int slot;
-            if (place_slot(b, cfgs[j], lays[j], &slot) != AF_OK) break;
+            if (cand < 0 || place_slot(b, cfgs[j], lays[j], &slot) != AF_OK) {
+                bool made_room = false;
+                rc = preempt_for(j, &made_room);
+                if (made_room) continue;
+                break;
+            }
             HSlot &h = b->slots[slot];
             h.desc_step = step;
             const DField U = slot_field(b, h, AF_FIELD_U), V = slot_field(b, h, AF_FIELD_V),
                          W = slot_field(b, h, AF_FIELD_W);
             const Geom G = slot_geom(h);
-            if (err == hipSuccess)
-                err = launch_band_limited(U, V, W, G, h.cfg.H, jobs[j].Seed, kmax, slope, rms,
-                                          b->ic, b->stream);
-            if (err == hipSuccess) err = launch_velocity_bc(U, V, W, G, b->stream);
-            out[j].Admitted = t;
+            int64_t left = jobs[j].Steps;
+            if (by_work && held[(size_t)j].buf) {
+                // Resume a preempted job from its snapshot.
+                Held &k = held[(size_t)j];
+                unsigned char *base = reinterpret_cast<unsigned char *>(b->pool.base + h.off);
+                auto *acc = reinterpret_cast<unsigned long long *>(base + h.L.out_off);
+                rc = snapshot_load(h.L.f, base, h.cfg, k.buf.get(), k.bytes, acc, b->stream);
+                if (rc != AF_OK) break;
+                if (k.flag) {
+                    err = hipMemcpyAsync(b->d_ctl + slot, &k.flag, sizeof(int),
+                                         hipMemcpyHostToDevice, b->stream);
+                    if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
+                }
+                left = k.left;
+                held_bytes -= k.bytes;
+                k.buf.reset();
+            } else {
+                if (err == hipSuccess)
+                    err = launch_band_limited(U, V, W, G, h.cfg.H, jobs[j].Seed, kmax, slope,
+                                              rms, b->ic, b->stream);
+                if (err == hipSuccess) err = launch_velocity_bc(U, V, W, G, b->stream);
+                out[j].Admitted = t;
+            }
             out[j].Slot = slot;
-            resident.push_back({j, slot, jobs[j].Steps});
-            queue.pop_front();
+            resident.push_back({j, slot, left});
+            queue.erase(queue.begin());
             st.MaxResident = std::max<int32_t>(st.MaxResident, (int32_t)resident.size());
             st.PeakBytes = std::max(st.PeakBytes, b->pool.used);
         }
-        if (err != hipSuccess) break;
+        if (err != hipSuccess || rc != AF_OK) break;
         if (resident.empty() && !queue.empty()) {
             rc = fail(AF_ERR_OOM, "aeroflux: an empty batch cannot place job %ld",
-                      (long)queue.front());
+                      (long)queue.begin()->second);
             break;
         }
         auto retire = [&](const std::vector<int64_t> &done_jobs) -> hipError_t {

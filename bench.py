@@ -280,7 +280,44 @@ def big_leg(af, n: int, steps: int) -> dict:
     return out
 
 
-def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dict:
+def serve_summary(jobs, res, st) -> dict:
+    ok = [r for r in res if r["Status"] == 0]
+    lat = [r["Done"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
+    que = [r["Admitted"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
+    svc = [r["Done"] - r["Admitted"] for r in ok]
+    return {
+        "completed": len(ok),
+        "faulted_or_refused": len(jobs) - len(ok),
+        "wall_s": st["Wall"],
+        "drain_s": st["Wall"] - (jobs[-1]["Arrival"] if jobs else 0.0),
+        "value": st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0,
+        "unit": "cell-updates/s",
+        "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat, default=0.0)},
+        "queueing_s": {"p50": pct(que, 50), "p99": pct(que, 99)},
+        "service_s": {"p50": pct(svc, 50), "p99": pct(svc, 99)},
+        "batched_steps": st["Steps"],
+        "max_resident_jobs": st["MaxResident"],
+        "peak_pool_bytes": st["PeakBytes"],
+        "preemptions": st["Preemptions"],
+    }
+
+
+def serve_once(af, jobs, budget: float, policy: int, label: str):
+    b = af.Batch(af.BatchConfig(Precision=af.PRECISION_MIXED, MemoryBudget=int(budget),
+                                MaxSlots=4096, ServePolicy=policy))
+
+    def report(so_far, done):
+        print(f"served ({label}): {so_far['Wall']:.0f} s, {done}/{len(jobs)} jobs done, "
+              f"{so_far['Steps']} batched steps", file=sys.stderr, flush=True)
+
+    res, st = af.Serve(b, jobs, realtime=True, progress=report, progress_interval=30.0)
+    info = b.info()
+    b.Close()
+    return res, st, info
+
+
+def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
+               policies: str) -> dict:
     """Served step-job stream: n_jobs jobs arriving as a Poisson process at
     `rate`/s, axis sizes from {32,64,96,128,192,256}, N uniform in [10,200],
     nu log-uniform in [1e-6,1e-4], DT = min(0.004, 0.5 H^2/(6 nu)), seeded
@@ -289,19 +326,17 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dic
     admitted first come first served under the HBM budget.  The first
     n_plain jobs are also run on the plain FP64 path, each alone (New,
     init_band_limited, StepN; StepN timed), for its cell-updates/s and each
-    job's final max|div|."""
+    job's final max|div|.  policies "both" runs the same stream again under
+    least-remaining-work-first admission with preemption (snapshots to host
+    memory) and compares its results with the first run's; "preempt" runs
+    only that policy."""
     from aeroflux_amd import workload
 
     jobs = workload.served_jobs(n_jobs, seed=7, rate=rate)
-    b = af.Batch(af.BatchConfig(Precision=af.PRECISION_MIXED, MemoryBudget=int(budget),
-                                MaxSlots=4096))
-    def report(so_far, done):
-        print(f"served: {so_far['Wall']:.0f} s, {done}/{n_jobs} jobs done, "
-              f"{so_far['Steps']} batched steps", file=sys.stderr, flush=True)
-
-    res, st = af.Serve(b, jobs, realtime=True, progress=report, progress_interval=30.0)
-    info = b.info()
-    b.Close()
+    first = af.SERVE_LEAST_WORK_PREEMPT if policies == "preempt" else af.SERVE_FCFS
+    label = {af.SERVE_FCFS: "first come first served",
+             af.SERVE_LEAST_WORK_PREEMPT: "least work first, preempting"}[first]
+    res, st, info = serve_once(af, jobs, budget, first, label)
     plain = None
     if n_plain > 0:
         el, cu, rel = 0.0, 0, []
@@ -325,28 +360,14 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dic
                  "served_mixed_final_max_divergence_rel_diff": {
                      "max_abs": float(np.abs(rel).max()) if rel.size else 0.0,
                      "mean": float(rel.mean()) if rel.size else 0.0}}
-    ok = [r for r in res if r["Status"] == 0]
-    lat = [r["Done"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
-    que = [r["Admitted"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
-    svc = [r["Done"] - r["Admitted"] for r in ok]
     demand = sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
-    return {
+    out = {
         "jobs": n_jobs,
-        "completed": len(ok),
-        "faulted_or_refused": n_jobs - len(ok),
         "arrival_rate_per_s": rate,
         "last_arrival_s": jobs[-1]["Arrival"] if jobs else 0.0,
-        "wall_s": st["Wall"],
-        "drain_s": st["Wall"] - (jobs[-1]["Arrival"] if jobs else 0.0),
-        "value": st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0,
-        "unit": "cell-updates/s",
         "offered_cell_updates_per_s": demand / jobs[-1]["Arrival"] if jobs else 0.0,
-        "latency_s": {"p50": pct(lat, 50), "p99": pct(lat, 99), "max": max(lat, default=0.0)},
-        "queueing_s": {"p50": pct(que, 50), "p99": pct(que, 99)},
-        "service_s": {"p50": pct(svc, 50), "p99": pct(svc, 99)},
-        "batched_steps": st["Steps"],
-        "max_resident_jobs": st["MaxResident"],
-        "peak_pool_bytes": st["PeakBytes"],
+        "policy": label,
+        **serve_summary(jobs, res, st),
         "memory_budget_bytes": int(budget),
         "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
                   "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
@@ -354,6 +375,22 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int) -> dic
         "speedup_vs_plain_fp64": (st["CellUpdates"] / st["Wall"] / plain["value"]
                                   if plain and st["Wall"] > 0 else None),
     }
+    if policies == "both":
+        pres, pst, _ = serve_once(af, jobs, budget, af.SERVE_LEAST_WORK_PREEMPT,
+                                  "least work first, preempting")
+        same = all((a["MaxDivergence"], a["KineticEnergy"], a["Status"]) ==
+                   (c["MaxDivergence"], c["KineticEnergy"], c["Status"])
+                   for a, c in zip(res, pres))
+        out["least_work_first_preempting"] = {
+            "policy": "least remaining work (cells x steps left) first; a resident job with "
+                      "more than 4x the work of a queued job that cannot be placed is "
+                      "snapshotted to host memory and requeued (at most once per job)",
+            **serve_summary(jobs, pres, pst),
+            "speedup_vs_plain_fp64": (pst["CellUpdates"] / pst["Wall"] / plain["value"]
+                                      if plain and pst["Wall"] > 0 else None),
+            "results_identical_to_first_come_first_served": same,
+        }
+    return out
 
 
 def cpu_baseline(cfg: dict, f: dict, seconds: float, workers: int) -> dict:
@@ -427,6 +464,9 @@ def main() -> int:
                     help="jobs in the served stream (the full stream is 10000)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--served-budget", type=float, default=200e9)
+    ap.add_argument("--served-policy", choices=("fcfs", "preempt", "both"), default="both",
+                    help="admission policies of the served stream: first come first served, "
+                         "least remaining work first with preemption, or both in turn")
     ap.add_argument("--served-plain-jobs", type=int, default=60,
                     help="jobs of the stream also run alone on the plain FP64 path")
     args = ap.parse_args()
@@ -474,6 +514,14 @@ def main() -> int:
         "gradient_ms": e.time_kernel(3, 10),
         "pressure_solve_ms": e.time_kernel(6, 10),
     }
+    # What a plain streaming kernel with the pass's pattern (two reads, one
+    # write) and the gradient's (four reads, three writes) attain from HBM:
+    # 512 MiB arrays, so the 256 MB last-level cache holds none of the
+    # working set (the pass's own four split arrays, 268 MB at 256^3, partly
+    # fit it).
+    stream_n = 64 << 20
+    stream_2r1w = af.MeasureStream(af.STREAM_2R1W, stream_n, 20)
+    stream_4r3w = af.MeasureStream(af.STREAM_4R3W, stream_n, 10)
     tuned = {"sweep_shape": e.sweep_shape()[0] + 1, "sweep_slab_rows": e.sweep_slab(),
              "pass_slab_workgroup_rows": e.pass_slab()}
     e.Close()
@@ -511,7 +559,10 @@ def main() -> int:
             "traffic": traffic_from_profile(f"split_pass_{n}_pmc.json"),
             "algorithmic_bytes_per_launch": rb_bytes,
             "avg_launch_ms": t_rb,
+            "attainable_same_pattern": stream_2r1w,
+            "frac_of_attainable": achieved / stream_2r1w,
         },
+        "stream_gbps": {"2r1w_512MiB_arrays": stream_2r1w, "4r3w_512MiB_arrays": stream_4r3w},
         "kernels": kernels,
         "tuned": tuned,
     }
@@ -603,7 +654,8 @@ def main() -> int:
         out["batched"] = batched_leg(af, ic, args.batched_engines, args.batched_steps, 2)
     if not args.no_served:
         out["served"] = served_leg(af, args.served_jobs, args.served_rate, args.served_budget,
-                                   min(args.served_plain_jobs, args.served_jobs))
+                                   min(args.served_plain_jobs, args.served_jobs),
+                                   args.served_policy)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, f, args.cpu_seconds, args.cpu_workers)

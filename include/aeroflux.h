@@ -257,6 +257,14 @@ int af_project(af_engine *e);                           /* project.go:12-34 */
  * counterpart (measurement only); it overwrites P and the Tmp fields. */
 int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
 
+/* Attainable streaming bandwidth (GB/s, bytes moved / time) of one access
+ * pattern over arrays of n doubles each (n even), timed over `reps`
+ * back-to-back launches after one warm-up: kind 0 = read one array, 1 =
+ * copy, 2 = two reads and one write (a colour pass's pattern), 3 = four
+ * reads and three writes (the gradient's).  Allocates and frees its arrays.
+ * No reference counterpart (measurement only). */
+int af_measure_stream(int kind, int64_t n, int reps, double *gbps);
+
 /* Mixed precision only: max|div| implied by the last FP64 residual check
  * (0 if MixedTarget == 0) and the cycles the last solve ran.  No reference
  * counterpart. */
@@ -319,6 +327,17 @@ typedef struct af_batch_config {
                                   gathers); k (1..4) -> variant k-1, i.e.
                                   (rows, z-planes) of (4,16), (4,32), (8,16),
                                   (8,32) per workgroup */
+    int32_t ServePolicy;       /* af_serve admission: 0 -> first come first
+                                  served; 1 -> least remaining work
+                                  (cells x steps left) first, preempting: a
+                                  resident job is snapshotted to host memory
+                                  and requeued when a queued job that cannot
+                                  be placed has less than 1/PreemptRatio of
+                                  its remaining work; each job is preempted
+                                  at most once */
+    double PreemptRatio;       /* policy 1 (0 -> 4) */
+    int64_t SnapshotBudget;    /* policy 1: host bytes that preempted jobs'
+                                  snapshots may hold at once (0 -> 32 GiB) */
 } af_batch_config;
 
 typedef struct af_batch_info {
@@ -428,7 +447,7 @@ typedef struct af_serve_stats {
     int64_t Steps;             /* batched steps replayed */
     int64_t CellUpdates;       /* sum over jobs of NX*NY*NZ*N */
     int32_t MaxResident;       /* peak concurrent jobs */
-    int32_t Pad;
+    int32_t Preemptions;       /* jobs snapshotted out and requeued */
     int64_t PeakBytes;         /* peak pool bytes in use */
 } af_serve_stats;
 

@@ -139,3 +139,20 @@ def test_snapshot_header_layout_matches_c(tmp_path):
 def C_sizeof(t):
     import ctypes
     return ctypes.sizeof(t)
+
+
+@pytest.mark.parametrize("kw", [dict(ServePolicy=2), dict(ServePolicy=-1),
+                                dict(ServePolicy=1, PreemptRatio=0.5),
+                                dict(SnapshotBudget=-1)])
+def test_batch_serve_policy_is_validated(lib, kw):
+    """Rejected before any device call, so this runs without a GPU."""
+    with pytest.raises(af.AeroFluxError) as ei:
+        af.Batch(af.BatchConfig(**kw))
+    assert ei.value.code == af.AF_ERR_CONFIG
+
+
+def test_stream_measurement_rejects_bad_arguments(lib):
+    for args in ((4, 1024, 1), (0, 1023, 1), (0, 1024, 0), (-1, 1024, 1)):
+        with pytest.raises(af.AeroFluxError) as ei:
+            af.MeasureStream(*args)
+        assert ei.value.code == af.AF_ERR_ARG

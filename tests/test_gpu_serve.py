@@ -31,13 +31,15 @@ def alone(af, prec, j):
     return out
 
 
+@pytest.mark.parametrize("policy", [afm.SERVE_FCFS, afm.SERVE_LEAST_WORK_PREEMPT])
 @pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
 @pytest.mark.parametrize("realtime", [False, True])
-def test_served_results_equal_alone(af, prec, realtime):
+def test_served_results_equal_alone(af, prec, realtime, policy):
     jobs = small_stream(14, 5 + prec, rate=200.0)
     biggest = max(af.SlotBytes(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], Precision=prec),
                                prec) for j in jobs)
-    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=3 * biggest, MaxSlots=16))
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=3 * biggest, MaxSlots=16,
+                                ServePolicy=policy))
     res, st = af.Serve(b, jobs, realtime=realtime)
     assert st["Steps"] > 0 and 1 <= st["MaxResident"] <= len(jobs)
     assert st["PeakBytes"] <= 3 * biggest
@@ -72,6 +74,34 @@ def test_served_staged_sweep_results_equal_alone(af, prec):
         d, ke = alone(af, prec, j)
         assert r["MaxDivergence"] == d, j
         assert r["KineticEnergy"] == ke, j
+    b.Close()
+
+
+@pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
+def test_preempted_jobs_resume_with_the_same_results(af, prec):
+    """Least-remaining-work-first with preemption: two long 32^3 jobs fill
+    the pool; short 8^3 jobs arriving while they run cannot be placed, so a
+    long job is snapshotted to host memory and requeued, the short jobs run,
+    and the long job resumes from its snapshot.  Every result equals the job
+    run alone, and the short jobs finish before the long ones."""
+    big = dict(NX=32, NY=32, NZ=32, H=1 / 32, Viscosity=1e-4, DT=0.002, Steps=600)
+    jobs = [dict(big, Seed=1, Arrival=0.0), dict(big, Seed=2, Arrival=0.0)]
+    for i in range(6):
+        jobs.append(dict(NX=8, NY=8, NZ=8, H=1 / 8, Viscosity=1e-4, DT=0.004, Steps=3,
+                         Seed=10 + i, Arrival=0.03 + 0.002 * i))
+    pool = 2 * af.SlotBytes(af.Config(NX=32, NY=32, NZ=32, Precision=prec), prec)
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=pool, MaxSlots=16,
+                                ServePolicy=afm.SERVE_LEAST_WORK_PREEMPT))
+    res, st = af.Serve(b, jobs, realtime=True)
+    assert st["Preemptions"] >= 1
+    assert st["CellUpdates"] == sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
+    assert b.info()["Slots"] == 0 and b.info()["BytesInUse"] == 0
+    for j, r in zip(jobs, res):
+        assert r["Status"] == afm.AF_OK
+        d, ke = alone(af, prec, j)
+        assert r["MaxDivergence"] == d, j
+        assert r["KineticEnergy"] == ke, j
+    assert max(r["Done"] for r in res[2:]) <= max(r["Done"] for r in res[:2])
     b.Close()
 
 
