@@ -50,6 +50,7 @@ COLOR_RED, COLOR_BLACK = range(2)
 PRECISION_FP64, PRECISION_MIXED = range(2)
 STREAM_READ, STREAM_COPY, STREAM_2R1W, STREAM_4R3W = range(4)
 SERVE_FCFS, SERVE_LEAST_WORK_PREEMPT = 0, 1
+DIRECT_AUTO, DIRECT_FFT, DIRECT_DENSE = 0, 1, 2
 
 
 class AeroFluxError(RuntimeError):
@@ -169,6 +170,7 @@ _SIGNATURES = [
     ("af_cfl", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("af_enable_sparse_solver", C.c_int, [C.c_void_p]),
     ("af_enable_direct_solver", C.c_int, [C.c_void_p]),
+    ("af_enable_direct_solver_method", C.c_int, [C.c_void_p, C.c_int32]),
     ("af_direct_solver_enabled", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     ("af_field_desc", C.c_int, [C.c_void_p, C.c_int, C.POINTER(_CField)]),
     ("af_field_read", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]),
@@ -469,10 +471,19 @@ class Engine:
         reference build."""
         _check(load_library().af_enable_direct_solver(self._handle()))
 
-    def direct_solver_enabled(self) -> bool:
+    def EnableDirectSolverMethod(self, method: int) -> None:
+        """EnableDirectSolver with the transform chosen: DIRECT_AUTO,
+        DIRECT_FFT (power-of-two extents 8..512) or DIRECT_DENSE."""
+        _check(load_library().af_enable_direct_solver_method(self._handle(), method))
+
+    def direct_solver_method(self) -> int:
+        """0 (red-black solve), DIRECT_FFT or DIRECT_DENSE."""
         on = C.c_int32()
         _check(load_library().af_direct_solver_enabled(self._handle(), C.byref(on)))
-        return bool(on.value)
+        return int(on.value)
+
+    def direct_solver_enabled(self) -> bool:
+        return self.direct_solver_method() != 0
 
     def Close(self) -> None:
         """Close, aeroflux.go:419-439; idempotent."""

@@ -19,7 +19,10 @@
 #include <cmath>
 #include <vector>
 
+#include <algorithm>
+
 #include "af_bodies.h"
+#include "af_dct.h"
 #include "af_xcd.h"
 
 #pragma clang fp contract(off)
@@ -177,7 +180,13 @@ rocblas_status transform(DirectState &d, const double *in, double *tmp, double *
 
 }  // namespace
 
-hipError_t init_direct(DirectState &d, const Geom &G, double H, hipStream_t s, int *blas_status)
+bool direct_fft_supported(const Geom &G)
+{
+    return dct_fft_supported(G.NX) && dct_fft_supported(G.NY) && dct_fft_supported(G.NZ);
+}
+
+hipError_t init_direct(DirectState &d, const Geom &G, double H, int method, hipStream_t s,
+                       int *blas_status)
 {
     *blas_status = 0;
     d.nx = G.NX; d.ny = G.NY; d.nz = G.NZ;
@@ -190,6 +199,32 @@ hipError_t init_direct(DirectState &d, const Geom &G, double H, hipStream_t s, i
             e = hipMemcpy(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
         return e;
     };
+    if (method == DIRECT_AUTO) method = direct_fft_supported(G) ? DIRECT_FFT : DIRECT_DENSE;
+    if (method == DIRECT_FFT) {
+        const int ext[3] = {G.NX, G.NY, G.NZ};
+        for (int ax = 0; ax < 3; ax++) {
+            std::vector<double> tw, pw, wc, iq, c0((size_t)ext[ax]);
+            dct_tables(ext[ax], &tw, &pw, &wc, &iq);
+            for (int k = 0; k < ext[ax]; k++) c0[(size_t)k] = pw[2 * (size_t)k];  // s_k cos(pi k / 2N)
+            if ((err = up(&d.tw[ax], tw)) != hipSuccess) return err;
+            if ((err = up(&d.pw[ax], pw)) != hipSuccess) return err;
+            if ((err = up(&d.wc[ax], wc)) != hipSuccess) return err;
+            if ((err = up(&d.iq[ax], iq)) != hipSuccess) return err;
+            if ((err = up(&d.c0[ax], c0)) != hipSuccess) return err;
+        }
+        if ((err = up(&d.lx, eigenvalues(G.NX, d.h2))) != hipSuccess) return err;
+        if ((err = up(&d.ly, eigenvalues(G.NY, d.h2))) != hipSuccess) return err;
+        if ((err = up(&d.lz, eigenvalues(G.NZ, d.h2))) != hipSuccess) return err;
+        d.npartials = std::max(dct_blocks(G.NX, G.NY, G.NZ), dct_blocks(G.NZ, G.NX, G.NY));
+        if ((err = hipMalloc(&d.w0, (size_t)n * sizeof(double))) != hipSuccess) return err;
+        if ((err = hipMalloc(&d.w1, (size_t)n * sizeof(double))) != hipSuccess) return err;
+        if ((err = hipMalloc(&d.partials, (size_t)d.npartials * sizeof(double))) != hipSuccess)
+            return err;
+        if ((err = hipMalloc(&d.sum, sizeof(double))) != hipSuccess) return err;
+        if ((err = hipMalloc(&d.shift, sizeof(double))) != hipSuccess) return err;
+        d.method = DIRECT_FFT;
+        return hipSuccess;
+    }
     if ((err = up(&d.cx, dct_matrix(G.NX))) != hipSuccess) return err;
     if ((err = up(&d.cy, dct_matrix(G.NY))) != hipSuccess) return err;
     if ((err = up(&d.cz, dct_matrix(G.NZ))) != hipSuccess) return err;
@@ -211,24 +246,105 @@ hipError_t init_direct(DirectState &d, const Geom &G, double H, hipStream_t s, i
         *blas_status = (int)st;
         return hipErrorUnknown;
     }
+    d.npartials = kSumBlocks;
+    d.method = DIRECT_DENSE;
     return hipStreamSynchronize(s);
 }
 
 void release_direct(DirectState &d)
 {
     if (d.blas) (void)rocblas_destroy_handle(d.blas);
-    double **bufs[] = {&d.cx, &d.cy, &d.cz, &d.lx, &d.ly, &d.lz, &d.w0, &d.w1, &d.partials, &d.sum};
+    double **bufs[] = {&d.cx, &d.cy, &d.cz, &d.lx, &d.ly, &d.lz, &d.w0, &d.w1, &d.partials, &d.sum,
+                       &d.shift};
     for (double **b : bufs) {
         if (*b) (void)hipFree(*b);
         *b = nullptr;
     }
+    for (int ax = 0; ax < 3; ax++) {
+        double **t[] = {&d.tw[ax], &d.pw[ax], &d.wc[ax], &d.iq[ax], &d.c0[ax]};
+        for (double **b : t) {
+            if (*b) (void)hipFree(*b);
+            *b = nullptr;
+        }
+    }
     d.blas = nullptr;
+    d.method = 0;
 }
+
+namespace {
+
+// The FFT form: five transform passes and two fixed-order reductions.
+hipError_t enqueue_fft_solve(DirectState &d, const DField &P, const DField &Div, const Geom &G,
+                             hipStream_t s)
+{
+    const int64_t NX = G.NX, NY = G.NY, NXY = NX * NY;
+    hipError_t err;
+    auto tables = [&](DctPass &p, int ax) {
+        p.tw = d.tw[ax];
+        p.pw = d.pw[ax];
+        p.wc = d.wc[ax];
+        p.iq = d.iq[ax];
+    };
+    // x forward: -Div (arena interior) -> w0, with block sums of the rhs.
+    DctPass x;
+    // Field.Index (grid.go:24-26) of the first interior cell.
+    const int64_t g = G.g;
+    x.in = Div.d + (g + g * Div.sy + g * (int64_t)Div.sz);
+    x.in_sn = 1; x.in_sa = Div.sy; x.in_sb = Div.sz;
+    x.out = d.w0;
+    x.out_sn = 1; x.out_sa = NX; x.out_sb = NXY;
+    x.A = G.NY; x.B = G.NZ;
+    x.sign = -1.0;
+    x.partials = d.partials;
+    tables(x, 0);
+    if ((err = launch_dct_pass(1, G.NX, x, s)) != hipSuccess) return err;
+    if ((err = launch_reduce_partials(d.partials, dct_blocks(G.NX, G.NY, G.NZ), nullptr, 0.0, 0,
+                                      d.sum, s)) != hipSuccess)
+        return err;
+    // y forward: w0 -> w1.
+    DctPass y;
+    y.in = d.w0; y.out = d.w1;
+    y.in_sn = y.out_sn = NX; y.in_sa = y.out_sa = 1; y.in_sb = y.out_sb = NXY;
+    y.A = G.NX; y.B = G.NZ;
+    tables(y, 1);
+    if ((err = launch_dct_pass(0, G.NY, y, s)) != hipSuccess) return err;
+    // z forward, division, z inverse, in place in w1.
+    DctPass z;
+    z.in = d.w1; z.out = d.w1;
+    z.in_sn = z.out_sn = NXY; z.in_sa = z.out_sa = 1; z.in_sb = z.out_sb = NX;
+    z.A = G.NX; z.B = G.NY;
+    z.lx = d.lx; z.ly = d.ly; z.lz = d.lz;
+    z.c0x = d.c0[0]; z.c0y = d.c0[1]; z.c0z = d.c0[2];
+    z.sum = d.sum;
+    z.partials = d.partials;
+    tables(z, 2);
+    if ((err = launch_dct_pass(2, G.NZ, z, s)) != hipSuccess) return err;
+    if ((err = launch_reduce_partials(d.partials, dct_blocks(G.NZ, G.NX, G.NY), d.sum, d.h2, 1,
+                                      d.shift, s)) != hipSuccess)
+        return err;
+    // y inverse: w1 -> w0.
+    y.in = d.w1; y.out = d.w0;
+    if ((err = launch_dct_pass(3, G.NY, y, s)) != hipSuccess) return err;
+    // x inverse: w0 -> P's interior, plus the shift.
+    DctPass xi;
+    xi.in = d.w0;
+    xi.in_sn = 1; xi.in_sa = NX; xi.in_sb = NXY;
+    xi.out = P.d + (g + g * P.sy + g * (int64_t)P.sz);
+    xi.out_sn = 1; xi.out_sa = P.sy; xi.out_sb = P.sz;
+    xi.A = G.NY; xi.B = G.NZ;
+    xi.shift = d.shift;
+    tables(xi, 0);
+    if ((err = launch_dct_pass(4, G.NX, xi, s)) != hipSuccess) return err;
+    return launch_pressure_bc(P, G, false, s);
+}
+
+}  // namespace
 
 hipError_t enqueue_direct_solve(DirectState &d, const DField &P, const DField &Div, const Geom &G,
                                 hipStream_t s, int *blas_status)
 {
     *blas_status = 0;
+    if (d.method == DIRECT_FFT) return enqueue_fft_solve(d, P, Div, G, s);
     const int64_t n = (int64_t)G.NX * G.NY * G.NZ;
     const unsigned gb = grid_for(n);
     const int np = (int)(n < kSumBlocks ? n : kSumBlocks);

@@ -1031,12 +1031,28 @@ int af_enable_sparse_solver(af_engine *e)
 
 int af_enable_direct_solver(af_engine *e)
 {
+    return af_enable_direct_solver_method(e, DIRECT_AUTO);
+}
+
+int af_enable_direct_solver_method(af_engine *e, int32_t method)
+{
     if (int rc = check_open(e)) return rc;
-    if (e->direct.ready()) return AF_OK;
+    if (method < DIRECT_AUTO || method > DIRECT_DENSE)
+        return fail(AF_ERR_ARG, "aeroflux: direct solver method %d is not 0, 1 or 2", (int)method);
+    if (method == DIRECT_FFT && !direct_fft_supported(e->G))
+        return fail(AF_ERR_UNSUPPORTED,
+                    "aeroflux: the FFT direct solve needs every extent a power of two in 8..512 "
+                    "(grid %dx%dx%d)",
+                    e->G.NX, e->G.NY, e->G.NZ);
+    const int want = method == DIRECT_AUTO
+                         ? (direct_fft_supported(e->G) ? DIRECT_FFT : DIRECT_DENSE)
+                         : method;
+    if (e->direct.method == want) return AF_OK;
     HIP_TRY(hipSetDevice(e->device), "set device");
     HIP_TRY(hipStreamSynchronize(e->stream), "enable direct solver");
+    release_direct(e->direct);
     int bs = 0;
-    const hipError_t err = init_direct(e->direct, e->G, e->cfg.H, e->stream, &bs);
+    const hipError_t err = init_direct(e->direct, e->G, e->cfg.H, want, e->stream, &bs);
     if (err != hipSuccess) {
         release_direct(e->direct);
         if (bs != 0) return fail(AF_ERR_HIP, "aeroflux: direct solver: BLAS status %d", bs);
@@ -1054,7 +1070,7 @@ int af_enable_direct_solver(af_engine *e)
 int af_direct_solver_enabled(const af_engine *e, int32_t *on)
 {
     if (!e || !on) return fail(AF_ERR_ARG, "aeroflux: null argument");
-    *on = e->direct.ready() ? 1 : 0;
+    *on = e->direct.method;
     return AF_OK;
 }
 

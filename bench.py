@@ -631,7 +631,12 @@ def main() -> int:
         dsteps = args.steps
         de, delt = run_gpu(af, cfg, f, args.warmup, dsteps, direct=True)
         ddiv = de.MaxDivergence()
+        method = de.direct_solver_method()
         t_solve = de.time_kernel(6, 10)
+        t_dense = None
+        if method == af.DIRECT_FFT:
+            de.EnableDirectSolverMethod(af.DIRECT_DENSE)
+            t_dense = de.time_kernel(6, 10)
         de.Close()
         dval = cells * dsteps / delt
         out["direct_solver"] = {
@@ -641,11 +646,14 @@ def main() -> int:
             "ms_per_step": 1e3 * delt / dsteps,
             "speedup_vs_fp64": dval / value,
             "pressure_solve_ms": t_solve,
+            "dense_transform_pressure_solve_ms": t_dense,
             "fp64_pressure_solve_ms": kernels["pressure_solve_ms"],
+            "transforms": {af.DIRECT_FFT: "FFT passes", af.DIRECT_DENSE: "dense GEMMs"}[method],
             "final_max_divergence": ddiv,
             "fp64_final_max_divergence": d64,
             "solver": "pinned Neumann system solved exactly: orthonormal DCT-II along each "
-                      "axis as FP64 GEMMs (rocBLAS), spectral division, inverse DCT",
+                      "axis (FFT passes for power-of-two extents, else FP64 GEMMs), spectral "
+                      "division, inverse DCT",
         }
 
     if not args.no_big:

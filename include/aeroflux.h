@@ -171,10 +171,18 @@ int af_enable_sparse_solver(af_engine *e);
  * in solvePressureDirect (:127-172): the interior 7-point Neumann Laplacian
  * with the first cell's diagonal raised by 1/H^2, right-hand side -Div,
  * followed by applyPressureBC.  Solved spectrally (orthonormal DCT-II along
- * each axis as FP64 GEMMs), exact up to rounding.  Once enabled every
- * projection uses it in place of the red-black sweeps (project.go:19-27).
- * No counterpart in the reference's Linux build. */
+ * each axis), exact up to rounding.  Once enabled every projection uses it
+ * in place of the red-black sweeps (project.go:19-27).  The transforms run
+ * as FFT passes when every extent is a power of two in 8..512, otherwise as
+ * FP64 GEMMs against DCT matrices.  No counterpart in the reference's Linux
+ * build. */
 int af_enable_direct_solver(af_engine *e);
+/* The same with the transform chosen: 0 = as above, 1 = FFT passes
+ * (AF_ERR_UNSUPPORTED for other extents), 2 = dense GEMMs.  Switching
+ * method releases the other one's buffers. */
+int af_enable_direct_solver_method(af_engine *e, int32_t method);
+/* *on = 0 when the red-black solve runs, else the method in use (1 FFT,
+ * 2 dense). */
 int af_direct_solver_enabled(const af_engine *e, int32_t *on);
 
 /* --------------------------------------------------------- field access */

@@ -43,16 +43,26 @@ def pinned_matrix(nx, ny, nz, h):
                          shape=(n, n))
 
 
-GRIDS = [(10, 12, 8, 1), (16, 16, 16, 1), (7, 5, 9, 2), (24, 20, 6, 1)]
+# Grids with a non-power-of-two extent take the dense transforms; the rest
+# the FFT passes (every tile shape: extents 8 .. 512, ghost 1 and 2).
+GRIDS = [(10, 12, 8, 1), (16, 16, 16, 1), (7, 5, 9, 2), (24, 20, 6, 1),
+         (32, 8, 16, 2), (8, 64, 32, 1), (128, 16, 8, 1), (8, 8, 256, 1), (512, 8, 8, 1)]
+
+
+def is_pow2_grid(shp):
+    return all(n in (8, 16, 32, 64, 128, 256, 512) for n in shp[:3])
 
 
 @pytest.mark.parametrize("shp", GRIDS)
 def test_direct_solve_matches_pinned_system(af, shp):
     nx, ny, nz, g = shp
     h = 1.0 / max(nx, ny, nz)
-    e = af.New(af.Config(NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, DT=0.01, Viscosity=1e-4))
+    # The solve does not depend on DT or the viscosity; these keep every grid
+    # inside Config.Normalize's diffusion stability limit.
+    e = af.New(af.Config(NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, DT=0.01, Viscosity=1e-6))
     e.EnableDirectSolver()
     assert e.direct_solver_enabled()
+    assert e.direct_solver_method() == (afm.DIRECT_FFT if is_pow2_grid(shp) else afm.DIRECT_DENSE)
     rng = np.random.default_rng(nx * 100 + ny)
     div = np.zeros(e.Mesh.field(afm.FIELD_DIV).shape)
     interior = rng.standard_normal((nz, ny, nx)) * 50.0
@@ -74,7 +84,8 @@ def test_direct_solve_matches_pinned_system(af, shp):
     assert np.array_equal(p[g:g + nz, g + ny, g:g + nx], pin[:, -1, :])
 
 
-def test_direct_projection_is_divergence_free(af):
+@pytest.mark.parametrize("method", [afm.DIRECT_FFT, afm.DIRECT_DENSE])
+def test_direct_projection_is_divergence_free(af, method):
     n = 32
     cfg = dict(NX=n, NY=n, NZ=n, H=1.0 / n, DT=0.002, Viscosity=1e-4)
     f = ic.taylor_green(n, n, n, 1, 1.0 / n, seed=4, amp=2.0, noise=0.3)
@@ -83,7 +94,8 @@ def test_direct_projection_is_divergence_free(af):
         e.Mesh.SetData(e.Mesh.field(fid), f[name])
     e.apply_velocity_bc()
     before = e.MaxDivergence()
-    e.EnableDirectSolver()
+    e.EnableDirectSolverMethod(method)
+    assert e.direct_solver_method() == method
     e.project()
     after = e.MaxDivergence()
     e.Close()
@@ -105,12 +117,14 @@ def stage_step(e, dt):
     e.project()
 
 
-def test_direct_step_graph_equals_stages(af):
+@pytest.mark.parametrize("shp", [(24, 20, 16), (32, 16, 64)])
+def test_direct_step_graph_equals_stages(af, shp):
     """The direct solve inside the captured step graph, replayed step after
-    step, gives the bits of the same stages enqueued one by one."""
-    n = 24
-    cfg = dict(NX=n, NY=20, NZ=16, H=1.0 / n, DT=0.004, Viscosity=1e-4)
-    f = ic.taylor_green(n, 20, 16, 1, 1.0 / n, seed=6, amp=1.5, noise=0.1)
+    step, gives the bits of the same stages enqueued one by one (dense
+    transforms with their GEMMs in the graph, and the FFT passes)."""
+    n, ny, nz = shp
+    cfg = dict(NX=n, NY=ny, NZ=nz, H=1.0 / n, DT=0.004, Viscosity=1e-4)
+    f = ic.taylor_green(n, ny, nz, 1, 1.0 / n, seed=6, amp=1.5, noise=0.1)
     out = []
     for graph in (True, False):
         e = af.New(af.Config(**cfg))
@@ -180,3 +194,49 @@ def test_direct_solver_steps_stably(af):
     div = e.MaxDivergence()
     e.Close()
     assert div < 1e-12, div
+
+
+def random_div_solve(af, shp, method, seed):
+    nx, ny, nz, g = shp
+    h = 1.0 / max(nx, ny, nz)
+    e = af.New(af.Config(NX=nx, NY=ny, NZ=nz, Ghost=g, H=h, DT=0.01, Viscosity=1e-6))
+    e.EnableDirectSolverMethod(method)
+    rng = np.random.default_rng(seed)
+    div = np.zeros(e.Mesh.field(afm.FIELD_DIV).shape)
+    div[g:g + nz, g:g + ny, g:g + nx] = rng.standard_normal((nz, ny, nx)) * 20.0
+    e.Mesh.SetData(e.Mesh.field(afm.FIELD_DIV), div)
+    e.solve_pressure()
+    p = e.Mesh.Data(e.Mesh.field(afm.FIELD_P))
+    e.Close()
+    return p
+
+
+@pytest.mark.parametrize("shp", [(64, 32, 16, 1), (16, 128, 8, 2), (256, 16, 32, 1)])
+def test_fft_and_dense_transforms_agree(af, shp):
+    """The FFT passes and the dense DCT GEMMs solve the same system; on the
+    same right-hand side they agree to rounding, ghost layers included."""
+    a = random_div_solve(af, shp, afm.DIRECT_FFT, 17)
+    b = random_div_solve(af, shp, afm.DIRECT_DENSE, 17)
+    scale = np.abs(b).max()
+    assert np.abs(a - b).max() <= 1e-12 * scale, (np.abs(a - b).max(), scale)
+
+
+def test_direct_method_switch_and_unsupported_grid(af):
+    e = af.New(af.Config(NX=32, NY=16, NZ=8))
+    e.EnableDirectSolverMethod(afm.DIRECT_DENSE)
+    assert e.direct_solver_method() == afm.DIRECT_DENSE
+    e.EnableDirectSolverMethod(afm.DIRECT_FFT)
+    assert e.direct_solver_method() == afm.DIRECT_FFT
+    e.EnableDirectSolver()                      # auto keeps the FFT passes here
+    assert e.direct_solver_method() == afm.DIRECT_FFT
+    e.Close()
+    e = af.New(af.Config(NX=24, NY=16, NZ=8))
+    with pytest.raises(af.AeroFluxError) as ei:
+        e.EnableDirectSolverMethod(afm.DIRECT_FFT)
+    assert ei.value.code == afm.AF_ERR_UNSUPPORTED
+    assert e.direct_solver_method() == 0
+    with pytest.raises(af.AeroFluxError):
+        e.EnableDirectSolverMethod(3)
+    e.EnableDirectSolver()
+    assert e.direct_solver_method() == afm.DIRECT_DENSE
+    e.Close()
