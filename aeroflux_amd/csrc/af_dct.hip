@@ -42,15 +42,19 @@ enum Mode { FWD = 0, FWD_SUM = 1, SOLVE = 2, INV = 3, INV_SHIFT = 4 };
 template <int N>
 struct Shape {
     static constexpr int T = N / 4;                 // threads per FFT
-    static constexpr int C = 256 / T;               // FFTs in flight per workgroup
-    static constexpr int L = N >= 512 ? 8 : (2 * C > 16 ? 2 * C : 16);   // lines per tile
+    static constexpr int L = N >= 512 ? 8 : (2048 / N > 16 ? 2048 / N : 16);  // lines per tile
     static constexpr int P = L / 2;                 // line pairs (complex FFTs) per tile
+    // 256 threads: each takes one radix-4 butterfly of C = 1024/N pairs at
+    // a time, in R rounds.  (512 threads, one round, measured slower at
+    // N = 256: the fused solve pass then fits one workgroup per CU.)
+    static constexpr int THREADS = 256;
+    static constexpr int C = THREADS / T;           // FFTs in flight per workgroup
     static constexpr int R = P / C;                 // rounds of FFTs
     static constexpr int LOG = N == 8 ? 3 : N == 16 ? 4 : N == 32 ? 5 : N == 64 ? 6
                              : N == 128 ? 7 : N == 256 ? 8 : 9;
     static constexpr int NP = N + 1;                // LDS stride of a pair (padded: the
                                                     // strided tile loads hit distinct banks)
-    static constexpr int PER = L * N / 256;         // tile elements per thread
+    static constexpr int PER = L * N / THREADS;     // tile elements per thread
     static_assert(P % C == 0, "pairs must be a multiple of the FFTs in flight");
     static_assert((1 << LOG) == N, "N must be a power of two in 8..512");
 };
@@ -134,7 +138,7 @@ __device__ __forceinline__ double block_sum(double v, double *red)
     red[threadIdx.x] = v;
     __syncthreads();
 #pragma unroll
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = Shape<N>::THREADS / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
@@ -142,11 +146,11 @@ __device__ __forceinline__ double block_sum(double v, double *red)
 }
 
 template <int N, int MODE>
-__global__ __launch_bounds__(256) void k_dct(DctPass a)
+__global__ __launch_bounds__(Shape<N>::THREADS) void k_dct(DctPass a)
 {
     using S = Shape<N>;
     __shared__ double2 buf[S::P * S::NP];
-    __shared__ double red[256];
+    __shared__ double red[S::THREADS];
     const int tiles_a = (a.A + S::L - 1) / S::L;
     const int b = blockIdx.x / tiles_a;
     const int a0 = (blockIdx.x % tiles_a) * S::L;
@@ -165,13 +169,13 @@ __global__ __launch_bounds__(256) void k_dct(DctPass a)
     double vals[S::PER];
 #pragma unroll
     for (int r = 0; r < S::PER; r++) {
-        const int i = threadIdx.x + r * 256;
+        const int i = threadIdx.x + r * S::THREADS;
         const int l = in_rows ? i / N : i % S::L, n = in_rows ? i % N : i / S::L;
         vals[r] = a0 + l < a.A ? at[n * a.in_sn + (int64_t)l * a.in_sa] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < S::PER; r++) {
-        const int i = threadIdx.x + r * 256;
+        const int i = threadIdx.x + r * S::THREADS;
         const int l = in_rows ? i / N : i % S::L, n = in_rows ? i % N : i / S::L;
         const double v = a.sign * vals[r];
         if (MODE == FWD_SUM) acc += v;
@@ -279,7 +283,7 @@ __global__ __launch_bounds__(256) void k_dct(DctPass a)
     const auto to = out + (int64_t)a0 * a.out_sa + (int64_t)b * a.out_sb;
 #pragma unroll
     for (int r = 0; r < S::PER; r++) {
-        const int i = threadIdx.x + r * 256;
+        const int i = threadIdx.x + r * S::THREADS;
         const int l = out_rows ? i / N : i % S::L, n = out_rows ? i % N : i / S::L;
         if (a0 + l >= a.A) continue;
         const double *slot;
@@ -323,8 +327,8 @@ hipError_t launch_mode(int n, const DctPass &p, hipStream_t s)
     switch (n) {
 #define AF_DCT_CASE(NN)                                                                        \
     case NN:                                                                                   \
-        hipLaunchKernelGGL((k_dct<NN, MODE>), dim3(dct_blocks(NN, p.A, p.B)), dim3(256), 0, s, \
-                           p);                                                                 \
+        hipLaunchKernelGGL((k_dct<NN, MODE>), dim3(dct_blocks(NN, p.A, p.B)),                 \
+                           dim3(Shape<NN>::THREADS), 0, s, p);                                 \
         break;
         AF_DCT_CASE(8)
         AF_DCT_CASE(16)

@@ -42,6 +42,17 @@ def bytes_per_cell_step(iters: int) -> int:
     return 288 + 2 * (88 + 48 * iters)
 
 
+def model_gbps(value: float, bytes_per_cell: int = 0) -> dict:
+    """Effective bandwidth of a cell-updates/s rate under the FP64 bytes model
+    (one read of each input, one write of each output per reference stage;
+    24 sweep pairs unless bytes_per_cell is given), and its fraction of the
+    8 TB/s HBM peak.  For mixed-precision runs it is the FP64-equivalent
+    rate, not traffic."""
+    b = bytes_per_cell or bytes_per_cell_step(24)
+    g = value * b / 1e9
+    return {"model_bytes_per_cell_step": b, "model_gbps": g, "model_roofline_frac": g / HBM_PEAK_GBPS}
+
+
 def hip_sync() -> None:
     rc = ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
     if rc != 0:
@@ -264,6 +275,7 @@ def big_leg(af, n: int, steps: int) -> dict:
         out[name] = {"value": n ** 3 * (steps - 1) / el, "unit": "cell-updates/s",
                      "ms_per_step": 1e3 * el / (steps - 1),
                      "final_max_divergence": e.MaxDivergence()}
+        out[name].update(model_gbps(out[name]["value"]))
         fields[name] = {k: e.Mesh.Data(e.Mesh.field(i)) for i, k in enumerate("PUVW")}
         e.Close()
     err = {}
@@ -277,6 +289,42 @@ def big_leg(af, n: int, steps: int) -> dict:
     d64, dmx = out["fp64"]["final_max_divergence"], out["mixed"]["final_max_divergence"]
     out["mixed"]["final_max_divergence_rel_diff"] = (dmx - d64) / d64
     out["timing"] = "steps 2..N timed (the first step also builds the step graph)"
+    return out
+
+
+def small_leg(af, ic, n: int, steps: int, workers: int, cpu: bool) -> dict:
+    """The 64^3 FP64 workload (H=1/64, DT=0.001, nu=1e-5, seeded Taylor-Green
+    + noise), 100 steps on the GPU and, unless skipped, the same 100 steps
+    on the C oracle (16-thread Z-slab pool): both rates, and whether all
+    eight fields end bit-identical.  The 19 MB arena stays in the last-level
+    cache, so its bandwidth is not an HBM figure."""
+    h = 1.0 / n
+    cfg = dict(NX=n, NY=n, NZ=n, H=h, DT=0.001, Viscosity=1e-5)
+    f = ic.taylor_green(n, n, n, 1, h, seed=1)
+    e, el = run_gpu(af, cfg, f, 0, steps)
+    gpu = [e.Mesh.Data(e.Mesh.field(k)) for k in range(8)]
+    e.Close()
+    value = n ** 3 * steps / el
+    out = {"grid": [n, n, n], "steps": steps, "value": value, "unit": "cell-updates/s",
+           "ms_per_step": 1e3 * el / steps, "cache_resident": True, **model_gbps(value)}
+    if cpu:
+        from oracle import oracle
+        o = oracle.OracleEngine(dict(cfg, Workers=workers))
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            o.field(fid)[...] = f[name]
+        o.apply_velocity_bc()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if o.step() != 0:
+                raise RuntimeError("oracle step faulted")
+        cel = time.perf_counter() - t0
+        same = all(np.array_equal(gpu[k].view(np.uint64).ravel(),
+                                  o.field(k).view(np.uint64).ravel()) for k in range(8))
+        o.close()
+        out["cpu_baseline"] = {"value": n ** 3 * steps / cel, "unit": "cell-updates/s",
+                               "cores": workers, "kind": "port",
+                               "sample": f"all {steps} steps, same initial condition"}
+        out["bit_identical_to_oracle"] = same
     return out
 
 
@@ -299,6 +347,7 @@ def serve_summary(jobs, res, st) -> dict:
         "max_resident_jobs": st["MaxResident"],
         "peak_pool_bytes": st["PeakBytes"],
         "preemptions": st["Preemptions"],
+        **model_gbps(st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0),
     }
 
 
@@ -456,6 +505,7 @@ def main() -> int:
     ap.add_argument("--target-sweeps", type=int, default=24,
                     help="FP32 sweep pairs of the first cycle of the iterated-to-target run")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the 64^3 leg")
     ap.add_argument("--no-big", action="store_true")
     ap.add_argument("--big-n", type=int, default=512)
     ap.add_argument("--big-steps", type=int, default=100)
@@ -567,6 +617,9 @@ def main() -> int:
         "tuned": tuned,
     }
 
+    if not args.no_small:
+        out["fp64_64"] = small_leg(af, ic, 64, 100, args.cpu_workers, not args.no_cpu_baseline)
+
     if not args.no_mixed:
         # Same 24 sweep pairs, one FP32 cycle: the iterate is the FP64 one up
         # to FP32 rounding, so the final max|div| matches the FP64 run's to
@@ -594,6 +647,7 @@ def main() -> int:
             "final_max_divergence_rel_diff": (dmx - d64) / d64,
             "cycles_last_solve": cyc,
             "max_rel_error_vs_fp64": errs,
+            **model_gbps(mval),
         }
         # Iterated to the FP64 path's final max|div|: a short first cycle, then
         # correction cycles until the FP64 residual says the projection's
@@ -622,6 +676,7 @@ def main() -> int:
                         "final value",
             "cycles_last_solve": tcyc,
             "max_rel_error_vs_fp64": terrs,
+            **model_gbps(tval),
         }
     del fp64_fields
 
@@ -649,6 +704,9 @@ def main() -> int:
             "dense_transform_pressure_solve_ms": t_dense,
             "fp64_pressure_solve_ms": kernels["pressure_solve_ms"],
             "transforms": {af.DIRECT_FFT: "FFT passes", af.DIRECT_DENSE: "dense GEMMs"}[method],
+            # 6 sweeps 288 B + per projection divergence 32, solve 80 (FFT
+            # passes), gradient 56.
+            **model_gbps(dval, 288 + 2 * (32 + 80 + 56)),
             "final_max_divergence": ddiv,
             "fp64_final_max_divergence": d64,
             "solver": "pinned Neumann system solved exactly: orthonormal DCT-II along each "
