@@ -687,15 +687,76 @@ int choose_pass_slab(af_engine *e)
     return AF_OK;
 }
 
+// TileK = 0: one colour pass per launch, on the colour-split layout or on the
+// arena layout, whichever solves this grid faster.  The split layout moves
+// half the bytes per pass but adds a gather and a scatter per solve and
+// launches more workgroups; at 256^3 it is 1.7x faster, at 64^3 and below the
+// arena layout wins (fewer, lighter launches).  Timed on the engine's own
+// zeroed arena (a solve from zeros writes +0 everywhere, so nothing changes);
+// cached per grid, ghost width and sweep count for the process.
+std::map<std::tuple<int, int, int, int, int64_t>, int> g_layout_cache;
+
+int choose_one_pass_layout(af_engine *e)
+{
+    const Geom &G = e->G;
+    Tile &t = e->tile64;
+    // The mixed solve has its own FP32 split buffers and never runs the FP64
+    // schedule (whose split buffers it does not allocate).
+    if (e->cfg.Precision == AF_PRECISION_MIXED) {
+        t.k = 0;
+        return AF_OK;
+    }
+    const auto key = std::make_tuple(G.NX, G.NY, G.NZ, G.g, (int64_t)e->cfg.GaussSeidelIters);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_layout_cache.find(key);
+        if (it != g_layout_cache.end()) {
+            t.k = it->second;
+            return AF_OK;
+        }
+    }
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0), "create event");
+    HIP_TRY(hipEventCreate(&t1), "create event");
+    float best = 1e30f;
+    int pick = 0;
+    hipError_t err = hipSuccess;
+    for (int k : {0, -1}) {
+        t.k = k;
+        if (err == hipSuccess) err = enq_solve_fp64(e, e->cur, false);
+        if (err == hipSuccess) err = hipEventRecord(t0, e->stream);
+        for (int r = 0; r < 2 && err == hipSuccess; r++) err = enq_solve_fp64(e, e->cur, false);
+        if (err == hipSuccess) err = hipEventRecord(t1, e->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(t1);
+        float ms = 0;
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, t0, t1);
+        if (err == hipSuccess && ms < best) {
+            best = ms;
+            pick = k;
+        }
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (err != hipSuccess) return hip_fail(err, "tune solve layout");
+    t.k = pick;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_layout_cache[key] = pick;
+    return AF_OK;
+}
+
 int choose_schedule(af_engine *e)
 {
     const af_config &c = e->cfg;
     const Geom &G = e->G;
     Tile &t = e->tile64;
-    if (c.TileK <= 0) {
-        t.k = c.TileK < 0 ? -1 : 0;
+    if (c.TileK < -2)
+        return fail(AF_ERR_CONFIG, "aeroflux: TileK %d must be 0, -1, -2 or 2, 4, 6",
+                    (int)c.TileK);
+    if (c.TileK < 0) {
+        t.k = c.TileK == -1 ? -1 : 0;
         return AF_OK;
     }
+    if (c.TileK == 0) return choose_one_pass_layout(e);
     t.k = c.TileK;
     t.rows = c.TileRows > 0 ? c.TileRows : 2;
     t.cz = c.TileZ > 0 ? c.TileZ : 16;
@@ -866,7 +927,7 @@ int af_new(const af_config *cfg_in, af_engine **out)
     if (cfg.Precision == AF_PRECISION_MIXED) {
         if ((err = init_mixed(e->mixed, e->G, e->stream)) != hipSuccess)
             return bail(err, "allocate mixed-precision buffers", AF_ERR_OOM);
-    } else if (cfg.TileK == 0) {
+    } else if (cfg.TileK == 0 || cfg.TileK == -2) {
         const size_t n = (size_t)split_count(e->G) * sizeof(double);
         double **split[4] = {&e->ps_red, &e->ps_black, &e->ds_red, &e->ds_black};
         for (double **b : split) {

@@ -89,10 +89,13 @@ typedef struct af_config {
                                  max|div| implied by the FP64 residual is
                                  <= this (0 -> run exactly one cycle) */
     int32_t TileK;            /* FP64 red-black schedule: 0 -> one pass per
-                                 launch on the colour-split layout (default);
-                                 -1 -> one pass per launch on the arena
+                                 launch on the colour-split or the arena
+                                 layout, whichever is faster on this grid
+                                 (timed at New; split at 128^3 and up);
+                                 -2 -> colour-split layout; -1 -> arena
                                  layout; 2, 4, 6 -> that many passes per
-                                 launch, temporally blocked, arena layout */
+                                 launch, temporally blocked, arena layout.
+                                 All give the same bits. */
     int32_t TileRows;         /* blocked: rows per thread, 1 or 2 (0 -> tuned) */
     int32_t TileZ;            /* blocked: z-planes per workgroup (0 -> tuned) */
     int32_t SweepShape;       /* advection workgroup shape: 0 -> tuned, k ->

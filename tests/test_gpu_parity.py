@@ -22,7 +22,7 @@ NAMES = ("P", "U", "V", "W", "Div", "Tmp0", "Tmp1", "Tmp2")
 def make_pair(af, oracle_lib, fields=None, **kw):
     kw.setdefault("Workers", 4)
     g = af.New(af.Config(**kw))
-    o = oracle_lib.OracleEngine(dict(kw))
+    o = oracle_lib.OracleEngine({k: v for k, v in kw.items() if not k.startswith("Tile")})
     if fields is not None:
         for fid, name in ((1, "U"), (2, "V"), (3, "W")):
             g.Mesh.SetData(g.Mesh.field(fid), fields[name])
@@ -92,11 +92,18 @@ def test_sweep_bitwise(af, oracle_lib, shp, ax, nu):
     ge.Close()
 
 
+# The one-pass solve layouts TileK picks between by timing: colour-split and
+# arena.  Tests that check the solve pin each one.
+LAYOUTS = {"split": -2, "arena": -1}
+
+
+@pytest.mark.parametrize("layout", list(LAYOUTS))
 @pytest.mark.parametrize("shp", SHAPES)
-def test_projection_stages_bitwise(af, oracle_lib, shp):
+def test_projection_stages_bitwise(af, oracle_lib, shp, layout):
     nx, ny, nz, g = shp
     f = random_fields(shp, 5)
-    kw = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=0.37, DT=0.021, Density=1.3, GaussSeidelIters=7)
+    kw = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=0.37, DT=0.021, Density=1.3, GaussSeidelIters=7,
+              TileK=LAYOUTS[layout])
     ge, oe = make_pair(af, oracle_lib, f, **kw)
     rng = np.random.default_rng(11)
     p = rng.standard_normal(oe.field(0).shape)
@@ -143,10 +150,11 @@ def initial(kind, cfg):
     return ic.taylor_green(nx, ny, nz, g, h, seed=4)
 
 
+@pytest.mark.parametrize("layout", list(LAYOUTS))
 @pytest.mark.parametrize("case", STEP_CASES, ids=[c[0] for c in STEP_CASES])
-def test_step_bitwise(af, oracle_lib, case):
+def test_step_bitwise(af, oracle_lib, case, layout):
     name, cfg, kind, steps = case
-    ge, oe = make_pair(af, oracle_lib, initial(kind, cfg), **cfg)
+    ge, oe = make_pair(af, oracle_lib, initial(kind, cfg), **cfg, TileK=LAYOUTS[layout])
     ge.apply_velocity_bc(); oe.apply_velocity_bc()
     for s in range(steps):
         ge.Step()
@@ -160,10 +168,12 @@ def test_step_bitwise(af, oracle_lib, case):
     ge.Close()
 
 
-def test_config0_64cubed_100_steps_bitwise(af, oracle_lib):
+@pytest.mark.parametrize("layout", list(LAYOUTS))
+def test_config0_64cubed_100_steps_bitwise(af, oracle_lib, layout):
     """The reference CPU case: 64^3, H=1/64, DT=0.004, nu=1e-5, seeded
     Taylor-Green + 1e-3 noise, default 24 sweeps, 100 Step() calls."""
-    cfg = dict(NX=64, NY=64, NZ=64, H=1 / 64, DT=0.004, Viscosity=1e-5, Workers=16)
+    cfg = dict(NX=64, NY=64, NZ=64, H=1 / 64, DT=0.004, Viscosity=1e-5, Workers=16,
+               TileK=LAYOUTS[layout])
     f = ic.taylor_green(64, 64, 64, 1, 1 / 64, seed=1)
     ge, oe = make_pair(af, oracle_lib, f, **cfg)
     ge.apply_velocity_bc(); oe.apply_velocity_bc()
@@ -379,8 +389,10 @@ def test_blocked_solve_bitwise_across_tiles(af, oracle_lib, tile, shp):
     worker-count determinism test (engine_test.go:134-170)."""
     nx, ny, nz, g = shp
     k, rows, cz = tile
+    # k = 0 in the table is the colour-split layout (TileK = -2 pins it; 0
+    # would let the timing choose).
     kw = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=0.41, DT=0.017, GaussSeidelIters=5,
-              TileK=k, TileRows=rows, TileZ=cz)
+              TileK=k if k != 0 else -2, TileRows=rows, TileZ=cz)
     ge = af.New(af.Config(**kw))
     assert ge.solver_tile()[0] == k
     oe = oracle_lib.OracleEngine({k2: v for k2, v in kw.items() if not k2.startswith("Tile")})
@@ -413,8 +425,14 @@ def test_step_bitwise_across_tiles(af, oracle_lib, tile):
 
 def test_schedule_is_reported_and_blocked_tiles_are_tuned(af):
     e = af.New(af.Config(NX=48, NY=40, NZ=32))
-    assert e.solver_tile()[0] == 0  # colour-split by default
+    assert e.solver_tile()[0] in (0, -1)  # timed: colour-split or arena layout
     e.Close()
+    for k, want in ((-2, 0), (-1, -1)):
+        e = af.New(af.Config(NX=48, NY=40, NZ=32, TileK=k))
+        assert e.solver_tile()[0] == want
+        e.Close()
+    with pytest.raises(af.AeroFluxError):
+        af.New(af.Config(NX=48, NY=40, NZ=32, TileK=-3))
     e = af.New(af.Config(NX=48, NY=40, NZ=32, TileK=4))
     k, rows, cz = e.solver_tile()
     assert k == 4 and rows in (1, 2) and cz in (8, 16, 32, 64)
@@ -461,8 +479,8 @@ def test_step_bitwise_across_pass_orders(af, oracle_lib, slab):
     f = ic.taylor_green(40, 30, 18, 1, 1 / 40, seed=23, amp=3.0, noise=0.05)
     ge, oe = make_pair(af, oracle_lib, f, **cfg)
     ge.Close()
-    ge = af.New(af.Config(**cfg, PassSlab=slab))
-    assert ge.pass_slab() == slab
+    ge = af.New(af.Config(**cfg, PassSlab=slab, TileK=-2))
+    assert ge.pass_slab() == slab and ge.solver_tile()[0] == 0
     for fid, name in ((1, "U"), (2, "V"), (3, "W")):
         ge.Mesh.SetData(ge.Mesh.field(fid), f[name])
     for s in range(2):
