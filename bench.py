@@ -118,12 +118,19 @@ def profile_kernels(af, ic, n: int, reps: int) -> None:
     t = e.time_kernel(0, reps)
     name, nb = dominant_kernel(e, n ** 3)
     t_sweep = e.time_kernel(1, reps)
+    # The direct solve's five FFT passes (16 B/cell each), reps + 1 solves.
+    e.EnableDirectSolver()
+    t_direct = e.time_kernel(6, reps)
+    method = e.direct_solver_method()
     e.Close()
     print(json.dumps({"profile_kernels": name, "copy_bytes": nbytes, "copies": 3,
                       "kernel_launches": reps + 1, "avg_ms": t,
                       "algorithmic_bytes_per_launch": nb,
                       "sweep_x_launches": reps + 1, "sweep_x_avg_ms": t_sweep,
-                      "sweep_algorithmic_bytes_per_launch": 48 * n ** 3}), flush=True)
+                      "sweep_algorithmic_bytes_per_launch": 48 * n ** 3,
+                      "direct_solves": reps + 1, "direct_solve_avg_ms": t_direct,
+                      "direct_method": method,
+                      "dct_pass_algorithmic_bytes_per_launch": 16 * n ** 3}), flush=True)
 
 
 def pct(a, q) -> float:
