@@ -295,6 +295,26 @@ def big_leg(af, n: int, steps: int) -> dict:
     out["mixed"]["speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     d64, dmx = out["fp64"]["final_max_divergence"], out["mixed"]["final_max_divergence"]
     out["mixed"]["final_max_divergence_rel_diff"] = (dmx - d64) / d64
+    # The same run with the direct pressure solve (FFT passes at 512^3): an
+    # exact projection, so its fields differ from the 24-sweep solve's by the
+    # iteration's error; only rates and the final max|div| are compared.
+    e = af.New(af.Config(**cfg))
+    e.init_band_limited(5)
+    e.EnableDirectSolver()
+    e.StepN(1)
+    hip_sync()
+    t0 = time.perf_counter()
+    e.StepN(steps - 1)
+    hip_sync()
+    el = time.perf_counter() - t0
+    dv = n ** 3 * (steps - 1) / el
+    out["direct"] = {"value": dv, "unit": "cell-updates/s", "ms_per_step": 1e3 * el / (steps - 1),
+                     "speedup_vs_fp64": dv / out["fp64"]["value"],
+                     "final_max_divergence": e.MaxDivergence(),
+                     "transforms": {af.DIRECT_FFT: "FFT passes",
+                                    af.DIRECT_DENSE: "dense GEMMs"}[e.direct_solver_method()],
+                     "pressure_solve_ms": e.time_kernel(6, 3)}
+    e.Close()
     out["timing"] = "steps 2..N timed (the first step also builds the step graph)"
     return out
 
