@@ -3,8 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle for the HIP
  * path and the timed CPU baseline.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg may load it, and only as the checker; the
- * product library (aeroflux_amd/) never links or calls it.
+ * bench.py's cpu_baseline leg may load it, and only to check results or to
+ * time the reference's algorithm; the product library (aeroflux_amd/) never
+ * links or calls it.
  *
  * It restates, function by function and in the same floating-point operation
  * order, the Go reference:

@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle (``oracle/liboracle.so``).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()``
-and bench.py's cpu_baseline leg, always as the checker -- never by the
-product package.  See aeroflux_oracle.c for the function-by-function
+and bench.py's cpu_baseline leg, always to check or time the reference's
+algorithm -- never by the product package.  See aeroflux_oracle.c for the function-by-function
 citations of the Go reference.
 """
 from __future__ import annotations
