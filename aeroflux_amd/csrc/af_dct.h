@@ -7,6 +7,8 @@
 
 #include <vector>
 
+#include "af_internal.h"
+
 namespace af {
 
 // One transform pass over lines of length n: element e of line (a, b) is at
@@ -28,6 +30,11 @@ struct DctPass {
     const double *c0x = nullptr, *c0y = nullptr, *c0z = nullptr;  // C(k, 0)
     const double *sum = nullptr;    // S, the rhs sum
     const double *shift = nullptr;  // mode 4: added to every output
+    // Mode 5 (x forward from the velocity): computeDivergence's fields and
+    // scale; the divergence is stored to D and -D is transformed.
+    DField U{}, V{}, W{}, D{};
+    Geom G{};
+    double dscale = 0;
 };
 
 bool dct_fft_supported(int n);        // n a power of two in 8 .. 512
@@ -39,7 +46,8 @@ void dct_tables(int n, std::vector<double> *tw, std::vector<double> *pw, std::ve
 
 // mode: 0 forward, 1 forward + block sums of the loaded values, 2 forward +
 // spectral division + inverse (with block partials of q_0), 3 inverse,
-// 4 inverse + *shift.
+// 4 inverse + *shift, 5 as 1 along x with the values computed as the
+// divergence of U, V, W (stored to D on the way).
 hipError_t launch_dct_pass(int mode, int n, const DctPass &p, hipStream_t s);
 
 // One workgroup: mode 0 *out = sum of partials[0..n); mode 1

@@ -37,7 +37,7 @@
 namespace af {
 namespace {
 
-enum Mode { FWD = 0, FWD_SUM = 1, SOLVE = 2, INV = 3, INV_SHIFT = 4 };
+enum Mode { FWD = 0, FWD_SUM = 1, SOLVE = 2, INV = 3, INV_SHIFT = 4, FWD_DIV = 5 };
 
 template <int N>
 struct Shape {
@@ -171,14 +171,30 @@ __global__ __launch_bounds__(Shape<N>::THREADS) void k_dct(DctPass a)
     for (int r = 0; r < S::PER; r++) {
         const int i = threadIdx.x + r * S::THREADS;
         const int l = in_rows ? i / N : i % S::L, n = in_rows ? i % N : i / S::L;
-        vals[r] = a0 + l < a.A ? at[n * a.in_sn + (int64_t)l * a.in_sa] : 0.0;
+        if (MODE == FWD_DIV) {
+            // computeDivergence (project.go:42-62) at interior cell
+            // (n, a0 + l, b), the operations of body::divergence_cell.
+            double dv = 0.0;
+            if (a0 + l < a.A) {
+                const int g = a.G.g, x = n + g, y = a0 + l + g, z = b + g;
+                const auto u = body::gp(a.U.d), v = body::gp(a.V.d), w = body::gp(a.W.d);
+                const double du = u[body::idx(a.U, x + 1, y, z)] - u[body::idx(a.U, x, y, z)];
+                const double dvv = v[body::idx(a.V, x, y + 1, z)] - v[body::idx(a.V, x, y, z)];
+                const double dw = w[body::idx(a.W, x, y, z + 1)] - w[body::idx(a.W, x, y, z)];
+                dv = a.dscale * (du + dvv + dw);
+                body::gp(a.D.d)[body::idx(a.D, x, y, z)] = dv;
+            }
+            vals[r] = dv;
+        } else {
+            vals[r] = a0 + l < a.A ? at[n * a.in_sn + (int64_t)l * a.in_sa] : 0.0;
+        }
     }
 #pragma unroll
     for (int r = 0; r < S::PER; r++) {
         const int i = threadIdx.x + r * S::THREADS;
         const int l = in_rows ? i / N : i % S::L, n = in_rows ? i % N : i / S::L;
         const double v = a.sign * vals[r];
-        if (MODE == FWD_SUM) acc += v;
+        if (MODE == FWD_SUM || MODE == FWD_DIV) acc += v;
         const int pos = inverse ? n : reorder<N>(n);
         double *slot = reinterpret_cast<double *>(&buf[(l >> 1) * S::NP + pos]);
         slot[l & 1] = v;
@@ -297,7 +313,7 @@ __global__ __launch_bounds__(Shape<N>::THREADS) void k_dct(DctPass a)
         }
         to[n * a.out_sn + (int64_t)l * a.out_sa] = v + shift;
     }
-    if (MODE == FWD_SUM || MODE == SOLVE) {
+    if (MODE == FWD_SUM || MODE == FWD_DIV || MODE == SOLVE) {
         const double s = block_sum<N>(acc, red);
         if (threadIdx.x == 0) body::gp(a.partials)[blockIdx.x] = s;
     }
@@ -401,6 +417,7 @@ hipError_t launch_dct_pass(int mode, int n, const DctPass &p, hipStream_t s)
     case 2: return launch_mode<SOLVE>(n, p, s);
     case 3: return launch_mode<INV>(n, p, s);
     case 4: return launch_mode<INV_SHIFT>(n, p, s);
+    case 5: return launch_mode<FWD_DIV>(n, p, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -49,4 +49,11 @@ void release_direct(DirectState &d);
 hipError_t enqueue_direct_solve(DirectState &d, const DField &P, const DField &Div, const Geom &G,
                                 hipStream_t s, int *blas_status);
 
+// The FFT form with the projection's divergence folded into its first pass:
+// computeDivergence(U, V, W) with `scale` is written to Div (bit-identical
+// to the divergence kernel) and solved for in one go.  FFT method only.
+hipError_t enqueue_direct_project_solve(DirectState &d, const DField &P, const DField &Div,
+                                        const DField &U, const DField &V, const DField &W,
+                                        double scale, const Geom &G, hipStream_t s);
+
 }  // namespace af

@@ -273,9 +273,15 @@ void release_direct(DirectState &d)
 
 namespace {
 
-// The FFT form: five transform passes and two fixed-order reductions.
+// The FFT form: five transform passes and two fixed-order reductions.  With
+// `vel` the first pass computes the divergence from the velocity itself.
+struct VelocitySource {
+    DField U, V, W;
+    double scale;
+};
+
 hipError_t enqueue_fft_solve(DirectState &d, const DField &P, const DField &Div, const Geom &G,
-                             hipStream_t s)
+                             hipStream_t s, const VelocitySource *vel = nullptr)
 {
     const int64_t NX = G.NX, NY = G.NY, NXY = NX * NY;
     hipError_t err;
@@ -297,7 +303,13 @@ hipError_t enqueue_fft_solve(DirectState &d, const DField &P, const DField &Div,
     x.sign = -1.0;
     x.partials = d.partials;
     tables(x, 0);
-    if ((err = launch_dct_pass(1, G.NX, x, s)) != hipSuccess) return err;
+    if (vel) {
+        x.in = nullptr;
+        x.U = vel->U; x.V = vel->V; x.W = vel->W; x.D = Div;
+        x.G = G;
+        x.dscale = vel->scale;
+    }
+    if ((err = launch_dct_pass(vel ? 5 : 1, G.NX, x, s)) != hipSuccess) return err;
     if ((err = launch_reduce_partials(d.partials, dct_blocks(G.NX, G.NY, G.NZ), nullptr, 0.0, 0,
                                       d.sum, s)) != hipSuccess)
         return err;
@@ -339,6 +351,15 @@ hipError_t enqueue_fft_solve(DirectState &d, const DField &P, const DField &Div,
 }
 
 }  // namespace
+
+hipError_t enqueue_direct_project_solve(DirectState &d, const DField &P, const DField &Div,
+                                        const DField &U, const DField &V, const DField &W,
+                                        double scale, const Geom &G, hipStream_t s)
+{
+    if (d.method != DIRECT_FFT) return hipErrorInvalidValue;
+    const VelocitySource vel{U, V, W, scale};
+    return enqueue_fft_solve(d, P, Div, G, s, &vel);
+}
 
 hipError_t enqueue_direct_solve(DirectState &d, const DField &P, const DField &Div, const Geom &G,
                                 hipStream_t s, int *blas_status)

@@ -361,10 +361,20 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 // separate launch.
 hipError_t enq_project(af_engine *e, const Desc &d)
 {
-    const bool fuse = fused_rhs(e);
     hipError_t err = enq_velocity_bc(e, d);
-    if (err == hipSuccess) err = enq_divergence(e, d, fuse);
-    if (err == hipSuccess) err = enq_solve(e, d, fuse);
+    if (e->direct.method == DIRECT_FFT) {
+        // The direct solve's first pass computes the divergence itself (the
+        // same values, stored to Div), which saves re-reading Div.
+        if (err == hipSuccess)
+            err = enqueue_direct_project_solve(
+                e->direct, dfield(e, d.f[AF_FIELD_P]), dfield(e, d.f[AF_FIELD_DIV]),
+                dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                dfield(e, d.f[AF_FIELD_W]), e->div_scale, e->G, e->stream);
+    } else {
+        const bool fuse = fused_rhs(e);
+        if (err == hipSuccess) err = enq_divergence(e, d, fuse);
+        if (err == hipSuccess) err = enq_solve(e, d, fuse);
+    }
     if (err == hipSuccess) err = enq_gradient(e, d, true);
     return err;
 }

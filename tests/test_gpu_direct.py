@@ -240,3 +240,32 @@ def test_direct_method_switch_and_unsupported_grid(af):
     e.EnableDirectSolver()
     assert e.direct_solver_method() == afm.DIRECT_DENSE
     e.Close()
+
+
+@pytest.mark.parametrize("shp", [(32, 16, 64, 1), (16, 8, 32, 2)])
+def test_fused_divergence_projection_equals_stages(af, shp):
+    """With the FFT passes, project computes the divergence inside the first
+    transform pass.  It must give the bits of the unfused stages
+    (applyVelocityBC, computeDivergence, solvePressure, subtractGradient,
+    applyVelocityBC; project.go:12-34), Div included."""
+    nx, ny, nz, g = shp
+    cfg = dict(NX=nx, NY=ny, NZ=nz, Ghost=g, H=1.0 / nx, DT=0.004, Viscosity=1e-4)
+    f = ic.taylor_green(nx, ny, nz, g, 1.0 / nx, seed=8, amp=2.0, noise=0.2)
+    out = []
+    for fused in (True, False):
+        e = af.New(af.Config(**cfg))
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            e.Mesh.SetData(e.Mesh.field(fid), f[name])
+        e.EnableDirectSolverMethod(afm.DIRECT_FFT)
+        if fused:
+            e.project()
+        else:
+            e.apply_velocity_bc()
+            e.compute_divergence()
+            e.solve_pressure()
+            e.subtract_gradient()
+            e.apply_velocity_bc()
+        out.append([e.Mesh.Data(e.Mesh.field(k)).tobytes() for k in range(8)])
+        e.Close()
+    for k in range(8):
+        assert out[0][k] == out[1][k], afm.FIELD_NAMES[k]
