@@ -222,7 +222,10 @@ def test_descriptor_swap_state_is_honoured(af, oracle_lib):
 def test_nan_departure_is_a_contained_fault(af):
     """The reference's guard-page/bounds-panic path: a NaN departure
     coordinate makes Step return an error; the engine survives and can be
-    reseeded and stepped again."""
+    reseeded and stepped again.  The counterpart of
+    TestGuardPageFaultIsContained (arena_test.go:51-78), whose deliberate
+    write past the arena has no safe device form: here the fault is raised
+    from the flag, and no access ever leaves the arena."""
     cfg = dict(NX=8, NY=8, NZ=8, H=1 / 8, DT=0.01)
     e = af.New(af.Config(**cfg))
     f = ic.taylor_green(8, 8, 8, 1, 1 / 8, seed=2)
