@@ -543,6 +543,67 @@ __device__ __forceinline__ void gradient_cell(const DField &U, const DField &V, 
     put_grad<BC, 2>(G, W, x, y, z, w[iw] - scale * (p[ip] - p[ip - P.sz]));
 }
 
+// gradient_cell for the ZG positions z0 .. z0+ZG-1 of column (x, y), with
+// the same values.  Every position's loads are issued before the first
+// wait and every result is held in a register before the first store, so
+// ZG positions' loads are in flight together.  A position past the last
+// (z > g+NZ) loads from the last and stores nothing.  The wall faces of the
+// last column and row are updated afterwards, position by position.
+template <bool BC, int ZG>
+__device__ __forceinline__ void gradient_column(const DField &U, const DField &V, const DField &W,
+                                                const DField &P, const Geom &G, double scale,
+                                                int x, int y, int z0)
+{
+    const int g = G.g;
+    if (x >= g + G.NX || y >= g + G.NY) return;
+    const auto p = gp((const double *)P.d);
+    const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
+    const int zl = g + G.NZ;  // last position (W's wall face)
+    double pc[ZG], px[ZG], py[ZG], pz[ZG], uu[ZG], vv[ZG], ww[ZG];
+#pragma unroll
+    for (int r = 0; r < ZG; r++) {
+        const int z = z0 + r < zl ? z0 + r : zl;
+        const int ip = idx(P, x, y, z);
+        pc[r] = p[ip];
+        px[r] = p[ip - 1];
+        py[r] = p[ip - P.sy];
+        pz[r] = p[ip - P.sz];
+        uu[r] = u[idx(U, x, y, z)];  // z = g+NZ is a ghost plane of U and V: in range
+        vv[r] = v[idx(V, x, y, z)];
+        ww[r] = w[idx(W, x, y, z)];
+    }
+    double ou[ZG], ov[ZG], ow[ZG];
+#pragma unroll
+    for (int r = 0; r < ZG; r++) {
+        ou[r] = uu[r] - scale * (pc[r] - px[r]);
+        ov[r] = vv[r] - scale * (pc[r] - py[r]);
+        ow[r] = ww[r] - scale * (pc[r] - pz[r]);
+        asm volatile("" : "+v"(ou[r]), "+v"(ov[r]), "+v"(ow[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < ZG; r++) {
+        const int z = z0 + r;
+        if (z < zl) {
+            put_grad<BC, 0>(G, U, x, y, z, ou[r]);
+            put_grad<BC, 1>(G, V, x, y, z, ov[r]);
+        }
+        if (z <= zl) put_grad<BC, 2>(G, W, x, y, z, ow[r]);
+    }
+    if (x == g + G.NX - 1 || y == g + G.NY - 1) {
+        for (int r = 0; r < ZG; r++) {
+            const int z = z0 + r;
+            if (z >= zl) break;
+            const int ip = idx(P, x, y, z);
+            if (x == g + G.NX - 1)
+                put_grad<BC, 0>(G, U, x + 1, y, z,
+                                u[idx(U, x, y, z) + 1] - scale * (p[ip + 1] - p[ip]));
+            if (y == g + G.NY - 1)
+                put_grad<BC, 1>(G, V, x, y + 1, z,
+                                v[idx(V, x, y, z) + V.sy] - scale * (p[ip + P.sy] - p[ip]));
+        }
+    }
+}
+
 // ---------------------------------------------------- colour-split layout
 //
 // A colour pass of gaussSeidelSweep (project.go:103-130) writes every cell of

@@ -135,8 +135,11 @@ __global__ __launch_bounds__(256) void k_zero(double *d, int64_t n)
     if (i < n) d[i] = 0;
 }
 
-// subtractGradient, project.go:172-205, one thread per interior column and z
-// (BC: with the velocity BC that follows it applied, body::gradient_cell).
+// subtractGradient, project.go:172-205, one thread per interior column and
+// GZ consecutive z positions (BC: with the velocity BC that follows it
+// applied, body::gradient_column).
+constexpr int GZ = 4;
+
 template <bool BC>
 __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField W, DField P,
                                                      Geom G, double scale, int3 nb)
@@ -144,8 +147,8 @@ __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField 
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    body::gradient_cell<BC>(U, V, W, P, G, scale, g + bx * BX + threadIdx.x,
-                            g + by * BY + threadIdx.y, g + bz);
+    body::gradient_column<BC, GZ>(U, V, W, P, G, scale, g + bx * BX + threadIdx.x,
+                                  g + by * BY + threadIdx.y, g + bz * GZ);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -320,7 +323,7 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s)
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
                            hipStream_t s, bool velocity_bc)
 {
-    const int3 nb = nblocks(G.NX, G.NY, G.NZ + 1);
+    const int3 nb = nblocks(G.NX, G.NY, (G.NZ + 1 + GZ - 1) / GZ);  // z = g .. g+NZ
     if (velocity_bc)
         hipLaunchKernelGGL(k_gradient<true>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
                            nb);
