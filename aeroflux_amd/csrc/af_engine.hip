@@ -873,10 +873,11 @@ int af_new(const af_config *cfg_in, af_engine **out)
     *out = nullptr;
     af_config cfg = *cfg_in;
     if (int rc = af_config_normalize(&cfg)) return rc;
-    if (cfg.UseSparseSolver)
-        return fail(AF_ERR_UNSUPPORTED,
-                    "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
-                    "Red-Black iterative solver is used instead");
+    // UseSparseSolver is accepted and changes nothing: New never reads it,
+    // and project() takes the direct solve only once EnableSparseSolver has
+    // attached a factorisation, which the non-Darwin build never does
+    // (aeroflux.go:176-213, project.go:19, sparse_fallback.go:20-24).  The
+    // red-black solve runs, exactly as with the flag clear.
 
     af_engine *e = new (std::nothrow) af_engine();
     if (!e) return fail(AF_ERR_OOM, "aeroflux: out of host memory");

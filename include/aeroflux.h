@@ -75,7 +75,11 @@ typedef struct af_config {
     int64_t Workers;          /* CPU worker count in the reference; kept and
                                  normalised identically, unused on the GPU */
     int64_t GaussSeidelIters;
-    int32_t UseSparseSolver;  /* must stay 0: no direct solver here */
+    int32_t UseSparseSolver;  /* accepted and ignored, as by the reference's
+                                 non-Darwin build: the red-black solve runs
+                                 (project.go:19, sparse_fallback.go:20-24);
+                                 the GPU direct solve is switched on by
+                                 af_enable_direct_solver */
     int32_t Lock;             /* accepted; device memory is always resident */
 
     /* --- extensions --- */

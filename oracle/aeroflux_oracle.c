@@ -42,8 +42,11 @@
  * fault, and makes Step return right after that dispatch.
  *
  * Pinning: every function here is checked by tests/test_oracle.py against
- * closed forms, invariants and an independent numpy restatement; see the
- * docstrings there.
+ * what the mathematics fixes (closed forms on linear, quadratic, uniform and
+ * eigenmode fields, the summation-by-parts identity, planted extrema and
+ * NaNs, the stated Step sequence) and the reference's own test invariants;
+ * each planted-mistake build (-DORC_MUTANT=n) fails at least one closed
+ * form.  An independent numpy restatement is a further cross-check.
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -319,6 +322,13 @@ static inline int sample_trilinear(const orc_field *f, const double *d, double p
     double c0 = c00 * (1 - ty) + c10 * ty;
     double c1 = c01 * (1 - ty) + c11 * ty;
 
+#if ORC_MUTANT == 13
+    if (ty == 0) c0 = c00;
+    if (tz == 0) {
+        *out = c0;
+        return 0;
+    }
+#endif
     *out = c0 * (1 - tz) + c1 * tz;
     return 0;
 }
@@ -439,7 +449,11 @@ void orc_apply_velocity_bc(orc_engine *e)
             u[IDX(U, x, y0 - 1, z)] = NEG(u[IDX(U, x, y0, z)]);
             u[IDX(U, x, y1, z)] = NEG(u[IDX(U, x, y1 - 1, z)]);
         }
+#if ORC_MUTANT == 15
+    for (int64_t y = y0; y < y1; y++)
+#else
     for (int64_t y = y0 - 1; y <= y1; y++)
+#endif
         for (int64_t x = ux0; x <= ux1; x++) {
             u[IDX(U, x, y, z0 - 1)] = NEG(u[IDX(U, x, y, z0)]);
             u[IDX(U, x, y, z1)] = NEG(u[IDX(U, x, y, z1 - 1)]);
@@ -750,7 +764,11 @@ int orc_step(orc_engine *e)
     orc_project(e);
     for (int k = 0; k < 3; k++) {
         orc_apply_velocity_bc(e);
+#if ORC_MUTANT == 11
+        e->cur_axis = fwd[k];
+#else
         e->cur_axis = rev[k];
+#endif
         if (dispatch(e, OP_SWEEP)) return 1;
         orc_commit_velocity(e);
     }
@@ -772,7 +790,11 @@ double orc_max_divergence(orc_engine *e)
                                    (v[IDX(V, x, y + 1, z)] - v[IDX(V, x, y, z)]) +
                                    (w[IDX(W, x, y, z + 1)] - w[IDX(W, x, y, z)]));
                 double a = fabs(d);
+#if ORC_MUTANT == 12
+                if (a > mx || isnan(a)) mx = a;
+#else
                 if (a > mx) mx = a;
+#endif
             }
     return mx;
 }
@@ -787,7 +809,11 @@ double orc_kinetic_energy(orc_engine *e)
     for (int64_t z = g; z < g + e->NZ; z++)
         for (int64_t y = g; y < g + e->NY; y++)
             for (int64_t x = g; x < g + e->NX; x++) {
+#if ORC_MUTANT == 9
+                double uc = 0.5 * (u[IDX(U, x - 1, y, z)] + u[IDX(U, x, y, z)]);
+#else
                 double uc = 0.5 * (u[IDX(U, x, y, z)] + u[IDX(U, x + 1, y, z)]);
+#endif
                 double vc = 0.5 * (v[IDX(V, x, y, z)] + v[IDX(V, x, y + 1, z)]);
                 double wc = 0.5 * (w[IDX(W, x, y, z)] + w[IDX(W, x, y, z + 1)]);
                 sum += uc * uc + vc * vc + wc * wc;
@@ -802,7 +828,11 @@ double orc_max_velocity(orc_engine *e)
     static const int ids[3] = {F_U, F_V, F_W};
     for (int k = 0; k < 3; k++) {
         const double *d = DATA(e, ids[k]);
+#if ORC_MUTANT == 10
+        for (int64_t i = e->f[ids[k]].sz; i < e->f[ids[k]].count - e->f[ids[k]].sz; i++) {
+#else
         for (int64_t i = 0; i < e->f[ids[k]].count; i++) {
+#endif
             double a = fabs(d[i]);
             if (a > mx) mx = a;
         }
@@ -811,4 +841,11 @@ double orc_max_velocity(orc_engine *e)
 }
 
 /* CFL, stencil.go:336-338 */
-double orc_cfl(orc_engine *e) { return orc_max_velocity(e) * e->cfg.DT / e->cfg.H; }
+double orc_cfl(orc_engine *e)
+{
+#if ORC_MUTANT == 14
+    return orc_max_velocity(e) * (e->cfg.DT * 0.5) / e->cfg.H;
+#else
+    return orc_max_velocity(e) * e->cfg.DT / e->cfg.H;
+#endif
+}

@@ -106,10 +106,23 @@ def test_new_without_gpu_raises_cleanly(lib):
     assert ei.value.code == af.AF_ERR_HIP
 
 
-def test_sparse_solver_is_rejected(lib):
-    # sparse_fallback.go:17-19 -- checked before any device work
-    with pytest.raises(af.AeroFluxError, match="sparse direct solver"):
+def test_sparse_solver_flag_is_accepted(lib):
+    """New ignores UseSparseSolver (aeroflux.go:176-213; project.go:19 only
+    takes the direct solve once EnableSparseSolver succeeded, which it never
+    does off Darwin, sparse_fallback.go:17-24): the flag passes
+    normalisation and New gets as far as device work, failing on a host
+    without a GPU with the same error as without the flag."""
+    import ctypes
+    c = af.Config(NX=8, NY=8, NZ=8, UseSparseSolver=True)
+    c.Normalize()
+    assert c.UseSparseSolver
+    cnt = ctypes.c_int(0)
+    hip = ctypes.CDLL("libamdhip64.so")
+    if hip.hipGetDeviceCount(ctypes.byref(cnt)) == 0 and cnt.value > 0:
+        pytest.skip("a HIP device is present (the GPU test covers stepping)")
+    with pytest.raises(af.AeroFluxError) as ei:
         af.New(af.Config(NX=8, NY=8, NZ=8, UseSparseSolver=True))
+    assert ei.value.code == af.AF_ERR_HIP
 
 
 def test_snapshot_header_layout_matches_c(tmp_path):
