@@ -168,6 +168,8 @@ _SIGNATURES = [
     ("af_kinetic_energy", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("af_max_velocity", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("af_cfl", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("af_max_divergence_noalloc", C.c_int, [C.c_uint64] * 3 + [C.c_int64] * 4
+     + [C.c_double, C.c_uint64, C.c_uint64]),
     ("af_enable_sparse_solver", C.c_int, [C.c_void_p]),
     ("af_enable_direct_solver", C.c_int, [C.c_void_p]),
     ("af_enable_direct_solver_method", C.c_int, [C.c_void_p, C.c_int32]),

@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["af_kernels.hip", "af_advect.hip", "af_advect_lds.hip", "af_rbgs_tb.hip", "af_rbgs_split.hip", "af_mixed.hip",
            "af_engine.hip", "af_batch.hip", "af_ic.hip", "af_direct.hip",
-           "af_snapshot.hip", "af_measure.hip", "af_dct.hip"]
+           "af_snapshot.hip", "af_measure.hip", "af_dct.hip", "af_noalloc.hip"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

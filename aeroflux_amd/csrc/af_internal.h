@@ -65,6 +65,8 @@ hipError_t launch_set_int(int *p, int v, hipStream_t s);
 // Reductions: two launches, fixed tree, no atomics.  `partials` must hold
 // reduce_partials_needed(G) doubles; the result lands in *out (device).
 int reduce_partials_needed(Geom G);
+// Partials launch_max_divergence itself writes (<= reduce_partials_needed).
+int max_divergence_partials(Geom G);
 hipError_t launch_max_divergence(DField U, DField V, DField W, Geom G, double invH,
                                  double *partials, double *out, hipStream_t s);
 hipError_t launch_kinetic_energy(DField U, DField V, DField W, Geom G, double *partials,

@@ -347,6 +347,11 @@ int reduce_partials_needed(Geom G)
     return m > 1024 ? m : 1024;
 }
 
+int max_divergence_partials(Geom G)
+{
+    return ((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY);
+}
+
 hipError_t launch_max_divergence(DField U, DField V, DField W, Geom G, double invH,
                                  double *partials, double *out, hipStream_t s)
 {

@@ -169,6 +169,23 @@ int af_kinetic_energy(af_engine *e, double *out);   /* project.go:238-258 */
 int af_max_velocity(af_engine *e, double *out);     /* stencil.go:319-331 */
 int af_cfl(af_engine *e, double *out);              /* stencil.go:336-338 */
 
+/* af_max_divergence without any allocation, for callers that cannot
+ * allocate device or pinned memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  No counterpart in the
+ * reference.  u, v, w are the device addresses of element (0,0,0) of
+ * fields laid out as NewMAC lays out U, V and W for this grid (for an
+ * engine: af_field_device_ptr); the result, the same bits as
+ * af_max_divergence on those fields, is written to the device double at
+ * `out` when `stream` (a hipStream_t as an integer, 0 = the null stream, of
+ * the current device) reaches it: the call only enqueues two kernels and
+ * does not synchronise.  Reduction partials live in a fixed 64 KiB device
+ * array per GPU, held until the process ends, so grids need
+ * ceil(NX/64) * ceil(NY/4) <= 8192 (AF_ERR_ARG otherwise).  Calls that
+ * overlap on one GPU share that array and race. */
+int af_max_divergence_noalloc(uint64_t u, uint64_t v, uint64_t w, int64_t nx, int64_t ny,
+                              int64_t nz, int64_t ghost, double h, uint64_t out,
+                              uint64_t stream);
+
 /* EnableSparseSolver, sparse_fallback.go:17-19: always fails here, with the
  * reference's non-Darwin behaviour. */
 int af_enable_sparse_solver(af_engine *e);
