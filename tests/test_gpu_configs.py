@@ -1,0 +1,193 @@
+"""The full-size workloads on the GPU, at their real shapes, through the C ABI.
+
+* 512^3 band-limited (H=1/512, DT=0.0005, nu=1e-5): the plain FP64 step bit
+  for bit against the oracle, then FP32 smoothing inside FP64 defect
+  correction iterated to the FP64 path's final max|div|, with every output
+  field's error against the FP64 path bounded and the final max|div| at or
+  below the FP64 path's.
+* 96 uneven engines (each axis from {32,48,64,96,128}, nu log-uniform in
+  [1e-6,1e-4], DT = min(0.004, 0.5 H^2/(6 nu)), Taylor-Green + noise): the
+  FP64 batch equals every engine stepped alone bit for bit; the batch
+  iterated to each engine's FP64 final max|div| ends at or below it.
+* The served stream (axis sizes from {32,64,96,128,192,256}, Poisson
+  arrivals, band-limited initial velocity, mixed precision), its first 40
+  jobs with steps capped at 3: every job's result equals the job run alone.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import aeroflux_amd as afm
+from aeroflux_amd import ic, workload
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("P", "U", "V", "W", "Div", "Tmp0", "Tmp1", "Tmp2")
+
+
+def record(name, obj):
+    """Keep the measured errors beside the test log when a results folder
+    is named (AF_TEST_RECORD)."""
+    d = os.environ.get("AF_TEST_RECORD")
+    if d:
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, name + ".json"), "w") as fh:
+            json.dump(obj, fh, indent=1)
+    print(name, json.dumps(obj), flush=True)
+
+
+def same_bits(a, b):
+    return a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+def _run_512(af, cfg, steps, **extra):
+    e = af.New(af.Config(**cfg, **extra))
+    e.init_band_limited(5)
+    e.StepN(steps)
+    d = e.MaxDivergence()
+    fields = {k: e.Mesh.Data(e.Mesh.field(i)) for i, k in enumerate("PUVW")}
+    cycles = e.mixed_stats()[1] if extra.get("Precision") == afm.PRECISION_MIXED else None
+    e.Close()
+    return fields, d, cycles
+
+
+def _rel_err(a, ref):
+    return {k: float(np.abs(a[k] - ref[k]).max() / np.abs(ref[k]).max()) for k in "PUVW"}
+
+
+def test_config2_512cubed_fp64_bitwise_and_mixed_to_target(af, oracle_lib):
+    """512^3 band-limited.  FP64: the first step bit for bit against the
+    oracle.  Mixed, one cycle of the same 24 sweep pairs: differs from FP64
+    by FP32 rounding of the smoothing only (fields within 1e-5 of their
+    maximum, max|div| within 1e-5 relative).  Mixed iterated to the FP64
+    path's final max|div|: ends at or below it; its extra correction sweeps
+    (at most 3 cycles of 2 pairs per projection) move the fields away from
+    the FP64 path's unconverged 24-pair iterate by no more than the FP64 path
+    itself moves with 30 pairs (24 + 3 x 2)."""
+    n = 512
+    cfg = dict(NX=n, NY=n, NZ=n, H=1.0 / n, DT=0.0005, Viscosity=1e-5)
+    steps = 3
+    e = af.New(af.Config(**cfg))
+    e.init_band_limited(5)
+    # The oracle starts from the device's initial condition (the generator
+    # already applied the velocity BC), so both run the same inputs.
+    o = oracle_lib.OracleEngine(dict(cfg, Workers=16))
+    for fid in (1, 2, 3):
+        o.field(fid)[...] = e.Mesh.Data(e.Mesh.field(fid))
+    e.Step()
+    assert o.step() == 0
+    for fid in range(8):
+        assert same_bits(e.Mesh.Data(e.Mesh.field(fid)), o.field(fid)), f"512^3 step 0 {NAMES[fid]}"
+    assert e.MaxDivergence() == o.max_divergence()
+    o.close()
+    e.StepN(steps - 1)
+    d64 = e.MaxDivergence()
+    ref = {k: e.Mesh.Data(e.Mesh.field(i)) for i, k in enumerate("PUVW")}
+    e.Close()
+
+    one, d_one, _ = _run_512(af, cfg, steps, Precision=afm.PRECISION_MIXED)
+    err_one = _rel_err(one, ref)
+    del one
+    more, d_more, _ = _run_512(af, cfg, steps, GaussSeidelIters=30)
+    err_more = _rel_err(more, ref)
+    del more
+    tgt, d_tgt, cycles = _run_512(af, cfg, steps, Precision=afm.PRECISION_MIXED, MixedSweeps=24,
+                                  MixedFixSweeps=2, MixedMaxCycles=4, MixedTarget=d64)
+    err_tgt = _rel_err(tgt, ref)
+    del tgt
+    record("config2_512_mixed", {
+        "steps": steps, "fp64_max_div": d64,
+        "one_cycle": {"max_div": d_one, "max_rel_error_vs_fp64": err_one},
+        "fp64_30_pairs": {"max_div": d_more, "max_rel_error_vs_fp64_24": err_more},
+        "to_target": {"max_div": d_tgt, "cycles_last_solve": cycles,
+                      "max_rel_error_vs_fp64": err_tgt}})
+    assert abs(d_one - d64) <= 1e-5 * d64
+    for k in "PUVW":
+        assert err_one[k] <= 1e-5, (k, err_one)
+    assert d_tgt <= d64 * (1 + 1e-9)
+    assert cycles >= 1
+    for k in "PUVW":
+        assert err_tgt[k] <= err_more[k], (k, err_tgt, err_more)
+
+
+def _solo(af, cfg, f, steps):
+    e = af.New(af.Config(**cfg))
+    for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+        e.Mesh.SetData(e.Mesh.field(fid), f[name])
+    e.apply_velocity_bc()
+    e.StepN(steps)
+    out = [e.Mesh.Data(e.Mesh.field(k)) for k in range(8)]
+    d = e.MaxDivergence()
+    e.Close()
+    return out, d
+
+
+def _batch(af, engines, fields, prec, bextra, cfg_extra):
+    need = sum(af.SlotBytes(af.Config(**c, Precision=prec, **cfg_extra(i)), prec)
+               for i, c in enumerate(engines))
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=need + (64 << 20),
+                                MaxSlots=len(engines), **bextra))
+    slots = []
+    for i, (c, f) in enumerate(zip(engines, fields)):
+        s = b.Add(af.Config(**c, Precision=prec, **cfg_extra(i)))
+        for fid, name in ((1, "U"), (2, "V"), (3, "W")):
+            b.SetData(s, fid, f[name])
+        b.apply_velocity_bc(s)
+        slots.append(s)
+    return b, slots
+
+
+def test_config3_96_uneven_engines(af):
+    engines = workload.batch_engines(96, seed=3)
+    fields = [ic.taylor_green(c["NX"], c["NY"], c["NZ"], 1, c["H"], seed=100 + i)
+              for i, c in enumerate(engines)]
+    steps = 2
+    alone = [_solo(af, c, f, steps) for c, f in zip(engines, fields)]
+
+    b, slots = _batch(af, engines, fields, afm.PRECISION_FP64, {}, lambda i: {})
+    assert b.StepN(steps) == []
+    for i, s in enumerate(slots):
+        got = [b.Data(s, k) for k in range(8)]
+        for k in range(8):
+            assert same_bits(got[k], alone[i][0][k]), (i, engines[i], NAMES[k])
+        assert b.MaxDivergence(s) == alone[i][1]
+    b.Close()
+
+    d64 = [a[1] for a in alone]
+    b, slots = _batch(af, engines, fields, afm.PRECISION_MIXED,
+                      {"MixedChecked": 1, "MixedSweeps": 24, "MixedFixSweeps": 2,
+                       "MixedMaxCycles": 4},
+                      lambda i: {"MixedSweeps": 24, "MixedFixSweeps": 2, "MixedMaxCycles": 4,
+                                 "MixedTarget": float(d64[i])})
+    assert b.StepN(steps) == []
+    dm = np.array([b.MaxDivergence(s) for s in slots])
+    b.Close()
+    rel = (dm - np.array(d64)) / np.array(d64)
+    record("config3_96_mixed_to_target", {"engines": 96, "steps": steps,
+                                          "rel_diff_max": float(rel.max()),
+                                          "rel_diff_min": float(rel.min())})
+    assert np.all(dm <= np.array(d64) * (1 + 1e-9)), rel.max()
+
+
+def test_config4_served_stream_sizes_to_256(af):
+    jobs = workload.served_jobs(40, seed=7, rate=200.0)
+    jobs = [dict(j, Steps=min(j["Steps"], 3)) for j in jobs]
+    assert max(j["NX"] for j in jobs) == 256 and min(j["NX"] for j in jobs) == 32
+    prec = afm.PRECISION_MIXED
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=48 << 30, MaxSlots=64))
+    res, st = af.Serve(b, jobs, realtime=True)
+    assert st["CellUpdates"] == sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
+    assert st["MaxResident"] > 1
+    b.Close()
+    for j, r in zip(jobs, res):
+        assert r["Status"] == afm.AF_OK
+        e = af.New(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], H=j["H"], DT=j["DT"],
+                             Viscosity=j["Viscosity"], Precision=prec))
+        e.init_band_limited(j["Seed"])
+        e.StepN(j["Steps"])
+        d, ke = e.MaxDivergence(), e.KineticEnergy()
+        e.Close()
+        assert r["MaxDivergence"] == d, j
+        assert r["KineticEnergy"] == ke, j
