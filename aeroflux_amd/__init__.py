@@ -130,7 +130,7 @@ class _CServeJob(C.Structure):
     _fields_ = [
         ("NX", C.c_int64), ("NY", C.c_int64), ("NZ", C.c_int64), ("Steps", C.c_int64),
         ("Viscosity", C.c_double), ("DT", C.c_double), ("H", C.c_double),
-        ("Seed", C.c_uint64), ("Arrival", C.c_double),
+        ("Seed", C.c_uint64), ("Arrival", C.c_double), ("MixedTarget", C.c_double),
     ]
 
 
@@ -800,7 +800,8 @@ def Serve(batch: Batch, jobs: list[dict], kmax: int = 8, slope: float = -5.0 / 3
           progress_interval: float = 30.0) -> tuple[list[dict], dict]:
     """Runs step-N jobs through an empty batch with continuous batching
     (af_serve).  Each job is a dict with NX, NY, NZ, Steps, Viscosity, DT,
-    H, Seed, Arrival (seconds; sorted).  Returns per-job results (Admitted,
+    H, Seed, Arrival (seconds; sorted) and, for a checked mixed batch,
+    MixedTarget.  Returns per-job results (Admitted,
     Done, MaxDivergence, KineticEnergy, Status, Slot) and run statistics.
     progress(stats_so_far, jobs_done), if given, is called between batched
     steps at most once per progress_interval seconds."""

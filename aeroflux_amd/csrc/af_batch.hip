@@ -1591,8 +1591,13 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         c.MixedMaxCycles = b->cfg.MixedMaxCycles;
         c.MixedSweeps = b->cfg.MixedSweeps;
         c.MixedFixSweeps = b->cfg.MixedFixSweeps;
-        if (b->cfg.MixedChecked)
-            return fail(AF_ERR_ARG, "aeroflux: af_serve runs one-cycle mixed or FP64 batches");
+        if (b->cfg.MixedChecked) {
+            if (!(j.MixedTarget > 0))
+                return fail(AF_ERR_ARG,
+                            "aeroflux: job %ld needs MixedTarget > 0 in a checked mixed batch",
+                            (long)i);
+            c.MixedTarget = j.MixedTarget;
+        }
         if (j.Steps < 1) return fail(AF_ERR_ARG, "aeroflux: job %ld has no steps", (long)i);
         if (int rc = af_config_normalize(&c)) return rc;
         if (!slot_layout(c, c.Precision, &lays[i]))

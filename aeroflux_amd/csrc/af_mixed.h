@@ -15,7 +15,8 @@ struct MixedState {
     int *done = nullptr;                         // device: set once the target is met
     double *last_res = nullptr;                  // device: last max|div| implied by the residual
     int *cycles = nullptr;                       // device: cycles run since the last reset
-    int np = 0;
+    int np = 0;        // residual partials (one per 64 x 4 column)
+    int np_merge = 0;  // first-cycle merge + residual partials (column x z-chunk)
 };
 
 struct MixedCtx {

@@ -17,6 +17,7 @@
 // flag), which keeps the whole solve capturable in one graph.
 #include "af_bodies.h"
 #include "af_mixed.h"
+#include "af_xcd.h"
 
 #pragma clang fp contract(off)
 
@@ -42,6 +43,147 @@ __global__ __launch_bounds__(BX *BY) void k_residual(DField P, const double *__r
                                            blockIdx.y * BY + threadIdx.y);
     const double b = body::block_max256(m, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = b;
+}
+
+// The first cycle's close in one pass: P's interior takes the FP32
+// correction (p = e, exact widening), and the FP64 residual of that p,
+// r = (sum p~ - H^2 div) - 6 p with the Neumann mirror fused, is formed from
+// the same e values (a neighbour's p is its e), stored in FP32 on the split
+// layout for the next cycle, and folded into a per-workgroup max|r|.  The
+// values and their operation order are those of the merge followed by
+// k_residual, so the results are the same bits; the pass moves 24 B per cell
+// (e 4, Div 8, P 8, r 4) instead of the 32 B of the two, and it issues all
+// of a thread's loads before the first wait.  A workgroup owns a 64 x 4
+// column of (x, y) and ZC planes of it; the z-neighbours of a plane are the
+// column's own values of the planes beside it.  (Walking whole columns
+// through z, as k_residual does, keeps too few loads in flight: measured
+// 189 us against 115 us at 256^3.)
+constexpr int ZC = 4;
+
+__global__ __launch_bounds__(BX *BY) void k_merge_residual(DField P, const double *__restrict__ div,
+                                                           const float *__restrict__ e_red,
+                                                           const float *__restrict__ e_black,
+                                                           float *r_red, float *r_black, Geom G,
+                                                           double h2, double *partials, int3 nb)
+{
+    __shared__ double sh[BY];
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, z0 = bz * ZC;
+    double m = 0;
+    if (xi < G.NX && yi < G.NY) {
+        const int g = G.g, NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2;
+        // The colour array is chosen per lane by address, not by a branch:
+        // neighbouring lanes alternate colours, and a branch would split
+        // every load in two dependent halves.
+        const auto e_at = [&](int x, int y, int z) -> double {
+            const int i = (z * NY + y) * NXH + (x >> 1);
+            const float *base = body::parity(x, y, z, g) ? e_black : e_red;
+            return (double)body::gp(base)[i];
+        };
+        const bool xlo = xi == 0, xhi = xi == NX - 1, ylo = yi == 0, yhi = yi == NY - 1;
+        // Phase 1: every load is unconditional (an outside neighbour reads an
+        // in-range stand-in that the mirror then replaces), so all of them
+        // are in flight together.  Own values of planes z0-1 .. z0+ZC are
+        // clamped into the grid; a clamped plane is only read where the
+        // mirror replaces it, or not stored.
+        const int xm1 = xlo ? xi : xi - 1, xp1 = xhi ? xi : xi + 1;
+        const int ym1 = ylo ? yi : yi - 1, yp1 = yhi ? yi : yi + 1;
+        double own[ZC + 2], xm[ZC], xp[ZC], ym[ZC], yp[ZC], dv[ZC];
+#pragma unroll
+        for (int r = 0; r < ZC + 2; r++) {
+            int z = z0 - 1 + r;
+            z = z < 0 ? 0 : (z >= NZ ? NZ - 1 : z);
+            own[r] = e_at(xi, yi, z);
+        }
+        const auto d = body::gp(div);
+#pragma unroll
+        for (int r = 0; r < ZC; r++) {
+            const int z = (z0 + r) < NZ ? z0 + r : NZ - 1;
+            xm[r] = e_at(xm1, yi, z);
+            xp[r] = e_at(xp1, yi, z);
+            ym[r] = e_at(xi, ym1, z);
+            yp[r] = e_at(xi, yp1, z);
+            dv[r] = d[(xi + g) + (yi + g) * P.sy + (z + g) * P.sz];
+        }
+        // Phase 2: results in registers.
+        double res[ZC];
+#pragma unroll
+        for (int r = 0; r < ZC; r++) {
+            const int zi = z0 + r;
+            const double c = own[r + 1];
+            const double s = (((((xlo ? c : xm[r]) + (xhi ? c : xp[r])) + (ylo ? c : ym[r])) +
+                               (yhi ? c : yp[r])) +
+                              (zi == 0 ? c : own[r])) +
+                             (zi >= NZ - 1 ? c : own[r + 2]);
+            res[r] = (s - h2 * dv[r]) - 6.0 * c;
+            const double a = fabs(res[r]);
+            m = (zi < NZ && a > m) ? a : m;
+        }
+#pragma unroll
+        for (int r = 0; r < ZC; r++) asm volatile("" : "+v"(res[r]));
+        // Phase 3: stores.
+        const auto p = body::gp(P.d);
+#pragma unroll
+        for (int r = 0; r < ZC; r++) {
+            const int zi = z0 + r;
+            if (zi < NZ) {
+                p[(xi + g) + (yi + g) * P.sy + (zi + g) * P.sz] = own[r + 1];
+                const int si = (zi * NY + yi) * NXH + (xi >> 1);
+                float *rbase = body::parity(xi, yi, zi, g) ? r_black : r_red;
+                body::gp(rbase)[si] = (float)res[r];
+            }
+        }
+    }
+    // Max over the workgroup: across each 64-lane wave by shuffles, then
+    // over the four waves (max is exact in any order).
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if (threadIdx.x == 0) sh[threadIdx.y] = m;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double b = sh[0];
+        for (int w = 1; w < BY; w++) b = sh[w] > b ? sh[w] : b;
+        partials[(bz * nb.y + by) * nb.x + bx] = b;
+    }
+}
+
+// k_res_final over many partials: 1024 lanes each fold a fixed stride of
+// them (loads issued in batches of 8), then a fixed tree.
+constexpr int RF = 1024;
+
+__global__ __launch_bounds__(RF) void k_res_final_wide(const double *partials, int n,
+                                                       double res_to_div, double target, int *done,
+                                                       double *last, int *cycles)
+{
+    __shared__ double sh[RF];
+    const int t = threadIdx.x;
+    double v = 0;
+    for (int k0 = t; k0 < n; k0 += 8 * RF) {
+        double q[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + j * RF;
+            q[j] = k < n ? partials[k] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) v = q[j] > v ? q[j] : v;
+    }
+    sh[t] = v;
+    __syncthreads();
+    for (int s = RF / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] = sh[t + s] > sh[t] ? sh[t + s] : sh[t];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double d = res_to_div * sh[0];
+        *last = d;
+        *cycles = *cycles + 1;
+        if (d <= target) *done = 1;
+    }
 }
 
 // Folds the partials in a fixed order, converts to max|div| and raises
@@ -102,13 +244,15 @@ hipError_t init_mixed(MixedState &m, Geom G, hipStream_t s)
 {
     const size_t n = (size_t)split_count(G);
     m.np = ((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY);
+    m.np_merge = m.np * ((G.NZ + ZC - 1) / ZC);
     hipError_t err;
     float **bufs[4] = {&m.e_red, &m.e_black, &m.r_red, &m.r_black};
     for (float **b : bufs) {
         if ((err = hipMalloc(b, n * sizeof(float))) != hipSuccess) return err;
         if ((err = hipMemsetAsync(*b, 0, n * sizeof(float), s)) != hipSuccess) return err;
     }
-    if ((err = hipMalloc(&m.partials, (size_t)m.np * sizeof(double))) != hipSuccess) return err;
+    if ((err = hipMalloc(&m.partials, (size_t)m.np_merge * sizeof(double))) != hipSuccess)
+        return err;
     if ((err = hipMalloc(&m.done, 2 * sizeof(int))) != hipSuccess) return err;
     if ((err = hipMalloc(&m.last_res, sizeof(double))) != hipSuccess) return err;
     m.cycles = m.done + 1;
@@ -142,16 +286,19 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
             hipSuccess)
         return err;
     if ((err = smooth(m, c, 2 * c.sweeps, false, s)) != hipSuccess) return err;
-    if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, false,
-                                        nullptr, s)) != hipSuccess)
-        return err;
     if (!check) {
+        if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, false,
+                                            nullptr, s)) != hipSuccess)
+            return err;
         hipLaunchKernelGGL(k_mixed_count, dim3(1), dim3(1), 0, s, m.cycles);
     } else {
-        hipLaunchKernelGGL(k_residual, gcol, blk, 0, s, c.P, c.D.d, m.r_red, m.r_black, G, c.h2,
-                           m.partials, m.done, 0);
-        hipLaunchKernelGGL(k_res_final, dim3(1), dim3(RB), 0, s, m.partials, m.np, c.res_to_div,
-                           c.target, m.done, m.last_res, m.cycles, 0);
+        const int3 nb = make_int3((G.NX + BX - 1) / BX, (G.NY + BY - 1) / BY,
+                                  (G.NZ + ZC - 1) / ZC);
+        hipLaunchKernelGGL(k_merge_residual, dim3((unsigned)(nb.x * nb.y * nb.z)), blk, 0, s, c.P,
+                           c.D.d, m.e_red, m.e_black, m.r_red, m.r_black, G, c.h2, m.partials,
+                           nb);
+        hipLaunchKernelGGL(k_res_final_wide, dim3(1), dim3(RF), 0, s, m.partials, m.np_merge,
+                           c.res_to_div, c.target, m.done, m.last_res, m.cycles);
         for (int cyc = 1; cyc < c.cycles; cyc++) {
             if ((err = smooth(m, c, 2 * c.fix_sweeps, true, s)) != hipSuccess) return err;
             if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, true,

@@ -58,6 +58,28 @@ __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int 
                                    by * BY + threadIdx.y, bz);
 }
 
+// Gated form (the mixed solve's later correction cycles, which are skipped
+// once the target is met): a fixed grid of workgroups strides over the
+// tiles, so a skipped launch dispatches a few thousand workgroups that exit
+// at once instead of one per tile.
+template <typename TS, bool ADD>
+__global__ __launch_bounds__(BX *BY) void k_from_split_gated(double *nat, int sy, int sz, int g,
+                                                             int NX, int NY, int NXH,
+                                                             const TS *red, const TS *black,
+                                                             const int *done, int3 nb)
+{
+    if (*done) return;
+    const int total = nb.x * nb.y * nb.z;
+    for (int l = blockIdx.x; l < total; l += gridDim.x) {
+        const int bx = l % nb.x, r = l / nb.x;
+        const int by = r % nb.y, bz = r / nb.y;
+        body::from_split_cell<TS, ADD>(nat, sy, sz, g, NX, NY, NXH, red, black,
+                                       bx * BX + threadIdx.x, by * BY + threadIdx.y, bz);
+    }
+}
+
+constexpr int kGatedGrid = 2048;
+
 __global__ __launch_bounds__(BX *BY) void k_rhs32_split(const double *div, int sy, int sz, int g,
                                                         int NX, int NY, int NXH, double h2,
                                                         float *red, float *black, int3 nb)
@@ -105,7 +127,16 @@ hipError_t launch_from_split(double *nat, int sy, int sz, Geom G, const TS *red,
 {
     const int NXH = (G.NX + 1) / 2;
     const int3 nb = nblocks(G.NX, G.NY, G.NZ);
-    if (add)
+    const int total = nb.x * nb.y * nb.z;
+    if (done) {
+        const dim3 grid((unsigned)(total < kGatedGrid ? total : kGatedGrid));
+        if (add)
+            hipLaunchKernelGGL((k_from_split_gated<TS, true>), grid, dim3(BX, BY), 0, s, nat, sy,
+                               sz, G.g, G.NX, G.NY, NXH, red, black, done, nb);
+        else
+            hipLaunchKernelGGL((k_from_split_gated<TS, false>), grid, dim3(BX, BY), 0, s, nat, sy,
+                               sz, G.g, G.NX, G.NY, NXH, red, black, done, nb);
+    } else if (add)
         hipLaunchKernelGGL((k_from_split<TS, true>), grid1(nb), dim3(BX, BY), 0, s, nat, sy, sz,
                            G.g, G.NX, G.NY, NXH, red, black, done, nb);
     else

@@ -7,16 +7,19 @@ sweep pairs per projection.  W warm-up Step()s, then K timed Step()s
 bracketed by device synchronisation.  Prints ONE JSON line:
 
 * value: cell-updates/s = NX*NY*NZ*K / wall seconds (bench_test.go:32);
-* hbm_gbps_effective: algorithmic bytes (288 + 2*(88 + 48*I) per cell per
-  step, I = sweep pairs) / wall seconds;
-* roofline: the dominant kernel (the red-black colour pass, 24 B/cell of
-  algorithmic traffic: read P, read Div, write P) timed with HIP events on
-  the engine's stream;
+* step_bandwidth: the schedule's own algorithmic bytes per cell per step
+  (schedule_bytes) times the rate, and its fraction of the 8 TB/s peak;
+  reference_layout_model: the same rate under the reference layout's byte
+  model, an FP64-equivalent figure only;
+* roofline: the dominant kernel (the red-black colour pass on the
+  colour-split layout, 12 B/cell: other colour, divergence, own colour)
+  timed with HIP events on the engine's stream;
 * cpu_baseline: the C restatement of the Go reference timed on this host
   on a bounded sample of the same workload;
-* mixed: the same workload with FP32 red-black smoothing inside FP64 defect
-  correction, iterated to the FP64 run's final max divergence, with its
-  speed-up and per-field error against the FP64 run.
+* mixed / mixed_to_target: the same workload with FP32 red-black smoothing
+  inside FP64 defect correction, one cycle, and iterated to the FP64 run's
+  final max divergence, with speed-up and per-field error against FP64;
+* fp64_64, mixed_512, batched, served, direct_solver: the other workloads.
 """
 from __future__ import annotations
 
@@ -35,22 +38,51 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 
 
-def bytes_per_cell_step(iters: int) -> int:
-    """Algorithmic FP64 bytes per interior cell per Step: 6 sweeps x (3 reads
-    + 3 writes) x 8 B, plus per projection divergence 32 B, 2*iters colour
-    passes of 24 B, gradient 56 B."""
+def reference_layout_bytes(iters: int) -> int:
+    """Algorithmic FP64 bytes per interior cell per Step on the reference's
+    interleaved arena: 6 sweeps x (3 reads + 3 writes) x 8 B, plus per
+    projection divergence 32 B, 2*iters colour passes of 24 B (interleaved
+    colours move whole lines), gradient 56 B.  A model of the reference's
+    layout, not of what this build moves (its colour-split passes move half
+    of the 24 B), so its rate is an FP64-equivalent figure, not traffic."""
     return 288 + 2 * (88 + 48 * iters)
 
 
-def model_gbps(value: float, bytes_per_cell: int = 0) -> dict:
-    """Effective bandwidth of a cell-updates/s rate under the FP64 bytes model
-    (one read of each input, one write of each output per reference stage;
-    24 sweep pairs unless bytes_per_cell is given), and its fraction of the
-    8 TB/s HBM peak.  For mixed-precision runs it is the FP64-equivalent
-    rate, not traffic."""
-    b = bytes_per_cell or bytes_per_cell_step(24)
-    g = value * b / 1e9
-    return {"model_bytes_per_cell_step": b, "model_gbps": g, "model_roofline_frac": g / HBM_PEAK_GBPS}
+def schedule_bytes(kind: str, iters: int = 24) -> int:
+    """Algorithmic bytes per interior cell per Step of the schedule this build
+    runs (one read of each input and one write of each output per kernel;
+    O(N^2) ghost fills not counted):
+
+    * advection: 6 sweeps x (read U, V, W + write Tmp0..2) = 288;
+    * fp64_split, per projection: divergence 40 (U, V, W 24, Div 8, split
+      right-hand side 8), first pass from zero 8, second pass 12, 2*iters-2
+      passes of 12, merge into P 16, gradient 56 = 108 + 24*iters;
+    * fp64_arena (small grids): divergence 32, first pass 16, 2*iters-1
+      passes of 24, gradient 56 = 80 + 48*iters;
+    * mixed (one FP32 cycle), per projection: divergence 36 (FP32 rhs 4),
+      first pass 4, second 6, 2*iters-2 passes of 6, merge 12, gradient 56
+      = 102 + 12*iters;
+    * mixed_checked: the same with the first cycle closed by the fused merge
+      and FP64 residual (24 instead of 12); later cycles, which run only when
+      the check asks for them, are not counted (a lower bound);
+    * direct (FFT passes): divergence folded into the first pass, five
+      passes 80, gradient 56, per projection 32 + 80 + 56."""
+    proj = {"fp64_split": 108 + 24 * iters, "fp64_arena": 80 + 48 * iters,
+            "mixed": 102 + 12 * iters, "mixed_checked": 114 + 12 * iters,
+            "direct": 32 + 80 + 56}[kind]
+    return 288 + 2 * proj
+
+
+def bandwidth(value: float, bytes_per_cell: int, kind: str) -> dict:
+    """Step-level bandwidth of a cell-updates/s rate under the schedule's
+    own algorithmic bytes, and its fraction of the 8 TB/s HBM peak."""
+    g = value * bytes_per_cell / 1e9
+    return {"step_bandwidth": {"schedule": kind, "bytes_per_cell_step": bytes_per_cell,
+                               "gbps": g, "frac_of_peak": g / HBM_PEAK_GBPS}}
+
+
+def fp64_kind(e) -> str:
+    return "fp64_split" if e.solver_tile()[0] == 0 else "fp64_arena"
 
 
 def hip_sync() -> None:
@@ -220,7 +252,12 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int) -> dict:
             "unit": "cell-updates/s",
             "ms_per_step": 1e3 * el / steps,
             "speedup_vs_plain_fp64": cells * steps / el / plain,
-            "hbm_gbps_model": bytes_per_cell_step(24) * cells * steps / el / 1e9,
+            **bandwidth(cells * steps / el,
+                        schedule_bytes("fp64_split" if prec == af.PRECISION_FP64 else
+                                       ("mixed_checked" if bc_extra.get("MixedChecked") else
+                                        "mixed")),
+                        "fp64_split" if prec == af.PRECISION_FP64 else
+                        ("mixed_checked" if bc_extra.get("MixedChecked") else "mixed")),
             "pass_roofline": {"bound": "hbm", "achieved": pass_bytes / (t_pass * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": pass_bytes / (t_pass * 1e-3) / 1e9 / HBM_PEAK_GBPS,
@@ -273,7 +310,14 @@ def big_leg(af, n: int, steps: int) -> dict:
     cfg = dict(NX=n, NY=n, NZ=n, H=1.0 / n, DT=0.0005, Viscosity=1e-5)
     out = {"grid": [n, n, n], "steps": steps, "initial_condition": "band-limited seed 5"}
     fields = {}
-    for name, extra in (("fp64", {}), ("mixed", {"Precision": af.PRECISION_MIXED})):
+    runs = [("fp64", {}), ("mixed", {"Precision": af.PRECISION_MIXED}), ("mixed_to_target", None)]
+    for name, extra in runs:
+        if extra is None:
+            # Iterated to the FP64 run's final max|div|: 24 FP32 sweep pairs
+            # closed by the FP64 residual check, then at most one more cycle.
+            extra = {"Precision": af.PRECISION_MIXED, "MixedSweeps": 24, "MixedFixSweeps": 1,
+                     "MixedMaxCycles": 2,
+                     "MixedTarget": out["fp64"]["final_max_divergence"]}
         e = af.New(af.Config(**cfg, **extra))
         e.init_band_limited(5)
         e.StepN(1)
@@ -285,19 +329,29 @@ def big_leg(af, n: int, steps: int) -> dict:
         out[name] = {"value": n ** 3 * (steps - 1) / el, "unit": "cell-updates/s",
                      "ms_per_step": 1e3 * el / (steps - 1),
                      "final_max_divergence": e.MaxDivergence()}
-        out[name].update(model_gbps(out[name]["value"]))
+        kind = {"fp64": "fp64_split", "mixed": "mixed"}.get(name, "mixed_checked")
+        out[name].update(bandwidth(out[name]["value"], schedule_bytes(kind), kind))
+        if name == "mixed_to_target":
+            out[name]["cycles_last_solve"] = e.mixed_stats()[1]
         fields[name] = {k: e.Mesh.Data(e.Mesh.field(i)) for i, k in enumerate("PUVW")}
         e.Close()
-    err = {}
-    for k in "PUVW":
-        a, r = fields["mixed"][k], fields["fp64"][k]
-        d = a - r
-        err[k] = {"max_rel": float(np.abs(d).max() / np.abs(r).max()),
-                  "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(r * r)))}
-    out["mixed"]["error_vs_fp64"] = err
-    out["mixed"]["speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
-    d64, dmx = out["fp64"]["final_max_divergence"], out["mixed"]["final_max_divergence"]
-    out["mixed"]["final_max_divergence_rel_diff"] = (dmx - d64) / d64
+        if name == "fp64":
+            continue
+        err = {}
+        for k in "PUVW":
+            a, r = fields[name][k], fields["fp64"][k]
+            d = a - r
+            err[k] = {"max_rel": float(np.abs(d).max() / np.abs(r).max()),
+                      "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(r * r)))}
+        del fields[name]
+        out[name]["error_vs_fp64"] = err
+        out[name]["speedup_vs_fp64"] = out[name]["value"] / out["fp64"]["value"]
+        d64, dmx = out["fp64"]["final_max_divergence"], out[name]["final_max_divergence"]
+        out[name]["final_max_divergence_rel_diff"] = (dmx - d64) / d64
+    out["mixed_to_target"]["error_note"] = (
+        "the correction cycles carry the iterate past the FP64 path's 24 unconverged sweep "
+        "pairs, so these errors are that path's own remainder, not FP32 rounding (the one-cycle "
+        "run's errors are the rounding)")
     # The same run with the direct pressure solve (FFT passes at 512^3): an
     # exact projection, so its fields differ from the 24-sweep solve's by the
     # iteration's error; only rates and the final max|div| are compared.
@@ -323,20 +377,22 @@ def big_leg(af, n: int, steps: int) -> dict:
 
 
 def small_leg(af, ic, n: int, steps: int, workers: int, cpu: bool) -> dict:
-    """The 64^3 FP64 workload (H=1/64, DT=0.001, nu=1e-5, seeded Taylor-Green
+    """The 64^3 FP64 workload (H=1/64, DT=0.004, nu=1e-5, seeded Taylor-Green
     + noise), 100 steps on the GPU and, unless skipped, the same 100 steps
-    on the C oracle (16-thread Z-slab pool): both rates, and whether all
-    eight fields end bit-identical.  The 19 MB arena stays in the last-level
-    cache, so its bandwidth is not an HBM figure."""
+    on the C oracle (Z-slab pool of `workers` threads): both rates, and
+    whether all eight fields end bit-identical.  The 19 MB arena stays in the
+    last-level cache, so its bandwidth is not an HBM figure."""
     h = 1.0 / n
-    cfg = dict(NX=n, NY=n, NZ=n, H=h, DT=0.001, Viscosity=1e-5)
+    cfg = dict(NX=n, NY=n, NZ=n, H=h, DT=0.004, Viscosity=1e-5)
     f = ic.taylor_green(n, n, n, 1, h, seed=1)
     e, el = run_gpu(af, cfg, f, 0, steps)
     gpu = [e.Mesh.Data(e.Mesh.field(k)) for k in range(8)]
+    kind = fp64_kind(e)
     e.Close()
     value = n ** 3 * steps / el
     out = {"grid": [n, n, n], "steps": steps, "value": value, "unit": "cell-updates/s",
-           "ms_per_step": 1e3 * el / steps, "cache_resident": True, **model_gbps(value)}
+           "ms_per_step": 1e3 * el / steps, "cache_resident": True,
+           **bandwidth(value, schedule_bytes(kind), kind)}
     if cpu:
         from oracle import oracle
         o = oracle.OracleEngine(dict(cfg, Workers=workers))
@@ -377,13 +433,16 @@ def serve_summary(jobs, res, st) -> dict:
         "max_resident_jobs": st["MaxResident"],
         "peak_pool_bytes": st["PeakBytes"],
         "preemptions": st["Preemptions"],
-        **model_gbps(st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0),
+        **bandwidth(st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0,
+                    schedule_bytes("mixed"), "mixed"),
     }
 
 
-def serve_once(af, jobs, budget: float, policy: int, label: str):
+def serve_once(af, jobs, budget: float, policy: int, label: str, checked: bool = False):
+    extra = {"MixedChecked": 1, "MixedSweeps": 24, "MixedFixSweeps": 2,
+             "MixedMaxCycles": 4} if checked else {}
     b = af.Batch(af.BatchConfig(Precision=af.PRECISION_MIXED, MemoryBudget=int(budget),
-                                MaxSlots=4096, ServePolicy=policy))
+                                MaxSlots=4096, ServePolicy=policy, **extra))
 
     def report(so_far, done):
         print(f"served ({label}): {so_far['Wall']:.0f} s, {done}/{len(jobs)} jobs done, "
@@ -401,11 +460,14 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
     `rate`/s, axis sizes from {32,64,96,128,192,256}, N uniform in [10,200],
     nu log-uniform in [1e-6,1e-4], DT = min(0.004, 0.5 H^2/(6 nu)), seeded
     band-limited initial velocity (|k| <= 8, |k|^-5/3, RMS 1) built on the
-    device at admission, one-cycle mixed-precision projection, arenas
-    admitted first come first served under the HBM budget.  The first
-    n_plain jobs are also run on the plain FP64 path, each alone (New,
-    init_band_limited, StepN; StepN timed), for its cell-updates/s and each
-    job's final max|div|.  policies "both" runs the same stream again under
+    device at admission, mixed-precision projection (one cycle of the same
+    24 sweep pairs in FP32), arenas admitted first come first served under
+    the HBM budget.  The first n_plain jobs are also run on the plain FP64
+    path, each alone (New, init_band_limited, StepN; StepN timed), for its
+    cell-updates/s and each job's final max|div|, against which every served
+    job's is compared; then those n_plain jobs are served again in a checked
+    batch, each iterated to its own FP64 final max|div| (24 FP32 pairs closed
+    by the FP64 residual check, then checked cycles of 2 pairs, at most 4).  policies "both" runs the same stream again under
     least-remaining-work-first admission with preemption (snapshots to host
     memory) and compares its results with the first run's; "preempt" runs
     only that policy."""
@@ -418,7 +480,7 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
     res, st, info = serve_once(af, jobs, budget, first, label)
     plain = None
     if n_plain > 0:
-        el, cu, rel = 0.0, 0, []
+        el, cu, rel, d_alone = 0.0, 0, [], []
         for j, r in list(zip(jobs, res))[:n_plain]:
             e = af.New(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], H=j["H"], DT=j["DT"],
                                  Viscosity=j["Viscosity"]))
@@ -431,6 +493,7 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
             cu += j["NX"] * j["NY"] * j["NZ"] * j["Steps"]
             d = e.MaxDivergence()
             e.Close()
+            d_alone.append(d)
             if r["Status"] == 0:
                 rel.append((r["MaxDivergence"] - d) / d)
         rel = np.array(rel)
@@ -438,7 +501,29 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
                  "how": "each job alone through New/init_band_limited/StepN, one after the other",
                  "served_mixed_final_max_divergence_rel_diff": {
                      "max_abs": float(np.abs(rel).max()) if rel.size else 0.0,
-                     "mean": float(rel.mean()) if rel.size else 0.0}}
+                     "mean": float(rel.mean()) if rel.size else 0.0,
+                     "jobs_at_or_below_fp64": int((rel <= 0).sum()),
+                     "jobs_compared": int(rel.size)}}
+        # The same jobs served again, each iterated to its own FP64 final
+        # max|div| (a checked batch with per-job targets).
+        cjobs = [dict(j, MixedTarget=d) for j, d in zip(jobs[:n_plain], d_alone)]
+        cres, cst, _ = serve_once(af, cjobs, budget, af.SERVE_FCFS,
+                                  "checked, first come first served", checked=True)
+        crel = np.array([(r["MaxDivergence"] - j["MixedTarget"]) / j["MixedTarget"]
+                         for j, r in zip(cjobs, cres) if r["Status"] == 0])
+        plain["served_checked"] = {
+            "jobs": n_plain,
+            "schedule": "24 FP32 sweep pairs closed by the FP64 residual check, then checked "
+                        "cycles of 2 pairs (at most 4 cycles) until max|div| <= the job's final "
+                        "max|div| on the FP64 path",
+            **serve_summary(cjobs, cres, cst),
+            "speedup_vs_plain_fp64": (cst["CellUpdates"] / cst["Wall"] / plain["value"]
+                                      if cst["Wall"] > 0 else None),
+            "final_max_divergence_vs_fp64": {
+                "jobs_at_or_below_fp64": int((crel <= 1e-9).sum()),
+                "jobs_compared": int(crel.size),
+                "rel_diff_max": float(crel.max()) if crel.size else 0.0,
+                "rel_diff_mean": float(crel.mean()) if crel.size else 0.0}}
     demand = sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
     out = {
         "jobs": n_jobs,
@@ -446,6 +531,7 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
         "last_arrival_s": jobs[-1]["Arrival"] if jobs else 0.0,
         "offered_cell_updates_per_s": demand / jobs[-1]["Arrival"] if jobs else 0.0,
         "policy": label,
+        "schedule": "24 FP32 sweep pairs per projection in one FP64 defect-correction cycle",
         **serve_summary(jobs, res, st),
         "memory_budget_bytes": int(budget),
         "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
@@ -526,7 +612,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--cpu-workers", type=int, default=int(os.environ.get("AF_CPU_WORKERS", "16")))
+    ap.add_argument("--cpu-workers", type=int, default=int(os.environ.get("AF_CPU_WORKERS", "0")),
+                    help="oracle threads for the CPU baseline (0: the host cores this process "
+                         "may use, capped at 16, the CPU share of one GPU)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only run the copy-calibration + dominant-kernel launch sequence")
     ap.add_argument("--no-batched", action="store_true")
@@ -550,6 +638,8 @@ def main() -> int:
     ap.add_argument("--served-plain-jobs", type=int, default=60,
                     help="jobs of the stream also run alone on the plain FP64 path")
     args = ap.parse_args()
+    if args.cpu_workers <= 0:
+        args.cpu_workers = min(16, len(os.sched_getaffinity(0)))
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -576,7 +666,6 @@ def main() -> int:
     d64 = e.MaxDivergence()
     ms_per_step = 1e3 * el / args.steps
     value = cells * args.steps / el
-    eff_gbps = bytes_per_cell_step(iters) * cells * args.steps / el / 1e9
     # The final state, before the kernel timings below overwrite it.
     fp64_fields = None
     if not args.no_mixed:
@@ -604,6 +693,7 @@ def main() -> int:
     stream_4r3w = af.MeasureStream(af.STREAM_4R3W, stream_n, 10)
     tuned = {"sweep_shape": e.sweep_shape()[0] + 1, "sweep_slab_rows": e.sweep_slab(),
              "pass_slab_workgroup_rows": e.pass_slab()}
+    kind = fp64_kind(e)
     e.Close()
 
     out = {
@@ -626,8 +716,13 @@ def main() -> int:
             "gauss_seidel_iters": iters,
             "precision": "fp64",
         },
-        "hbm_gbps_effective": eff_gbps,
-        "roofline_frac_effective": eff_gbps / HBM_PEAK_GBPS,
+        **bandwidth(value, schedule_bytes(kind, iters), kind),
+        "reference_layout_model": {
+            "bytes_per_cell_step": reference_layout_bytes(iters),
+            "fp64_equivalent_gbps": value * reference_layout_bytes(iters) / 1e9,
+            "note": "the reference's interleaved-arena byte model; this build's colour-split "
+                    "passes move half its pass bytes, so this is not traffic and not a fraction "
+                    "of peak"},
         "final_max_divergence": d64,
         "roofline": {
             "kernel": rb_name,
@@ -677,13 +772,13 @@ def main() -> int:
             "final_max_divergence_rel_diff": (dmx - d64) / d64,
             "cycles_last_solve": cyc,
             "max_rel_error_vs_fp64": errs,
-            **model_gbps(mval),
+            **bandwidth(mval, schedule_bytes("mixed", iters), "mixed"),
         }
         # Iterated to the FP64 path's final max|div|: a short first cycle, then
         # correction cycles until the FP64 residual says the projection's
         # max|div| is at most that value.
         tcfg = dict(cfg, Precision=af.PRECISION_MIXED, MixedSweeps=args.target_sweeps,
-                    MixedFixSweeps=2, MixedMaxCycles=4, MixedTarget=d64)
+                    MixedFixSweeps=1, MixedMaxCycles=2, MixedTarget=d64)
         t, tel = run_gpu(af, tcfg, f, args.warmup, args.steps)
         dt_ = t.MaxDivergence()
         terrs = {}
@@ -701,12 +796,12 @@ def main() -> int:
             "speedup_vs_fp64": tval / value,
             "target_max_divergence": d64,
             "final_max_divergence": dt_,
-            "schedule": f"{args.target_sweeps} FP32 sweep pairs, then FP64-residual-checked "
-                        "cycles of 2 pairs (at most 4 cycles) until max|div| <= the FP64 run's "
-                        "final value",
+            "schedule": f"{args.target_sweeps} FP32 sweep pairs closed by the FP64 "
+                        "residual check, then at most one more cycle of 1 pair if max|div| is "
+                        "still above the FP64 run's final value",
             "cycles_last_solve": tcyc,
             "max_rel_error_vs_fp64": terrs,
-            **model_gbps(tval),
+            **bandwidth(tval, schedule_bytes("mixed_checked", iters), "mixed_checked"),
         }
     del fp64_fields
 
@@ -736,7 +831,7 @@ def main() -> int:
             "transforms": {af.DIRECT_FFT: "FFT passes", af.DIRECT_DENSE: "dense GEMMs"}[method],
             # 6 sweeps 288 B + per projection divergence 32, solve 80 (FFT
             # passes), gradient 56.
-            **model_gbps(dval, 288 + 2 * (32 + 80 + 56)),
+            **bandwidth(dval, schedule_bytes("direct"), "direct"),
             "final_max_divergence": ddiv,
             "fp64_final_max_divergence": d64,
             "solver": "pinned Neumann system solved exactly: orthonormal DCT-II along each "

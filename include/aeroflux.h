@@ -462,6 +462,9 @@ typedef struct af_serve_job {
     double H;                  /* 0 -> 1 / max(NX, NY, NZ) */
     uint64_t Seed;             /* initial condition */
     double Arrival;            /* seconds after the start of the run */
+    double MixedTarget;        /* checked mixed batches (MixedChecked = 1):
+                                  the job's max|div| target, > 0 (see
+                                  af_config.MixedTarget); otherwise 0 */
 } af_serve_job;
 
 typedef struct af_serve_result {

@@ -108,6 +108,32 @@ def test_preempted_jobs_resume_with_the_same_results(af, prec):
     b.Close()
 
 
+def test_checked_served_jobs_meet_their_targets(af):
+    """A checked mixed batch serves jobs that each carry a max|div| target
+    (here the job's own final max|div| on the FP64 path): every job ends at
+    or below it, and equals the same checked engine run alone."""
+    jobs = small_stream(10, 13, rate=300.0)
+    fix = dict(MixedSweeps=24, MixedFixSweeps=1, MixedMaxCycles=2)
+    for j in jobs:
+        j["MixedTarget"] = alone(af, afm.PRECISION_FP64, j)[0]
+    prec = afm.PRECISION_MIXED
+    b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=256 << 20, MaxSlots=16,
+                                MixedChecked=1, **fix))
+    res, st = af.Serve(b, jobs, realtime=True)
+    assert st["CellUpdates"] == sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
+    b.Close()
+    for j, r in zip(jobs, res):
+        assert r["Status"] == afm.AF_OK
+        assert r["MaxDivergence"] <= j["MixedTarget"] * (1 + 1e-9), j
+        e = af.New(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], H=j["H"], DT=j["DT"],
+                             Viscosity=j["Viscosity"], Precision=prec,
+                             MixedTarget=j["MixedTarget"], **fix))
+        e.init_band_limited(j["Seed"])
+        e.StepN(j["Steps"])
+        assert (r["MaxDivergence"], r["KineticEnergy"]) == (e.MaxDivergence(), e.KineticEnergy())
+        e.Close()
+
+
 def test_serve_progress_reports(af):
     """The progress callback runs between batched steps with the running
     statistics: steps and completed jobs never decrease and stay within the
@@ -154,4 +180,10 @@ def test_serve_rejects_bad_input(af):
     b.Remove(s)
     res, st = af.Serve(b, [])
     assert res == [] and st["Steps"] == 0
+    b.Close()
+    # a checked batch needs every job's target
+    b = af.Batch(af.BatchConfig(Precision=afm.PRECISION_MIXED, MemoryBudget=16 << 20,
+                                MixedChecked=1))
+    with pytest.raises(af.AeroFluxError, match="MixedTarget"):
+        af.Serve(b, jobs)
     b.Close()
