@@ -49,7 +49,7 @@ AXIS_X, AXIS_Y, AXIS_Z = range(3)
 COLOR_RED, COLOR_BLACK = range(2)
 PRECISION_FP64, PRECISION_MIXED = range(2)
 STREAM_READ, STREAM_COPY, STREAM_2R1W, STREAM_4R3W = range(4)
-SERVE_FCFS, SERVE_LEAST_WORK_PREEMPT = 0, 1
+SERVE_FCFS, SERVE_LEAST_WORK_PREEMPT, SERVE_LEAST_WORK_STEPS = 0, 1, 2
 DIRECT_AUTO, DIRECT_FFT, DIRECT_DENSE = 0, 1, 2
 
 
@@ -87,6 +87,7 @@ class _CBatchConfig(C.Structure):
         ("MixedSweeps", C.c_int32), ("MixedFixSweeps", C.c_int32), ("MixedChecked", C.c_int32),
         ("SweepShape", C.c_int32), ("Workgroups", C.c_int32), ("SweepStaged", C.c_int32),
         ("ServePolicy", C.c_int32), ("PreemptRatio", C.c_double), ("SnapshotBudget", C.c_int64),
+        ("StepCellBudget", C.c_int64),
     ]
 
 
@@ -613,6 +614,7 @@ class BatchConfig:
     ServePolicy: int = SERVE_FCFS
     PreemptRatio: float = 0.0
     SnapshotBudget: int = 0
+    StepCellBudget: int = 0
 
     def _to_c(self) -> _CBatchConfig:
         c = _CBatchConfig()

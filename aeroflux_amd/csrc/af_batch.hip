@@ -22,6 +22,7 @@
 #include <cmath>
 #include <deque>
 #include <thread>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -48,6 +49,8 @@ constexpr int PZ = 8;       // z-planes per thread of a colour pass
 constexpr int CHUNK = 256;  // cells per item of the shell stages
 constexpr int CZ = 4;       // z-planes per item of K_CELL (divergence, merge)
 constexpr int XCDS = 8;
+// Serving policy 2: interior cells one batched step advances by default.
+constexpr int64_t kDefaultStepCells = (int64_t)32 << 20;
 
 // Stage kinds: each has its own tile decomposition and prefix table.
 enum Kind {
@@ -583,6 +586,9 @@ struct af_batch {
     bool tuned = false;
     int nact = 0;
     int64_t cells = 0;
+    // Serving policy 2: slots the next batched steps advance (empty: every
+    // resident slot).  Resident slots outside it keep their state.
+    std::vector<char> step_mask;
     ICScratch ic;  // initial-condition generator scratch (lazy, outside the pool)
 };
 
@@ -638,8 +644,11 @@ int build_table(af_batch *b, int plan, int ring = 0)
     int *t = b->h_tab + (int64_t)ring * b->tab_ints;
     int *act = t + 2;
     int n = 0;
+    const bool masked = !b->step_mask.empty();
     for (int s = 0; s < (int)b->slots.size(); s++)
-        if (b->slots[s].used) act[n++] = s;
+        if (b->slots[s].used &&
+            (!masked || ((size_t)s < b->step_mask.size() && b->step_mask[(size_t)s])))
+            act[n++] = s;
     t[0] = n;
     t[1] = stride;
     for (int k = 0; k < K_COUNT; k++) {
@@ -1071,10 +1080,12 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
                     (int)c.SweepStaged, kLdsVariants);
     if (c.PreemptRatio == 0) c.PreemptRatio = 4.0;
     if (c.SnapshotBudget == 0) c.SnapshotBudget = (int64_t)32 << 30;
-    if (c.ServePolicy < 0 || c.ServePolicy > 1 || !(c.PreemptRatio >= 1.0) ||
-        std::isinf(c.PreemptRatio) || c.SnapshotBudget < 0)
+    if (c.StepCellBudget == 0) c.StepCellBudget = kDefaultStepCells;
+    if (c.ServePolicy < 0 || c.ServePolicy > 2 || !(c.PreemptRatio >= 1.0) ||
+        std::isinf(c.PreemptRatio) || c.SnapshotBudget < 0 || c.StepCellBudget < 1)
         return fail(AF_ERR_CONFIG,
-                    "aeroflux: serve policy must be 0 or 1, PreemptRatio >= 1, SnapshotBudget >= 0");
+                    "aeroflux: serve policy must be 0, 1 or 2, PreemptRatio >= 1, "
+                    "SnapshotBudget >= 0, StepCellBudget >= 1");
 
     af_batch *b = new (std::nothrow) af_batch();
     if (!b) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
@@ -1635,6 +1646,8 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
     // Waiting jobs, ordered by key: arrival index (first come first served)
     // or remaining work, cells x steps left, ties by arrival (policy 1).
     const bool by_work = b->cfg.ServePolicy == 1;
+    const bool by_steps = b->cfg.ServePolicy == 2;
+    std::vector<std::pair<double, int64_t>> order;  // policy 2: (work left, resident index)
     std::set<std::pair<double, int64_t>> queue;
     auto work_of = [&](int64_t j, int64_t left) {
         return (double)(cfgs[j].NX * cfgs[j].NY * cfgs[j].NZ) * (double)left;
@@ -1840,6 +1853,34 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
             }
             continue;
         }
+        // Policy 2: this step advances the resident jobs with the least
+        // remaining work (cells x steps left, ties by arrival), as many as
+        // fit the cell budget, at least one; a job that does not fit is
+        // passed over for smaller ones behind it.  Every job still takes
+        // exactly its own steps, so results do not depend on the choice.
+        if (by_steps) {
+            order.clear();
+            for (size_t r = 0; r < resident.size(); r++)
+                order.push_back({work_of(resident[r].job, resident[r].left), (int64_t)r});
+            std::sort(order.begin(), order.end(), [&](const auto &p, const auto &q) {
+                return p.first != q.first ? p.first < q.first
+                                          : resident[(size_t)p.second].job <
+                                                resident[(size_t)q.second].job;
+            });
+            std::vector<char> mask(b->slots.size(), 0);
+            int64_t cells = 0;
+            for (const auto &o : order) {
+                const Res &x = resident[(size_t)o.second];
+                const int64_t c = cfgs[x.job].NX * cfgs[x.job].NY * cfgs[x.job].NZ;
+                if (cells > 0 && cells + c > b->cfg.StepCellBudget) continue;
+                mask[(size_t)x.slot] = 1;
+                cells += c;
+            }
+            if (mask != b->step_mask) {
+                b->step_mask.swap(mask);
+                b->tab_dirty = true;
+            }
+        }
         // Tune once the resident set is representative: a working set that
         // fills the GPU, or everything the stream will admit.  Until then the
         // default shape and grids run.
@@ -1871,7 +1912,9 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         if ((err = hipEventRecord(ev[step % R], b->stream)) != hipSuccess) break;
         std::vector<int64_t> done_now;
         for (size_t r = 0; r < resident.size();)
-            if (--resident[r].left == 0) {
+            if (by_steps && !b->step_mask[(size_t)resident[r].slot]) {
+                r++;  // not advanced by this step
+            } else if (--resident[r].left == 0) {
                 done_now.push_back(resident[r].job);
                 std::swap(resident[r], resident.back());
                 resident.pop_back();
@@ -1894,6 +1937,10 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
     if (err == hipSuccess) err = hipStreamSynchronize(b->stream);
     for (int k = 0; k < R; k++)
         if (ev[k]) (void)hipEventDestroy(ev[k]);
+    if (!b->step_mask.empty()) {
+        b->step_mask.clear();  // later batched steps advance every slot again
+        b->tab_dirty = true;
+    }
     if (err == hipSuccess && rc == AF_OK) {
         for (int64_t j = 0; j < n; j++) {
             if (status[j] == AF_ERR_OOM) continue;

@@ -447,6 +447,20 @@ def serve_summary(jobs, res, st) -> dict:
     }
 
 
+# Serving policies: name -> (ServePolicy, description, output key).
+SERVE_POLICIES = {
+    "fcfs": (0, "first come first served; every resident job advances each batched step",
+             "first_come_first_served"),
+    "steps": (2, "admission first come first served; each batched step advances the resident "
+                 "jobs with the least remaining work (cells x steps left), up to 32 Mi "
+                 "interior cells", "least_work_stepping"),
+    "preempt": (1, "least remaining work (cells x steps left) admitted first; a resident job "
+                   "with more than 4x the work of a queued job that cannot be placed is "
+                   "snapshotted to host memory and requeued (at most once per job)",
+                "least_work_first_preempting"),
+}
+
+
 def serve_once(af, jobs, budget: float, policy: int, label: str, checked: bool = False):
     extra = {"MixedChecked": 1, "MixedSweeps": 24, "MixedFixSweeps": 2,
              "MixedMaxCycles": 4} if checked else {}
@@ -476,16 +490,20 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
     cell-updates/s and each job's final max|div|, against which every served
     job's is compared; then those n_plain jobs are served again in a checked
     batch, each iterated to its own FP64 final max|div| (24 FP32 pairs closed
-    by the FP64 residual check, then checked cycles of 2 pairs, at most 4).  policies "both" runs the same stream again under
-    least-remaining-work-first admission with preemption (snapshots to host
-    memory) and compares its results with the first run's; "preempt" runs
-    only that policy."""
+    by the FP64 residual check, then checked cycles of 2 pairs, at most 4).
+
+    `policies` is a comma list of fcfs (first come first served, every
+    resident job advances each batched step), steps (admission as fcfs; each
+    batched step advances the resident jobs with the least remaining work,
+    up to the step cell budget) and preempt (least remaining work admitted
+    first, preempting through host snapshots); the first is the main run,
+    the others run the same stream again and compare their results with it."""
     from aeroflux_amd import workload
 
+    names = [p for p in policies.split(",") if p]
     jobs = workload.served_jobs(n_jobs, seed=7, rate=rate)
-    first = af.SERVE_LEAST_WORK_PREEMPT if policies == "preempt" else af.SERVE_FCFS
-    label = {af.SERVE_FCFS: "first come first served",
-             af.SERVE_LEAST_WORK_PREEMPT: "least work first, preempting"}[first]
+    first = SERVE_POLICIES[names[0]][0]
+    label = SERVE_POLICIES[names[0]][1]
     res, st, info = serve_once(af, jobs, budget, first, label)
     plain = None
     if n_plain > 0:
@@ -549,20 +567,18 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
         "speedup_vs_plain_fp64": (st["CellUpdates"] / st["Wall"] / plain["value"]
                                   if plain and st["Wall"] > 0 else None),
     }
-    if policies == "both":
-        pres, pst, _ = serve_once(af, jobs, budget, af.SERVE_LEAST_WORK_PREEMPT,
-                                  "least work first, preempting")
+    for name in names[1:]:
+        pol, plabel, key = SERVE_POLICIES[name]
+        pres, pst, _ = serve_once(af, jobs, budget, pol, plabel)
         same = all((a["MaxDivergence"], a["KineticEnergy"], a["Status"]) ==
                    (c["MaxDivergence"], c["KineticEnergy"], c["Status"])
                    for a, c in zip(res, pres))
-        out["least_work_first_preempting"] = {
-            "policy": "least remaining work (cells x steps left) first; a resident job with "
-                      "more than 4x the work of a queued job that cannot be placed is "
-                      "snapshotted to host memory and requeued (at most once per job)",
+        out[key] = {
+            "policy": plabel,
             **serve_summary(jobs, pres, pst),
             "speedup_vs_plain_fp64": (pst["CellUpdates"] / pst["Wall"] / plain["value"]
                                       if plain and pst["Wall"] > 0 else None),
-            "results_identical_to_first_come_first_served": same,
+            "results_identical_to_main_run": same,
         }
     return out
 
@@ -645,9 +661,9 @@ def main() -> int:
                     help="jobs in the served stream (the full stream is 10000)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--served-budget", type=float, default=200e9)
-    ap.add_argument("--served-policy", choices=("fcfs", "preempt", "both"), default="both",
-                    help="admission policies of the served stream: first come first served, "
-                         "least remaining work first with preemption, or both in turn")
+    ap.add_argument("--served-policy", default="fcfs,steps,preempt",
+                    help="comma list of serving policies, the first the main run: fcfs, steps "
+                         "(least remaining work stepped first under a cell budget), preempt")
     ap.add_argument("--served-plain-jobs", type=int, default=60,
                     help="jobs of the stream also run alone on the plain FP64 path")
     args = ap.parse_args()

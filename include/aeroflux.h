@@ -359,17 +359,26 @@ typedef struct af_batch_config {
                                   gathers); k (1..4) -> variant k-1, i.e.
                                   (rows, z-planes) of (4,16), (4,32), (8,16),
                                   (8,32) per workgroup */
-    int32_t ServePolicy;       /* af_serve admission: 0 -> first come first
-                                  served; 1 -> least remaining work
-                                  (cells x steps left) first, preempting: a
-                                  resident job is snapshotted to host memory
-                                  and requeued when a queued job that cannot
-                                  be placed has less than 1/PreemptRatio of
-                                  its remaining work; each job is preempted
-                                  at most once */
+    int32_t ServePolicy;       /* af_serve: 0 -> admission first come first
+                                  served, every resident job advances each
+                                  batched step; 1 -> least remaining work
+                                  (cells x steps left) admitted first,
+                                  preempting: a resident job is snapshotted
+                                  to host memory and requeued when a queued
+                                  job that cannot be placed has less than
+                                  1/PreemptRatio of its remaining work; each
+                                  job is preempted at most once; 2 ->
+                                  admission as 0, but each batched step
+                                  advances only the resident jobs with the
+                                  least remaining work, up to StepCellBudget
+                                  interior cells (at least one job), so short
+                                  jobs finish first while the GPU stays
+                                  full */
     double PreemptRatio;       /* policy 1 (0 -> 4) */
     int64_t SnapshotBudget;    /* policy 1: host bytes that preempted jobs'
                                   snapshots may hold at once (0 -> 32 GiB) */
+    int64_t StepCellBudget;    /* policy 2: interior cells one batched step
+                                  advances (0 -> 32 Mi cells) */
 } af_batch_config;
 
 typedef struct af_batch_info {

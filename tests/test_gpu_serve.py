@@ -31,15 +31,18 @@ def alone(af, prec, j):
     return out
 
 
-@pytest.mark.parametrize("policy", [afm.SERVE_FCFS, afm.SERVE_LEAST_WORK_PREEMPT])
+@pytest.mark.parametrize("policy", [afm.SERVE_FCFS, afm.SERVE_LEAST_WORK_PREEMPT,
+                                    afm.SERVE_LEAST_WORK_STEPS])
 @pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
 @pytest.mark.parametrize("realtime", [False, True])
 def test_served_results_equal_alone(af, prec, realtime, policy):
     jobs = small_stream(14, 5 + prec, rate=200.0)
     biggest = max(af.SlotBytes(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], Precision=prec),
                                prec) for j in jobs)
+    # policy 2 with a budget of about two jobs' cells: the resident set is
+    # stepped a few jobs at a time, and the subset changes between steps
     b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=3 * biggest, MaxSlots=16,
-                                ServePolicy=policy))
+                                ServePolicy=policy, StepCellBudget=2 * 24 ** 3))
     res, st = af.Serve(b, jobs, realtime=realtime)
     assert st["Steps"] > 0 and 1 <= st["MaxResident"] <= len(jobs)
     assert st["PeakBytes"] <= 3 * biggest
@@ -132,6 +135,28 @@ def test_checked_served_jobs_meet_their_targets(af):
         e.StepN(j["Steps"])
         assert (r["MaxDivergence"], r["KineticEnergy"]) == (e.MaxDivergence(), e.KineticEnergy())
         e.Close()
+
+
+def test_least_work_stepping_finishes_short_jobs_first(af):
+    """Policy 2 with a budget of one job's cells: each batched step advances
+    only the resident job with the least remaining work, so four jobs of 8,
+    2, 5 and 1 steps queued together take 16 batched steps and finish in
+    the order 1, 2, 5, 8 -- and each result equals the job run alone."""
+    jobs = [dict(NX=16, NY=16, NZ=16, H=1 / 16, Viscosity=1e-4, DT=0.004, Steps=s, Seed=30 + i,
+                 Arrival=0.0) for i, s in enumerate((8, 2, 5, 1))]
+    b = af.Batch(af.BatchConfig(MemoryBudget=64 << 20, MaxSlots=8,
+                                ServePolicy=afm.SERVE_LEAST_WORK_STEPS, StepCellBudget=16 ** 3))
+    res, st = af.Serve(b, jobs, realtime=False)
+    assert st["Steps"] == 16 and st["MaxResident"] == 4
+    assert res[3]["Done"] <= res[1]["Done"] <= res[2]["Done"] <= res[0]["Done"]
+    # after the run every slot advances again
+    s = b.Add(af.Config(NX=8, NY=8, NZ=8))
+    b.StepN(1)
+    b.Remove(s)
+    b.Close()
+    for j, r in zip(jobs, res):
+        assert r["Status"] == afm.AF_OK
+        assert (r["MaxDivergence"], r["KineticEnergy"]) == alone(af, afm.PRECISION_FP64, j)
 
 
 def test_serve_progress_reports(af):
