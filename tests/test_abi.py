@@ -154,9 +154,10 @@ def C_sizeof(t):
     return ctypes.sizeof(t)
 
 
-@pytest.mark.parametrize("kw", [dict(ServePolicy=2), dict(ServePolicy=-1),
+@pytest.mark.parametrize("kw", [dict(ServePolicy=3), dict(ServePolicy=-1),
                                 dict(ServePolicy=1, PreemptRatio=0.5),
-                                dict(SnapshotBudget=-1)])
+                                dict(SnapshotBudget=-1),
+                                dict(ServePolicy=2, StepCellBudget=-5)])
 def test_batch_serve_policy_is_validated(lib, kw):
     """Rejected before any device call, so this runs without a GPU."""
     with pytest.raises(af.AeroFluxError) as ei:
