@@ -49,7 +49,7 @@ AXIS_X, AXIS_Y, AXIS_Z = range(3)
 COLOR_RED, COLOR_BLACK = range(2)
 PRECISION_FP64, PRECISION_MIXED = range(2)
 STREAM_READ, STREAM_COPY, STREAM_2R1W, STREAM_4R3W = range(4)
-SERVE_FCFS, SERVE_LEAST_WORK_PREEMPT, SERVE_LEAST_WORK_STEPS = 0, 1, 2
+SERVE_FCFS, SERVE_LEAST_WORK_PREEMPT, SERVE_LEAST_WORK_STEPS, SERVE_LEAST_WORK = 0, 1, 2, 3
 DIRECT_AUTO, DIRECT_FFT, DIRECT_DENSE = 0, 1, 2
 
 

@@ -1081,10 +1081,10 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
     if (c.PreemptRatio == 0) c.PreemptRatio = 4.0;
     if (c.SnapshotBudget == 0) c.SnapshotBudget = (int64_t)32 << 30;
     if (c.StepCellBudget == 0) c.StepCellBudget = kDefaultStepCells;
-    if (c.ServePolicy < 0 || c.ServePolicy > 2 || !(c.PreemptRatio >= 1.0) ||
+    if (c.ServePolicy < 0 || c.ServePolicy > 3 || !(c.PreemptRatio >= 1.0) ||
         std::isinf(c.PreemptRatio) || c.SnapshotBudget < 0 || c.StepCellBudget < 1)
         return fail(AF_ERR_CONFIG,
-                    "aeroflux: serve policy must be 0, 1 or 2, PreemptRatio >= 1, "
+                    "aeroflux: serve policy must be 0, 1, 2 or 3, PreemptRatio >= 1, "
                     "SnapshotBudget >= 0, StepCellBudget >= 1");
 
     af_batch *b = new (std::nothrow) af_batch();
@@ -1645,8 +1645,11 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
     std::deque<std::pair<int64_t, std::vector<int64_t>>> pending;
     // Waiting jobs, ordered by key: arrival index (first come first served)
     // or remaining work, cells x steps left, ties by arrival (policy 1).
-    const bool by_work = b->cfg.ServePolicy == 1;
-    const bool by_steps = b->cfg.ServePolicy == 2;
+    // Policies 1 and 3 admit the least remaining work first; 1 preempts;
+    // 2 and 3 step the least remaining work first under the cell budget.
+    const bool by_work = b->cfg.ServePolicy == 1 || b->cfg.ServePolicy == 3;
+    const bool preempting = b->cfg.ServePolicy == 1;
+    const bool by_steps = b->cfg.ServePolicy == 2 || b->cfg.ServePolicy == 3;
     std::vector<std::pair<double, int64_t>> order;  // policy 2: (work left, resident index)
     std::set<std::pair<double, int64_t>> queue;
     auto work_of = [&](int64_t j, int64_t left) {
@@ -1705,7 +1708,7 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
     // the job's enqueued steps and `left` steps remain.
     auto preempt_for = [&](int64_t j, bool *done) -> int {
         *done = false;
-        if (!by_work || resident.empty()) return AF_OK;
+        if (!preempting || resident.empty()) return AF_OK;
         // A job is preempted at most once: after it resumes it runs to the
         // end, which bounds the snapshot traffic a steady stream of short
         // arrivals can cause.

@@ -454,6 +454,9 @@ SERVE_POLICIES = {
     "steps": (2, "admission first come first served; each batched step advances the resident "
                  "jobs with the least remaining work (cells x steps left), up to 32 Mi "
                  "interior cells", "least_work_stepping"),
+    "least": (3, "least remaining work (cells x steps left) admitted first, without "
+                 "preemption, and stepped first, up to 32 Mi interior cells per batched step",
+              "least_work_first"),
     "preempt": (1, "least remaining work (cells x steps left) admitted first; a resident job "
                    "with more than 4x the work of a queued job that cannot be placed is "
                    "snapshotted to host memory and requeued (at most once per job)",

@@ -373,11 +373,13 @@ typedef struct af_batch_config {
                                   least remaining work, up to StepCellBudget
                                   interior cells (at least one job), so short
                                   jobs finish first while the GPU stays
-                                  full */
+                                  full; 3 -> both: least remaining work
+                                  admitted first (no preemption) and stepped
+                                  first */
     double PreemptRatio;       /* policy 1 (0 -> 4) */
     int64_t SnapshotBudget;    /* policy 1: host bytes that preempted jobs'
                                   snapshots may hold at once (0 -> 32 GiB) */
-    int64_t StepCellBudget;    /* policy 2: interior cells one batched step
+    int64_t StepCellBudget;    /* policies 2, 3: interior cells one batched step
                                   advances (0 -> 32 Mi cells) */
 } af_batch_config;
 

@@ -154,7 +154,7 @@ def C_sizeof(t):
     return ctypes.sizeof(t)
 
 
-@pytest.mark.parametrize("kw", [dict(ServePolicy=3), dict(ServePolicy=-1),
+@pytest.mark.parametrize("kw", [dict(ServePolicy=4), dict(ServePolicy=-1),
                                 dict(ServePolicy=1, PreemptRatio=0.5),
                                 dict(SnapshotBudget=-1),
                                 dict(ServePolicy=2, StepCellBudget=-5)])

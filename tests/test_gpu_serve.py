@@ -32,7 +32,7 @@ def alone(af, prec, j):
 
 
 @pytest.mark.parametrize("policy", [afm.SERVE_FCFS, afm.SERVE_LEAST_WORK_PREEMPT,
-                                    afm.SERVE_LEAST_WORK_STEPS])
+                                    afm.SERVE_LEAST_WORK_STEPS, afm.SERVE_LEAST_WORK])
 @pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
 @pytest.mark.parametrize("realtime", [False, True])
 def test_served_results_equal_alone(af, prec, realtime, policy):
