@@ -495,12 +495,14 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
     batch, each iterated to its own FP64 final max|div| (24 FP32 pairs closed
     by the FP64 residual check, then checked cycles of 2 pairs, at most 4).
 
-    `policies` is a comma list of fcfs (first come first served, every
-    resident job advances each batched step), steps (admission as fcfs; each
-    batched step advances the resident jobs with the least remaining work,
-    up to the step cell budget) and preempt (least remaining work admitted
-    first, preempting through host snapshots); the first is the main run,
-    the others run the same stream again and compare their results with it."""
+    `policies` is a comma list of least (least remaining work admitted
+    first and stepped first, up to the step cell budget), fcfs (first come
+    first served, every resident job advances each batched step), steps
+    (admission as fcfs; each batched step advances the resident jobs with the
+    least remaining work, up to the step cell budget) and preempt (least
+    remaining work admitted first, preempting through host snapshots); the
+    first is the main run, the others run the same stream again and compare
+    their results with it."""
     from aeroflux_amd import workload
 
     names = [p for p in policies.split(",") if p]
@@ -664,9 +666,11 @@ def main() -> int:
                     help="jobs in the served stream (the full stream is 10000)")
     ap.add_argument("--served-rate", type=float, default=200.0)
     ap.add_argument("--served-budget", type=float, default=200e9)
-    ap.add_argument("--served-policy", default="fcfs,steps,preempt",
-                    help="comma list of serving policies, the first the main run: fcfs, steps "
-                         "(least remaining work stepped first under a cell budget), preempt")
+    ap.add_argument("--served-policy", default="least,fcfs",
+                    help="comma list of serving policies, the first the main run: least "
+                         "(least remaining work admitted and stepped first), fcfs (lock step), "
+                         "steps (first come first served admission, least work stepped "
+                         "first), preempt (least work admitted first, preempting)")
     ap.add_argument("--served-plain-jobs", type=int, default=60,
                     help="jobs of the stream also run alone on the plain FP64 path")
     args = ap.parse_args()
