@@ -87,7 +87,7 @@ class _CBatchConfig(C.Structure):
         ("MixedSweeps", C.c_int32), ("MixedFixSweeps", C.c_int32), ("MixedChecked", C.c_int32),
         ("SweepShape", C.c_int32), ("Workgroups", C.c_int32), ("SweepStaged", C.c_int32),
         ("ServePolicy", C.c_int32), ("PreemptRatio", C.c_double), ("SnapshotBudget", C.c_int64),
-        ("StepCellBudget", C.c_int64),
+        ("StepCellBudget", C.c_int64), ("SolveGroupBytes", C.c_int64),
     ]
 
 
@@ -97,6 +97,7 @@ class _CBatchInfo(C.Structure):
         ("PassWorkgroups", C.c_int32), ("OtherWorkgroups", C.c_int32), ("Tuned", C.c_int32),
         ("Cells", C.c_int64), ("BytesInUse", C.c_int64), ("BytesLargestFree", C.c_int64),
         ("MemoryBudget", C.c_int64), ("SweepStaged", C.c_int32), ("StagedSlots", C.c_int32),
+        ("SolveGroups", C.c_int32),
     ]
 
 
@@ -599,7 +600,9 @@ class BatchConfig:
     """Batch of engines stepped together (no reference counterpart; each slot
     steps as Step, aeroflux.go:348-409, does).  Zero fields take defaults:
     1024 slots, a 1 GiB pool, the Config.Normalize solve schedule, tuned
-    sweep shape and workgroup counts."""
+    sweep shape and workgroup counts, the pressure solve run in groups of
+    slots of about 224 MiB of solve buffers each (``SolveGroupBytes``; -1:
+    one group)."""
     Precision: int = PRECISION_FP64
     MaxSlots: int = 0
     MemoryBudget: int = 0
@@ -615,6 +618,7 @@ class BatchConfig:
     PreemptRatio: float = 0.0
     SnapshotBudget: int = 0
     StepCellBudget: int = 0
+    SolveGroupBytes: int = 0
 
     def _to_c(self) -> _CBatchConfig:
         c = _CBatchConfig()

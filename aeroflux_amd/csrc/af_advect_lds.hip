@@ -31,6 +31,9 @@
 #include "af_sweep_lds.h"
 #include "af_xcd.h"
 
+#include <climits>
+#include <cstdint>
+
 #pragma clang fp contract(off)
 
 namespace af {
@@ -38,7 +41,9 @@ namespace {
 
 constexpr int BX = body::BX;
 
-template <int AX, int BY, int D, int ZC>
+// nb = (x-tiles, y-tiles, z-chunks); the z-chunks split each tile column's
+// face planes evenly (slds::zchunk).
+template <int AX, int BY, int D>
 __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int ncol)
 {
     using B = slds::Box<AX, BY, D>;
@@ -49,10 +54,11 @@ __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int 
         slds::wall_face<AX, false>(
             a, ((int)blockIdx.x - ntile) * (BX * BY) + threadIdx.y * BX + threadIdx.x, bad);
     } else {
-        int bx, by, bz;
+        int bx, by, bz, zs, zend;
         xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+        slds::zchunk(a.G, nb.z, bz, zs, zend);
         auto src_of = [&](int c) -> const DField & { return c == 0 ? a.U : (c == 1 ? a.V : a.W); };
-        slds::tile<AX, BY, D, ZC, false>(a, src_of, (slds::lds_t *)lds_mem, bx, by, bz, bad);
+        slds::tile<AX, BY, D, false>(a, src_of, (slds::lds_t *)lds_mem, bx, by, zs, zend, bad);
     }
     if (bad) *body::gp(a.flag) = 1;
 }
@@ -60,21 +66,84 @@ __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int 
 template <int BY, int D, int ZC>
 hipError_t launch_lds(int axis, const SweepArgs &a, hipStream_t s)
 {
-    const int3 nb = slds::tiles<BY, ZC>(a.G);
+    int3 nb = slds::tiles<BY, ZC>(a.G);
+    if (a.zchunks > 0) nb.z = a.zchunks < a.G.NZ + 1 ? a.zchunks : a.G.NZ + 1;
     const int ncol = slds::wall_blocks<BY>(a.G);
     const dim3 grid((unsigned)(nb.x * nb.y * nb.z + ncol));
     const dim3 block(BX, BY);
     switch (axis) {
-    case 0: hipLaunchKernelGGL((k_sweep_lds<0, BY, D, ZC>), grid, block, 0, s, a, nb, ncol); break;
-    case 1: hipLaunchKernelGGL((k_sweep_lds<1, BY, D, ZC>), grid, block, 0, s, a, nb, ncol); break;
-    default: hipLaunchKernelGGL((k_sweep_lds<2, BY, D, ZC>), grid, block, 0, s, a, nb, ncol); break;
+    case 0: hipLaunchKernelGGL((k_sweep_lds<0, BY, D>), grid, block, 0, s, a, nb, ncol); break;
+    case 1: hipLaunchKernelGGL((k_sweep_lds<1, BY, D>), grid, block, 0, s, a, nb, ncol); break;
+    default: hipLaunchKernelGGL((k_sweep_lds<2, BY, D>), grid, block, 0, s, a, nb, ncol); break;
     }
     return hipGetLastError();
+}
+
+// Resident workgroups per CU of variant (BY, D)'s X-sweep kernel, and the
+// device's CU count (0 on error).
+template <int BY, int D>
+int lds_slots(int *cus)
+{
+    int dev = 0, n = 0;
+    *cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_lds<0, BY, D>, BX * BY, 0) !=
+            hipSuccess)
+        return 0;
+    return n;
+}
+
+template <int BY, int D, int ZC>
+int lds_chunks(const Geom &G)
+{
+    int cus = 0;
+    const int occ = lds_slots<BY, D>(&cus);
+    const int planes = G.NZ + 1;
+    const int3 nb = slds::tiles<BY, ZC>(G);
+    if (occ <= 0 || cus < kXcds) return nb.z;
+    const int64_t per_xcd = (int64_t)occ * (cus / kXcds);
+    // A round of tiles costs the chunk's planes plus about one plane of
+    // ring prologue; chunks of 6 to 64 planes are considered.
+    int best = nb.z;
+    int64_t best_cost = INT64_MAX;
+    for (int n = (planes + 63) / 64; n <= planes / 6 || n == (planes + 63) / 64; n++) {
+        const int64_t tiles = (int64_t)nb.x * nb.y * n;
+        const int64_t rounds = ((tiles + kXcds - 1) / kXcds + per_xcd - 1) / per_xcd;
+        const int64_t cost = rounds * ((planes + n - 1) / n + 1);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = n;
+        }
+    }
+    return best;
 }
 
 }  // namespace
 
 int sweep_lds_variant_count() { return 4; }
+
+// The z-chunk count per tile column of LDS-staged variant `variant` on grid
+// G whose tiles best fill whole rounds of the workgroups each XCD holds at
+// once (XCD x runs the x-th eighth of the tiles).  The tiles of a round run
+// side by side and take about equally long, so a last round that is mostly
+// empty costs nearly a full one.
+int sweep_lds_chunks(const Geom &G, int variant)
+{
+    switch (variant) {
+    case 0: return lds_chunks<4, 2, 16>(G);
+    case 1: return lds_chunks<4, 2, 32>(G);
+    case 2: return lds_chunks<8, 2, 16>(G);
+    default: return lds_chunks<8, 2, 32>(G);
+    }
+}
+
+// The shape's own chunk count: z-chunks of at most ZC planes.
+int sweep_lds_default_chunks(const Geom &G, int variant)
+{
+    const int zc = variant == 1 || variant == 3 ? 32 : 16;
+    return (G.NZ + 1 + zc - 1) / zc;
+}
 
 // LDS-staged variants: (rows per workgroup, planes per z-chunk); staged reach
 // two cells along x and y, one along z.

@@ -52,6 +52,11 @@ constexpr int XCDS = 8;
 // Serving policy 2: interior cells one batched step advances by default.
 constexpr int64_t kDefaultStepCells = (int64_t)32 << 20;
 
+// Default solve group size: bytes of split arrays (both colours, solution
+// and right-hand side) per group of slots whose solve runs together.  About
+// the 256 MB last-level cache, so a group's passes reread it from there.
+constexpr int64_t kDefaultSolveGroupBytes = (int64_t)224 << 20;
+
 // Stage kinds: each has its own tile decomposition and prefix table.
 enum Kind {
     K_VBC,    // velocity-BC shell cells, CHUNK per item
@@ -123,9 +128,10 @@ struct BSlot {
 // Kernel argument shared by every stage.
 struct BCtl {
     const BSlot *slots;
-    const int *tab;  // nact, stride, act[stride], first[K_COUNT][stride]
+    const int *tab;  // nact, stride, act[stride], first[K_COUNT][stride], groups
     int *flags, *done, *cycles;
     double *last;
+    int grp = -1;  // solve group whose slots a launch covers (-1: every slot)
 };
 
 // Tiles of the plane-wise stages flatten each xy-plane over a workgroup's
@@ -187,8 +193,9 @@ __device__ __forceinline__ int plane_point(int t, int row, int plane, int thread
     return zb;
 }
 
-// Persistent loop over the work-list of stage KIND.  XCD x (= blockIdx.x %
-// 8) owns the x-th eighth of the items, in order; its workgroups stride
+// Persistent loop over the work-list of stage KIND: every active slot's
+// items, or those of solve group c.grp's slots.  XCD x (= blockIdx.x % 8)
+// owns the x-th eighth of the items, in order; its workgroups stride
 // through that range.  f(slot, tile) runs with a workgroup-uniform slot and
 // tile, so bodies may synchronise the workgroup.  gridDim.x % 8 == 0.
 template <int KIND, typename F>
@@ -198,14 +205,23 @@ __device__ __forceinline__ void for_items(const BCtl &c, F f)
     const int nact = tab[0], stride = tab[1];
     const int *act = tab + 2;
     const int *first = tab + 2 + stride * (1 + KIND);
-    const int total = first[nact];
+    int a_lo = 0, a_hi = nact;
+    if (c.grp >= 0) {
+        // Groups: count, then count + 1 boundaries in act[] order.
+        const int *gr = tab + 2 + stride * (1 + K_COUNT);
+        if (c.grp >= gr[0]) return;
+        a_lo = gr[1 + c.grp];
+        a_hi = gr[2 + c.grp];
+    }
+    const int base = first[a_lo];
+    const int total = first[a_hi] - base;
     const int lanes = (int)(gridDim.x / XCDS);
     const int xcd = (int)(blockIdx.x % XCDS);
-    const int lo = (int)(((int64_t)total * xcd) / XCDS);
-    const int hi = (int)(((int64_t)total * (xcd + 1)) / XCDS);
+    const int lo = base + (int)(((int64_t)total * xcd) / XCDS);
+    const int hi = base + (int)(((int64_t)total * (xcd + 1)) / XCDS);
     int i = lo + (int)(blockIdx.x / XCDS);
     if (i >= hi) return;
-    int a = 0, r = nact - 1;  // largest a with first[a] <= i
+    int a = a_lo, r = a_hi - 1;  // largest a with first[a] <= i
     while (a < r) {
         const int m = (a + r + 1) >> 1;
         if (first[m] <= i) a = m; else r = m - 1;
@@ -307,8 +323,10 @@ __global__ __launch_bounds__(BX *LBY) void kb_sweep_lds(BCtl c, int sw)
         // Descriptors by run-time index come from the slot table in global
         // memory (a local copy indexed at run time would live in scratch).
         auto src_of = [&](int q) -> const DField & { return velf(e, sw, q); };
-        slds::tile<AX, LBY, 2, ZC, true>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
-                                         rest % nb.y, rest / nb.y, bad);
+        int zs, zend;
+        slds::zchunk(e.G, nb.z, rest / nb.y, zs, zend);
+        slds::tile<AX, LBY, 2, true>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
+                                     rest % nb.y, zs, zend, bad);
     }
     if (bad) *body::gp(sa.flag) = 1;
 }
@@ -578,7 +596,9 @@ struct af_batch {
     double *h_dbl = nullptr; // pinned, MaxSlots
     bool tab_dirty = true;
     int tab_shape = -1;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;  // step graph for the current table's group count
+    std::map<int, hipGraphExec_t> graphs;  // step graphs by solve-group count
+    int ngroups = 1;                       // solve groups of the current table
     int shape = 4;  // sweep plan (make_plan): gather shape + LDS-staged variant
     int nwg_sweep = 2048, nwg_pass = 2048, nwg_other = 2048;
     int lds_grid = 8;      // workgroups of the LDS-staged sweep launches in the graph
@@ -609,9 +629,10 @@ int check_slot(const af_batch *b, int32_t slot)
     return AF_OK;
 }
 
-BCtl ctl(const af_batch *b)
+BCtl ctl(const af_batch *b, int grp = -1)
 {
     BCtl c;
+    c.grp = grp;
     c.slots = b->d_slots;
     c.tab = b->d_tab;
     const int m = b->cfg.MaxSlots;
@@ -664,6 +685,36 @@ int build_table(af_batch *b, int plan, int ring = 0)
         }
     }
     b->nact = n;
+    // Solve groups: contiguous runs of act[] of about equal split-array
+    // bytes, as many as the group byte budget needs (one when it is -1); a
+    // group also closes before a slot that would take it past the budget.
+    {
+        int *gr = t + 2 + (int64_t)stride * (1 + K_COUNT);
+        const int64_t elem = b->cfg.Precision == AF_PRECISION_MIXED ? 4 : 8;
+        std::vector<int64_t> acc((size_t)n + 1, 0);
+        for (int a = 0; a < n; a++) {
+            const af_config &c = b->slots[act[a]].cfg;
+            acc[(size_t)a + 1] = acc[(size_t)a] + 4 * c.NZ * c.NY * ((c.NX + 1) / 2) * elem;
+        }
+        const int64_t cap = b->cfg.SolveGroupBytes;
+        int64_t want = cap < 0 || n == 0 ? 1 : (acc[(size_t)n] + cap - 1) / cap;
+        want = want < 1 ? 1 : (want > n && n > 0 ? n : want);
+        int g = 0, start = 0;
+        gr[1] = 0;
+        for (int a = 1; a < n; a++) {
+            // A boundary before act[a] when a passes the next equal share, or
+            // when act[a] would take the open group past the budget.
+            const bool share = g + 1 < want && acc[(size_t)a] * want >= acc[(size_t)n] * (g + 1);
+            const bool over = cap > 0 && acc[(size_t)a + 1] - acc[(size_t)start] > cap;
+            if (share || over) {
+                gr[2 + g++] = a;
+                start = a;
+            }
+        }
+        gr[2 + g] = n;
+        gr[0] = g + 1;
+        b->ngroups = g + 1;
+    }
     const int lds_total = t[2 + (int64_t)stride * (1 + K_LDS) + n];
     b->lds_need = 8 * ((lds_total + 7) / 8);
     if (b->lds_need < 8) b->lds_need = 8;
@@ -760,9 +811,9 @@ hipError_t enq_sweep(const af_batch *b, int plan, int nwg, int axis, int sw)
 }
 
 template <typename T>
-hipError_t enq_pass(const af_batch *b, int nwg, int color, int mode, int gated)
+hipError_t enq_pass(const af_batch *b, int nwg, int color, int mode, int gated, int grp = -1)
 {
-    const BCtl c = ctl(b);
+    const BCtl c = ctl(b, grp);
     const dim3 grid(nwg), blk(BX, BY);
     switch (mode) {
     case 0: hipLaunchKernelGGL((kb_pass<T, 0>), grid, blk, 0, b->stream, c, color, gated); break;
@@ -799,38 +850,51 @@ hipError_t enq_grad(const af_batch *b, int sw)
 
 // solvePressureRedBlack, project.go:73-100, on every slot: FP64 colour-split
 // passes (the first from P == 0), or the mixed-precision defect correction.
-hipError_t enq_solve(const af_batch *b)
+// Slots' solves are independent, so the solve runs group by group (the
+// table's solve groups, `ngroups` of them): all of one group's passes, then
+// the next group's, and a group's split arrays stay in the last-level cache
+// from one pass to the next instead of being streamed from HBM for every pass.
+hipError_t enq_solve(const af_batch *b, int ngroups)
 {
     const BCtl c = ctl(b);
     const dim3 gp(b->nwg_other), blk(BX, BY);
     hipError_t err = hipSuccess;
     auto mode = [](int p) { return p == 0 ? 1 : (p == 1 ? 2 : 0); };
-    if (b->cfg.Precision != AF_PRECISION_MIXED) {
-        const int passes = (int)(2 * b->cfg.GaussSeidelIters);
-        for (int p = 0; p < passes && err == hipSuccess; p++)
-            err = enq_pass<double>(b, b->nwg_pass, p & 1, mode(p), 0);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((kb_from_split<double, false>), gp, blk, 0, b->stream, c, 0);
-    } else {
-        const bool checked = b->cfg.MixedChecked != 0;
+    const bool mixed = b->cfg.Precision == AF_PRECISION_MIXED;
+    const bool checked = mixed && b->cfg.MixedChecked != 0;
+    if (mixed)
         hipLaunchKernelGGL(kb_mixed_reset, dim3(kSmallGrid), dim3(256), 0, b->stream, c,
                            checked ? 1 : 0);
-        for (int p = 0; p < 2 * b->cfg.MixedSweeps && err == hipSuccess; p++)
-            err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 0);
-        if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, c, 0);
-        if (checked) {
-            hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 0);
-            hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 0);
-            for (int cyc = 1; cyc < b->cfg.MixedMaxCycles; cyc++) {
-                for (int p = 0; p < 2 * b->cfg.MixedFixSweeps && err == hipSuccess; p++)
-                    err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 1);
-                if (err != hipSuccess) return err;
-                hipLaunchKernelGGL((kb_from_split<float, true>), gp, blk, 0, b->stream, c, 1);
-                hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 1);
-                hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 1);
-            }
+    for (int g = 0; g < ngroups && err == hipSuccess; g++) {
+        const BCtl cg = ctl(b, ngroups > 1 ? g : -1);
+        const int grp = cg.grp;
+        if (!mixed) {
+            const int passes = (int)(2 * b->cfg.GaussSeidelIters);
+            for (int p = 0; p < passes && err == hipSuccess; p++)
+                err = enq_pass<double>(b, b->nwg_pass, p & 1, mode(p), 0, grp);
+            if (err != hipSuccess) return err;
+            hipLaunchKernelGGL((kb_from_split<double, false>), gp, blk, 0, b->stream, cg, 0);
+            continue;
         }
+        for (int p = 0; p < 2 * b->cfg.MixedSweeps && err == hipSuccess; p++)
+            err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 0, grp);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, cg, 0);
+        if (!checked) continue;
+        hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, cg, 0);
+        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, cg, 0);
+    }
+    if (err != hipSuccess) return err;
+    // The later correction cycles (a few passes, only for slots whose target
+    // is not met yet) run over every slot at once: each slot still takes the
+    // same operations in the same order.
+    for (int cyc = 1; checked && cyc < b->cfg.MixedMaxCycles; cyc++) {
+        for (int p = 0; p < 2 * b->cfg.MixedFixSweeps && err == hipSuccess; p++)
+            err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 1);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((kb_from_split<float, true>), gp, blk, 0, b->stream, c, 1);
+        hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 1);
+        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 1);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     hipLaunchKernelGGL(kb_pbc, dim3(kSmallGrid), dim3(256), 0, b->stream, c);
@@ -841,17 +905,17 @@ hipError_t enq_solve(const af_batch *b)
 // project, project.go:12-34.  The velocity already holds the BC the
 // projection starts with (the sweep before it applied it), and the gradient
 // subtraction applies the closing one.
-hipError_t enq_project(const af_batch *b, int sw)
+hipError_t enq_project(const af_batch *b, int sw, int ngroups)
 {
     hipError_t err = enq_div(b, sw);
-    if (err == hipSuccess) err = enq_solve(b);
+    if (err == hipSuccess) err = enq_solve(b, ngroups);
     if (err == hipSuccess) err = enq_grad(b, sw);  // applies the closing velocity BC
     return err;
 }
 
 // Step, aeroflux.go:348-409, for every slot.  sw tracks the commitVelocity
 // swaps (0: velocity in U/V/W); the six swaps return it to 0.
-hipError_t enq_step(const af_batch *b)
+hipError_t enq_step(const af_batch *b, int ngroups)
 {
     static const int fwd[3] = {AF_AXIS_X, AF_AXIS_Y, AF_AXIS_Z};
     static const int rev[3] = {AF_AXIS_Z, AF_AXIS_Y, AF_AXIS_X};
@@ -863,29 +927,37 @@ hipError_t enq_step(const af_batch *b)
         err = enq_sweep(b, b->shape, b->nwg_sweep, fwd[k], sw);
         sw ^= 1;
     }
-    if (err == hipSuccess) err = enq_project(b, sw);
+    if (err == hipSuccess) err = enq_project(b, sw, ngroups);
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
         err = enq_sweep(b, b->shape, b->nwg_sweep, rev[k], sw);
         sw ^= 1;
     }
-    if (err == hipSuccess) err = enq_project(b, sw);
+    if (err == hipSuccess) err = enq_project(b, sw, ngroups);
     return err;
 }
 
 void drop_graph(af_batch *b)
 {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto &kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
     b->graph = nullptr;
 }
 
+// The step graph for the current table's solve-group count.  Graphs read
+// the work-list table at run time, so one graph per group count serves
+// every table with that count; they are kept until the plan or grids change.
 int get_graph(af_batch *b)
 {
-    if (b->graph) return AF_OK;
+    auto it = b->graphs.find(b->ngroups);
+    if (it != b->graphs.end()) {
+        b->graph = it->second;
+        return AF_OK;
+    }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamSynchronize(b->stream), "batch step capture");
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal),
             "begin batch step capture");
-    hipError_t err = enq_step(b);
+    hipError_t err = enq_step(b, b->ngroups);
     hipError_t err2 = hipStreamEndCapture(b->stream, &g);
     if (err != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
@@ -896,6 +968,7 @@ int get_graph(af_batch *b)
     hipError_t err3 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (err3 != hipSuccess) return hip_fail(err3, "instantiate batch step graph");
+    b->graphs[b->ngroups] = x;
     b->graph = x;
     return AF_OK;
 }
@@ -921,10 +994,28 @@ int time_launches(af_batch *b, int reps, F one, double *ms_out)
     return AF_OK;
 }
 
-hipError_t enq_timed_pass(const af_batch *b, int nwg)
+hipError_t enq_timed_pass(const af_batch *b, int nwg, int grp = -1)
 {
-    if (b->cfg.Precision == AF_PRECISION_MIXED) return enq_pass<float>(b, nwg, 0, 0, 0);
-    return enq_pass<double>(b, nwg, 0, 0, 0);
+    if (b->cfg.Precision == AF_PRECISION_MIXED) return enq_pass<float>(b, nwg, 0, 0, 0, grp);
+    return enq_pass<double>(b, nwg, 0, 0, 0, grp);
+}
+
+// One colour pass over every slot as the step runs it, group by group with
+// each group's passes back to back: the sum over groups of one pass's
+// average time.
+int time_grouped_pass(af_batch *b, int nwg, int reps, double *ms_out)
+{
+    if (b->ngroups <= 1)
+        return time_launches(b, reps, [&] { return enq_timed_pass(b, nwg); }, ms_out);
+    double sum = 0;
+    for (int g = 0; g < b->ngroups; g++) {
+        double ms = 0;
+        if (int rc = time_launches(b, reps, [&] { return enq_timed_pass(b, nwg, g); }, &ms))
+            return rc;
+        sum += ms;
+    }
+    *ms_out = sum;
+    return AF_OK;
 }
 
 // Autotuner: the gather shape (timed on X, Y and Z sweeps with every slot
@@ -992,8 +1083,7 @@ int tune(af_batch *b)
                 bs = ms;
                 b->nwg_sweep = w;
             }
-            if (int rc = time_launches(b, reps, [&] { return enq_timed_pass(b, w); }, &ms))
-                return rc;
+            if (int rc = time_grouped_pass(b, w, reps, &ms)) return rc;
             if (ms < bp) {
                 bp = ms;
                 b->nwg_pass = w;
@@ -1086,6 +1176,9 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
         return fail(AF_ERR_CONFIG,
                     "aeroflux: serve policy must be 0, 1, 2 or 3, PreemptRatio >= 1, "
                     "SnapshotBudget >= 0, StepCellBudget >= 1");
+    if (c.SolveGroupBytes == 0) c.SolveGroupBytes = kDefaultSolveGroupBytes;
+    if (c.SolveGroupBytes < -1)
+        return fail(AF_ERR_CONFIG, "aeroflux: SolveGroupBytes must be -1, 0 or positive");
 
     af_batch *b = new (std::nothrow) af_batch();
     if (!b) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
@@ -1103,7 +1196,7 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
     if (err != hipSuccess) return bail(err, "no HIP device", AF_ERR_HIP);
     const int m = c.MaxSlots;
     b->slots.resize(m);
-    b->tab_ints = 2 + (int64_t)(m + 1) * (1 + K_COUNT);
+    b->tab_ints = 2 + (int64_t)(m + 1) * (1 + K_COUNT) + (m + 2);
     b->shape = make_plan(c.SweepShape > 0 ? c.SweepShape - 1 : 4,
                          c.SweepStaged > 0 ? c.SweepStaged - 1 : -1);
     if (c.Workgroups > 0) b->nwg_sweep = b->nwg_pass = b->nwg_other = c.Workgroups;
@@ -1178,6 +1271,7 @@ int af_batch_get_info(const af_batch *b, af_batch_info *out)
     i.BytesInUse = b->pool.used;
     i.BytesLargestFree = b->pool.largest();
     i.MemoryBudget = b->cfg.MemoryBudget;
+    i.SolveGroups = b->ngroups;
     *out = i;
     return AF_OK;
 }
@@ -1186,10 +1280,9 @@ int af_batch_get_info(const af_batch *b, af_batch_info *out)
 // solve schedule.
 static int check_cfg(const af_batch *b, const af_config &c)
 {
-    if (c.UseSparseSolver)
-        return fail(AF_ERR_UNSUPPORTED,
-                    "aeroflux: sparse direct solver requires darwin with cgo (Accelerate); the "
-                    "Red-Black iterative solver is used instead");
+    // UseSparseSolver is accepted and the red-black solve runs, as for one
+    // engine (New never reads the flag; project.go:19 falls back off Darwin,
+    // sparse_fallback.go:20-24).
     const af_batch_config &bc = b->cfg;
     if (c.Precision != bc.Precision)
         return fail(AF_ERR_CONFIG, "aeroflux: engine precision %d differs from the batch's %d",
@@ -1537,13 +1630,13 @@ int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms)
         return fail(AF_ERR_ARG, "aeroflux: the residual stage needs a mixed-precision batch");
     HIP_TRY(hipSetDevice(b->device), "set device");
     if (int rc = upload_table(b, b->shape)) return rc;
+    if (stage == 0) return time_grouped_pass(b, b->nwg_pass, reps, avg_ms);
     const BCtl c = ctl(b);
     const dim3 gp(b->nwg_other), blk(BX, BY);
     return time_launches(
         b, reps,
         [&]() -> hipError_t {
             switch (stage) {
-            case 0: return enq_timed_pass(b, b->nwg_pass);
             case 1: return enq_sweep(b, b->shape, b->nwg_sweep, 0, 0);
             case 2: return enq_div(b, 0);
             case 3: return enq_grad(b, 0);

@@ -136,6 +136,7 @@ struct af_engine {
     Tile tile64;                // red-black schedule (see Tile)
     int sweep_shape = 4;        // advection workgroup shape (tuned at New)
     int sweep_slab = 1 << 30;   // advection y-slab rows of the block order (tuned at New)
+    int sweep_chunks = 0;       // LDS-staged sweep z-chunks per tile column (0: the shape's)
     int pass_slab = 1 << 30;    // colour-pass y-slab (workgroup rows) of the block order (tuned)
     hipGraphExec_t graphs[8] = {};  // keyed by the swap state of U/V/W
     MixedState mixed;
@@ -198,6 +199,7 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
     a.flag = e->flag;
     a.shape = e->sweep_shape;
     a.slab = e->sweep_slab;
+    a.zchunks = e->sweep_chunks;
     return launch_sweep(axis, a, e->stream);
 }
 
@@ -545,12 +547,14 @@ int tune_blocked(af_engine *e, int k, Tile *out)
     return AF_OK;
 }
 
-// Advection workgroup shape and y-slab height of the block order: every
-// choice gives the same bits (faces are independent), so the fastest on this
-// grid is kept.  Timed on the engine's own (zeroed) arena, X, Y and Z sweeps
-// twice each (the sweeps only write the Tmp fields, which are still zero
+// Advection workgroup shape, y-slab height of the block order and, for the
+// LDS-staged shapes, z-chunks per tile column (the shape's own, or the count
+// that best fills whole rounds of resident workgroups): every choice gives
+// the same bits (faces are independent), so the fastest on this grid is
+// kept.  Timed on the engine's own (zeroed) arena, X, Y and Z sweeps twice
+// each (the sweeps only write the Tmp fields, which are still zero
 // afterwards: advecting zeros gives +0); cached per grid for the process.
-std::map<std::tuple<int, int, int, int>, std::pair<int, int>> g_sweep_cache;
+std::map<std::tuple<int, int, int, int>, std::tuple<int, int, int>> g_sweep_cache;
 
 constexpr int kPlaneOrder = 1 << 30;
 
@@ -572,8 +576,9 @@ int choose_sweep_shape(af_engine *e)
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_sweep_cache.find(key);
         if (it != g_sweep_cache.end() && c.SweepShape == 0 && c.SweepSlab == 0) {
-            e->sweep_shape = it->second.first;
-            e->sweep_slab = it->second.second;
+            e->sweep_shape = std::get<0>(it->second);
+            e->sweep_slab = std::get<1>(it->second);
+            e->sweep_chunks = std::get<2>(it->second);
             return AF_OK;
         }
     }
@@ -582,29 +587,39 @@ int choose_sweep_shape(af_engine *e)
     HIP_TRY(hipEventCreate(&a), "create event");
     HIP_TRY(hipEventCreate(&b), "create event");
     float best = 1e30f;
-    int bs = e->sweep_shape, bl = e->sweep_slab;
+    int bs = e->sweep_shape, bl = e->sweep_slab, bc = 0;
     hipError_t err = hipSuccess;
     for (int s = 0; s < sweep_shape_count() && err == hipSuccess; s++) {
         if (c.SweepShape > 0 && s != c.SweepShape - 1) continue;
+        // Staged shapes: their own chunk count, and the round-filling one.
+        int chunks[2] = {0, 0};
+        if (s >= 8) {
+            const int v = s - 8, m = sweep_lds_chunks(G, v);
+            if (m != sweep_lds_default_chunks(G, v)) chunks[1] = m;
+        }
         for (int sl : kSlabs) {
             if (c.SweepSlab != 0 && sl != e->sweep_slab) continue;
             if (s >= 8 && c.SweepSlab == 0 && sl != kPlaneOrder) continue;  // no slab order
-            if (err != hipSuccess) break;
-            e->sweep_shape = s;
-            e->sweep_slab = sl;
-            err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
-            if (err == hipSuccess) err = hipEventRecord(a, e->stream);
-            for (int r = 0; r < 2 && err == hipSuccess; r++)
-                for (int ax = 0; ax < 3 && err == hipSuccess; ax++)
-                    err = enq_sweep(e, e->cur, ax, e->cfg.DT * 0.5);
-            if (err == hipSuccess) err = hipEventRecord(b, e->stream);
-            if (err == hipSuccess) err = hipEventSynchronize(b);
-            float ms = 0;
-            if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
-            if (err == hipSuccess && ms < best) {
-                best = ms;
-                bs = s;
-                bl = sl;
+            for (int ci = 0; ci < 2 && err == hipSuccess; ci++) {
+                if (ci == 1 && chunks[1] == 0) continue;
+                e->sweep_shape = s;
+                e->sweep_slab = sl;
+                e->sweep_chunks = chunks[ci];
+                err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
+                if (err == hipSuccess) err = hipEventRecord(a, e->stream);
+                for (int r = 0; r < 2 && err == hipSuccess; r++)
+                    for (int ax = 0; ax < 3 && err == hipSuccess; ax++)
+                        err = enq_sweep(e, e->cur, ax, e->cfg.DT * 0.5);
+                if (err == hipSuccess) err = hipEventRecord(b, e->stream);
+                if (err == hipSuccess) err = hipEventSynchronize(b);
+                float ms = 0;
+                if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+                if (err == hipSuccess && ms < best) {
+                    best = ms;
+                    bs = s;
+                    bl = sl;
+                    bc = chunks[ci];
+                }
             }
         }
     }
@@ -613,9 +628,10 @@ int choose_sweep_shape(af_engine *e)
     if (err != hipSuccess) return hip_fail(err, "tune advection workgroup shape");
     e->sweep_shape = bs;
     e->sweep_slab = bl;
+    e->sweep_chunks = bc;
     if (c.SweepShape == 0 && c.SweepSlab == 0) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_sweep_cache[key] = std::make_pair(bs, bl);
+        g_sweep_cache[key] = std::make_tuple(bs, bl, bc);
     }
     return AF_OK;
 }
@@ -1333,12 +1349,21 @@ int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchun
 int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper)
 {
     if (!e) return fail(AF_ERR_ARG, "aeroflux: null engine");
-    // Gathering shapes: rows x z per thread; LDS-staged: rows x z-chunk.
+    // Gathering shapes: rows x z per thread; LDS-staged: rows x the largest
+    // z-chunk (the NZ+1 face planes split into chunks of sizes that differ
+    // by at most one).
     static const int kRows[] = {1, 1, 2, 2, 4, 4, 8, 16, 4, 4, 8, 8};
     static const int kZ[] = {1, 2, 1, 2, 1, 2, 1, 1, 16, 32, 16, 32};
     if (shape) *shape = e->sweep_shape;
     if (rows) *rows = kRows[e->sweep_shape];
-    if (zper) *zper = kZ[e->sweep_shape];
+    if (zper) {
+        *zper = kZ[e->sweep_shape];
+        if (e->sweep_shape >= 8) {
+            const int n = e->sweep_chunks > 0 ? e->sweep_chunks
+                                              : sweep_lds_default_chunks(e->G, e->sweep_shape - 8);
+            *zper = (e->G.NZ + 1 + n - 1) / n;
+        }
+    }
     return AF_OK;
 }
 

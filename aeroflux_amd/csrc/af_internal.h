@@ -29,6 +29,7 @@ struct SweepArgs {
     int shape = 4;         // workgroup shape, 0 .. sweep_shape_count()-1
     int slab = 1 << 30;    // rows per y-slab of the block order (>= NY+1: plane order)
     int bc = 0;            // also apply applyVelocityBC to the output (body::store_face)
+    int zchunks = 0;       // LDS-staged shapes: z-chunks per tile column (0: the shape's own)
 };
 
 // Number of sweep shapes: gathering shapes (rows per block x z per thread)
@@ -36,6 +37,8 @@ struct SweepArgs {
 int sweep_shape_count();
 int sweep_lds_variant_count();
 hipError_t launch_sweep_lds(int axis, const SweepArgs &a, int variant, hipStream_t s);
+int sweep_lds_chunks(const Geom &G, int variant);
+int sweep_lds_default_chunks(const Geom &G, int variant);
 
 // Launchers.  All enqueue on `s` and return the hipGetLastError() code.
 hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t s);

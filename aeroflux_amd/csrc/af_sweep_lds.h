@@ -164,12 +164,22 @@ struct Stager {
 };
 
 // Tile grid of one sweep: x-tiles of 64 over [g, g+NX), y-tiles of BY
-// rows over [g, g+NY], z-chunks of ZC planes over [g, g+NZ]; and the number
-// of BX*BY-thread blocks that cover the wall column's NY*NZ U faces.
+// rows over [g, g+NY], z-chunks of at most ZC planes over [g, g+NZ]; and
+// the number of BX*BY-thread blocks that cover the wall column's NY*NZ U
+// faces.
 template <int BY, int ZC>
 __host__ __device__ inline int3 tiles(const Geom &G)
 {
     return make_int3((G.NX + BX - 1) / BX, (G.NY + 1 + BY - 1) / BY, (G.NZ + 1 + ZC - 1) / ZC);
+}
+
+// Planes [zs, zend) of z-chunk bz when the NZ+1 face planes [g, g+NZ] are
+// split into nzc chunks whose sizes differ by at most one.
+__device__ __forceinline__ void zchunk(const Geom &G, int nzc, int bz, int &zs, int &zend)
+{
+    const int n = G.NZ + 1;
+    zs = G.g + (int)(((long long)bz * n) / nzc);
+    zend = G.g + (int)(((long long)(bz + 1) * n) / nzc);
 }
 template <int BY>
 __host__ __device__ inline int wall_blocks(const Geom &G)
@@ -222,20 +232,18 @@ __device__ __forceinline__ double far_taps(const DField &s, const body::Departur
                            body::ld2(p + s.sz + s.sy), d);
 }
 
-// One tile (bx, by, bz), computed by the whole workgroup (64 x BY threads)
-// (src_of as for Stager::init):
-// the faces of x in [g+64bx, g+64bx+64) (x < g+NX), y = g+BY*by+ty
-// (y <= g+NY), z over the chunk.  Ends with a barrier, so the caller may
-// reuse `lds` at once.
-template <int AX, int BY, int DD, int ZC, bool BC, class Src>
+// One tile (bx, by) over face planes [zs, zend), computed by the whole
+// workgroup (64 x BY threads) (src_of as for Stager::init): the faces of x
+// in [g+64bx, g+64bx+64) (x < g+NX), y = g+BY*by+ty (y <= g+NY).  Ends with
+// a barrier, so the caller may reuse `lds` at once.
+template <int AX, int BY, int DD, bool BC, class Src>
 __device__ __forceinline__ void tile(const SweepArgs &a, const Src &src_of, lds_t *lds, int bx,
-                                     int by, int bz, bool &bad)
+                                     int by, int zs, int zend, bool &bad)
 {
     using B = Box<AX, BY, DD>;
     const int g = a.G.g;
     const int xe = g + a.G.NX, ye = g + a.G.NY, ze = g + a.G.NZ;
-    const int xs = g + bx * BX, ys = g + by * BY, zs = g + bz * ZC;
-    const int zend = zs + ZC < ze + 1 ? zs + ZC : ze + 1;
+    const int xs = g + bx * BX, ys = g + by * BY;
     const int ox = xs - B::XLO, oy = ys - B::YLO;
     const int x = xs + threadIdx.x, y = ys + threadIdx.y;
     const bool xin = x < xe, yin = y < ye, yok = y <= ye;

@@ -265,7 +265,11 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
             "pass_roofline": {"bound": "hbm", "achieved": pass_bytes / (t_pass * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": pass_bytes / (t_pass * 1e-3) / 1e9 / HBM_PEAK_GBPS,
-                              "algorithmic_bytes_per_launch": pass_bytes},
+                              "algorithmic_bytes_per_launch": pass_bytes,
+                              "launches_per_pass": info["SolveGroups"],
+                              "note": "one colour pass over every slot; with several solve "
+                                      "groups it runs as one launch per group, each group's "
+                                      "passes back to back, and the time is their sum"},
             "stages": {"rbgs_pass_ms": t_pass, "advect_sweep_ms": sw[0],
                        "advect_sweep_xyz_ms": sw, "velocity_bc_ms": t_vbc,
                        "divergence_ms": t_div, "gradient_ms": t_grad, "merge_ms": t_merge,
@@ -276,7 +280,8 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
                                 f"{npass} passes + merge + pressure BC + gradient, which applies the BC)"},
             "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
                       "staged_slots": info["StagedSlots"],
-                      "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]]},
+                      "workgroups": [info["SweepWorkgroups"], info["PassWorkgroups"]],
+                      "solve_groups": info["SolveGroups"]},
             "pool_bytes": info["BytesInUse"],
         }
         if cyc is not None:

@@ -308,9 +308,10 @@ int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles);
 int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchunk);
 
 /* Advection sweep workgroup shape in use: index, rows per workgroup and z
- * positions per thread (64 lanes along x).  Chosen by timing every shape on
- * the engine's grid at New unless SweepShape fixes it.  No reference
- * counterpart. */
+ * positions per thread (64 lanes along x; LDS-staged shapes 8-11: the
+ * largest z-chunk a workgroup walks).  Chosen by timing every shape -- and
+ * for the staged shapes two z-chunk counts -- on the engine's grid at New
+ * unless SweepShape fixes it.  No reference counterpart. */
 int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper);
 
 /* Rows per y-slab of the advection block order in use (-1: plane by plane).
@@ -381,6 +382,15 @@ typedef struct af_batch_config {
                                   snapshots may hold at once (0 -> 32 GiB) */
     int64_t StepCellBudget;    /* policies 2, 3: interior cells one batched step
                                   advances (0 -> 32 Mi cells) */
+    int64_t SolveGroupBytes;   /* pressure solve blocking: slots are split
+                                  into groups of about equal solve-buffer bytes
+                                  (split solution and right-hand side), at most
+                                  this many per group where slot sizes allow,
+                                  and the solve runs group after group so
+                                  that a group's buffers stay in the
+                                  last-level cache across its passes
+                                  (0 -> 224 MiB; -1 -> one group).  Results
+                                  do not depend on it */
 } af_batch_config;
 
 typedef struct af_batch_info {
@@ -396,6 +406,7 @@ typedef struct af_batch_info {
     int64_t MemoryBudget;
     int32_t SweepStaged;       /* LDS-staged variant in use (-1: none) */
     int32_t StagedSlots;       /* resident slots that use it */
+    int32_t SolveGroups;       /* solve groups of the last work-list built */
 } af_batch_info;
 
 int af_batch_new(const af_batch_config *cfg, af_batch **out);

@@ -125,28 +125,58 @@ def test_sparse_solver_flag_is_accepted(lib):
     assert ei.value.code == af.AF_ERR_HIP
 
 
-def test_snapshot_header_layout_matches_c(tmp_path):
-    """af_snapshot_header as gcc lays it out equals the ctypes mirror
-    (128 bytes, every field at the same offset)."""
+# Every struct of the C ABI and its ctypes mirror.
+ABI_STRUCTS = {
+    "af_config": af._CConfig,
+    "af_field": af._CField,
+    "af_snapshot_header": af.SnapshotHeader,
+    "af_batch_config": af._CBatchConfig,
+    "af_batch_info": af._CBatchInfo,
+    "af_serve_job": af._CServeJob,
+    "af_serve_result": af._CServeResult,
+    "af_serve_stats": af._CServeStats,
+}
+
+
+@pytest.mark.parametrize("cname", list(ABI_STRUCTS))
+def test_struct_layout_matches_c(tmp_path, cname):
+    """Each struct as gcc lays it out from include/aeroflux.h equals its
+    ctypes mirror: the same size, the same field names in the same order,
+    every field at the same offset (af_snapshot_header: 128 bytes)."""
     import shutil
     import subprocess
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
-    names = [n for n, _ in af.SnapshotHeader._fields_]
+    mirror = ABI_STRUCTS[cname]
+    names = [n for n, _ in mirror._fields_]
     src = tmp_path / "hdr.c"
     src.write_text(
         '#include <stddef.h>\n#include <stdio.h>\n#include "aeroflux.h"\n'
         "int main(void) {\n"
-        '  printf("%zu\\n", sizeof(af_snapshot_header));\n'
-        + "".join(f'  printf("%zu\\n", offsetof(af_snapshot_header, {n}));\n' for n in names)
+        f'  printf("%zu\\n", sizeof({cname}));\n'
+        + "".join(f'  printf("%zu\\n", offsetof({cname}, {n}));\n' for n in names)
         + "  return 0;\n}\n")
     exe = tmp_path / "hdr"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
                     str(exe)], check=True)
     out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
                                           text=True).stdout.split()]
-    assert out[0] == 128 == C_sizeof(af.SnapshotHeader)
-    assert out[1:] == [getattr(af.SnapshotHeader, n).offset for n in names]
+    assert out[0] == C_sizeof(mirror)
+    assert out[1:] == [getattr(mirror, n).offset for n in names]
+    if cname == "af_snapshot_header":
+        assert out[0] == 128
+    # No C field left out of the mirror: the header's member list, in order.
+    hdr = open(os.path.join(ROOT, "include", "aeroflux.h")).read()
+    body = hdr[hdr.index(f"typedef struct {cname} {{"):]
+    body = body[body.index("{") + 1:body.index(f"}} {cname};")]
+    import re
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    members = []
+    for stmt in body.split(";"):
+        decl = [d.strip() for d in re.sub(r"\[[^\]]*\]", "", stmt).split(",")]
+        if decl[0]:
+            members += [re.findall(r"\w+", d)[-1] for d in decl]
+    assert members == names
 
 
 def C_sizeof(t):
@@ -157,7 +187,8 @@ def C_sizeof(t):
 @pytest.mark.parametrize("kw", [dict(ServePolicy=4), dict(ServePolicy=-1),
                                 dict(ServePolicy=1, PreemptRatio=0.5),
                                 dict(SnapshotBudget=-1),
-                                dict(ServePolicy=2, StepCellBudget=-5)])
+                                dict(ServePolicy=2, StepCellBudget=-5),
+                                dict(SolveGroupBytes=-2)])
 def test_batch_serve_policy_is_validated(lib, kw):
     """Rejected before any device call, so this runs without a GPU."""
     with pytest.raises(af.AeroFluxError) as ei:

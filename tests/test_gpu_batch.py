@@ -74,19 +74,41 @@ PRECISIONS = {
 }
 
 
+def split_bytes(shp, p):
+    nx, ny, nz, _ = shp
+    return 4 * nz * ny * ((nx + 1) // 2) * (4 if p == afm.PRECISION_MIXED else 8)
+
+
+# Solve groups (SolveGroupBytes): the default (every slot in one group at
+# these sizes), about a third of the solve buffers per group, one slot per
+# group.
+GROUPS = {"one_group": 0, "thirds": -3, "each_slot": 1}
+
+
+@pytest.mark.parametrize("groups", list(GROUPS))
 @pytest.mark.parametrize("prec", list(PRECISIONS))
-def test_batch_equals_alone(af, prec):
+def test_batch_equals_alone(af, prec, groups):
     p, bextra, eextra = PRECISIONS[prec]
     steps = 4
     cfgs = [cfg_for(s, i, Precision=p, **eextra) for i, s in enumerate(GRIDS)]
     fs = [fields_for(s, i) for i, s in enumerate(GRIDS)]
-    b = af.Batch(af.BatchConfig(Precision=p, MemoryBudget=256 << 20, **bextra))
+    gb = GROUPS[groups]
+    total = sum(split_bytes(s, p) for s in GRIDS)
+    if gb < 0:
+        gb = (total + 2) // 3
+    b = af.Batch(af.BatchConfig(Precision=p, MemoryBudget=256 << 20, SolveGroupBytes=gb,
+                                **bextra))
     slots = [b.Add(af.Config(**c)) for c in cfgs]
     for s, f in zip(slots, fs):
         load(b, s, f)
     b.StepN(steps)
     info = b.info()
     assert info["Slots"] == len(GRIDS) and info["Tuned"] == 1
+    want = {"one_group": 1, "each_slot": len(GRIDS)}.get(groups)
+    if want is None:
+        assert 2 <= info["SolveGroups"] <= 3, info
+    else:
+        assert info["SolveGroups"] == want, info
     for i, (s, c, f) in enumerate(zip(slots, cfgs, fs)):
         ref, (d, ke), stats = solo(af, c, f, steps)
         assert_same(slot_state(b, s), ref, f"{prec} slot {s} grid {GRIDS[i]}")
@@ -97,10 +119,13 @@ def test_batch_equals_alone(af, prec):
     b.Close()
 
 
-def test_membership_changes_between_steps(af):
+@pytest.mark.parametrize("group_bytes", [0, 100000])
+def test_membership_changes_between_steps(af, group_bytes):
     """Continuous batching: engines join and leave between steps; each one's
-    state equals the same engine stepped alone for the steps it was in."""
-    b = af.Batch(af.BatchConfig(MemoryBudget=256 << 20))
+    state equals the same engine stepped alone for the steps it was in.
+    With 100 kB solve groups the group count changes with the membership
+    (1, 2, 3 ... groups), and each count's step graph is kept and reused."""
+    b = af.Batch(af.BatchConfig(MemoryBudget=256 << 20, SolveGroupBytes=group_bytes))
     cfgs = [cfg_for(s, i) for i, s in enumerate(GRIDS)]
     fs = [fields_for(s, i) for i, s in enumerate(GRIDS)]
     live = {}
@@ -281,6 +306,9 @@ def test_batch_rejects_mismatched_schedule(af):
     assert ei.value.code == afm.AF_ERR_CONFIG
     with pytest.raises(af.AeroFluxError):
         b.Remove(3)
+    # UseSparseSolver is accepted (the red-black solve runs), as by New.
+    s = b.Add(af.Config(**cfg_for((16, 16, 16, 1), 0, UseSparseSolver=True)))
+    b.Remove(s)
     b.Close()
     with pytest.raises(af.AeroFluxError) as ei:
         b.Add(af.Config(**cfg_for((16, 16, 16, 1), 0)))
