@@ -168,6 +168,36 @@ def profile_kernels(af, ic, n: int, reps: int) -> None:
                       "dct_pass_algorithmic_bytes_per_launch": 16 * n ** 3}), flush=True)
 
 
+def profile_batch_pass(af, n_eng: int, reps: int) -> None:
+    """Launch sequence for rocprofv3 counter runs of the batched colour
+    pass: a 512 MiB copy (af_measure_stream kind 1, the calibration kernel,
+    4 launches), then on the 96 uneven engines, in FP64 and in mixed
+    precision, one batched step and one colour pass over every slot timed as
+    the step runs it: group by group, 1 + reps launches per solve group."""
+    from aeroflux_amd import workload
+
+    out = {"profile_batch_pass": True, "copy_bytes": 512 << 20,
+           "copy_gbps": af.MeasureStream(1, 64 << 20, 3), "passes": []}
+    for prec in (af.PRECISION_FP64, af.PRECISION_MIXED):
+        cfgs = [dict(c, Precision=prec) for c in workload.batch_engines(n_eng, seed=3)]
+        need = sum(af.SlotBytes(af.Config(**c), prec) for c in cfgs)
+        b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=need + (64 << 20),
+                                    MaxSlots=n_eng))
+        for c in cfgs:
+            b.Add(af.Config(**c))
+        b.StepN(1)
+        info = b.info()
+        t = b.time_stage(0, reps)
+        cells = sum(c["NX"] * c["NY"] * c["NZ"] for c in cfgs)
+        per = 12 if prec == af.PRECISION_FP64 else 6
+        out["passes"].append({"precision": "fp64" if prec == af.PRECISION_FP64 else "mixed",
+                              "cells": cells, "solve_groups": info["SolveGroups"],
+                              "launches_per_group": reps + 1, "pass_ms": t,
+                              "algorithmic_bytes": per * cells})
+        b.Close()
+    print(json.dumps(out), flush=True)
+
+
 def pct(a, q) -> float:
     return float(np.percentile(np.asarray(a, dtype=np.float64), q)) if len(a) else 0.0
 
@@ -652,6 +682,8 @@ def main() -> int:
                          "may use, capped at 16, the CPU share of one GPU)")
     ap.add_argument("--profile-kernels", action="store_true",
                     help="only run the copy-calibration + dominant-kernel launch sequence")
+    ap.add_argument("--profile-batch-pass", action="store_true",
+                    help="only run the copy-calibration + batched colour pass launch sequence")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched leg, printed as its own JSON object (for profiling "
@@ -696,6 +728,9 @@ def main() -> int:
     n = args.n
     if args.profile_kernels:
         profile_kernels(af, ic, n, 10)
+        return 0
+    if args.profile_batch_pass:
+        profile_batch_pass(af, args.batched_engines, 5)
         return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(af, ic, args.batched_engines,

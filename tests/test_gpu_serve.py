@@ -35,14 +35,18 @@ def alone(af, prec, j):
                                     afm.SERVE_LEAST_WORK_STEPS, afm.SERVE_LEAST_WORK])
 @pytest.mark.parametrize("prec", [afm.PRECISION_FP64, afm.PRECISION_MIXED])
 @pytest.mark.parametrize("realtime", [False, True])
-def test_served_results_equal_alone(af, prec, realtime, policy):
+@pytest.mark.parametrize("group_bytes", [0, 40000])
+def test_served_results_equal_alone(af, prec, realtime, policy, group_bytes):
     jobs = small_stream(14, 5 + prec, rate=200.0)
     biggest = max(af.SlotBytes(af.Config(NX=j["NX"], NY=j["NY"], NZ=j["NZ"], Precision=prec),
                                prec) for j in jobs)
     # policy 2 with a budget of about two jobs' cells: the resident set is
-    # stepped a few jobs at a time, and the subset changes between steps
+    # stepped a few jobs at a time, and the subset changes between steps.
+    # 40 kB solve groups: most resident jobs solve in a group of their own,
+    # and the group count changes from step to step.
     b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=3 * biggest, MaxSlots=16,
-                                ServePolicy=policy, StepCellBudget=2 * 24 ** 3))
+                                ServePolicy=policy, StepCellBudget=2 * 24 ** 3,
+                                SolveGroupBytes=group_bytes))
     res, st = af.Serve(b, jobs, realtime=realtime)
     assert st["Steps"] > 0 and 1 <= st["MaxResident"] <= len(jobs)
     assert st["PeakBytes"] <= 3 * biggest
@@ -111,17 +115,19 @@ def test_preempted_jobs_resume_with_the_same_results(af, prec):
     b.Close()
 
 
-def test_checked_served_jobs_meet_their_targets(af):
+@pytest.mark.parametrize("group_bytes", [0, 40000])
+def test_checked_served_jobs_meet_their_targets(af, group_bytes):
     """A checked mixed batch serves jobs that each carry a max|div| target
     (here the job's own final max|div| on the FP64 path): every job ends at
-    or below it, and equals the same checked engine run alone."""
+    or below it, and equals the same checked engine run alone, with one
+    solve group or several."""
     jobs = small_stream(10, 13, rate=300.0)
     fix = dict(MixedSweeps=24, MixedFixSweeps=1, MixedMaxCycles=2)
     for j in jobs:
         j["MixedTarget"] = alone(af, afm.PRECISION_FP64, j)[0]
     prec = afm.PRECISION_MIXED
     b = af.Batch(af.BatchConfig(Precision=prec, MemoryBudget=256 << 20, MaxSlots=16,
-                                MixedChecked=1, **fix))
+                                MixedChecked=1, SolveGroupBytes=group_bytes, **fix))
     res, st = af.Serve(b, jobs, realtime=True)
     assert st["CellUpdates"] == sum(j["NX"] * j["NY"] * j["NZ"] * j["Steps"] for j in jobs)
     b.Close()
