@@ -88,6 +88,7 @@ class _CBatchConfig(C.Structure):
         ("SweepShape", C.c_int32), ("Workgroups", C.c_int32), ("SweepStaged", C.c_int32),
         ("ServePolicy", C.c_int32), ("PreemptRatio", C.c_double), ("SnapshotBudget", C.c_int64),
         ("StepCellBudget", C.c_int64), ("SolveGroupBytes", C.c_int64),
+        ("StepMaxWait", C.c_int32),
     ]
 
 
@@ -140,6 +141,7 @@ class _CServeResult(C.Structure):
     _fields_ = [
         ("Admitted", C.c_double), ("Done", C.c_double), ("MaxDivergence", C.c_double),
         ("KineticEnergy", C.c_double), ("Status", C.c_int32), ("Slot", C.c_int32),
+        ("FirstStep", C.c_int64), ("DoneStep", C.c_int64),
     ]
 
 
@@ -619,6 +621,7 @@ class BatchConfig:
     SnapshotBudget: int = 0
     StepCellBudget: int = 0
     SolveGroupBytes: int = 0
+    StepMaxWait: int = 0
 
     def _to_c(self) -> _CBatchConfig:
         c = _CBatchConfig()
@@ -808,7 +811,8 @@ def Serve(batch: Batch, jobs: list[dict], kmax: int = 8, slope: float = -5.0 / 3
     (af_serve).  Each job is a dict with NX, NY, NZ, Steps, Viscosity, DT,
     H, Seed, Arrival (seconds; sorted) and, for a checked mixed batch,
     MixedTarget.  Returns per-job results (Admitted,
-    Done, MaxDivergence, KineticEnergy, Status, Slot) and run statistics.
+    Done, MaxDivergence, KineticEnergy, Status, Slot, FirstStep, DoneStep)
+    and run statistics.
     progress(stats_so_far, jobs_done), if given, is called between batched
     steps at most once per progress_interval seconds."""
     n = len(jobs)

@@ -1179,6 +1179,8 @@ int af_batch_new(const af_batch_config *cfg_in, af_batch **out)
     if (c.SolveGroupBytes == 0) c.SolveGroupBytes = kDefaultSolveGroupBytes;
     if (c.SolveGroupBytes < -1)
         return fail(AF_ERR_CONFIG, "aeroflux: SolveGroupBytes must be -1, 0 or positive");
+    if (c.StepMaxWait < 0)
+        return fail(AF_ERR_CONFIG, "aeroflux: StepMaxWait must be 0 (no bound) or positive");
 
     af_batch *b = new (std::nothrow) af_batch();
     if (!b) return fail(AF_ERR_OOM, "aeroflux: out of host memory");
@@ -1733,6 +1735,7 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         int64_t job;
         int slot;
         int64_t left;
+        int64_t idle;  // consecutive batched steps that passed it over
     };
     std::vector<Res> resident;
     std::deque<std::pair<int64_t, std::vector<int64_t>>> pending;
@@ -1792,6 +1795,8 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         out[i] = af_serve_result{};
         out[i].Slot = -1;
         out[i].Status = status[i];
+        out[i].FirstStep = -1;
+        out[i].DoneStep = -1;
     }
     // Policy 1: snapshot the resident job with the most remaining work to
     // host memory, free its slot and requeue it, if it has more than
@@ -1905,7 +1910,7 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
                 out[j].Admitted = t;
             }
             out[j].Slot = slot;
-            resident.push_back({j, slot, left});
+            resident.push_back({j, slot, left, 0});
             queue.erase(queue.begin());
             st.MaxResident = std::max<int32_t>(st.MaxResident, (int32_t)resident.size());
             st.PeakBytes = std::max(st.PeakBytes, b->pool.used);
@@ -1949,15 +1954,22 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
             }
             continue;
         }
-        // Policy 2: this step advances the resident jobs with the least
-        // remaining work (cells x steps left, ties by arrival), as many as
-        // fit the cell budget, at least one; a job that does not fit is
-        // passed over for smaller ones behind it.  Every job still takes
-        // exactly its own steps, so results do not depend on the choice.
+        // Policies 2 and 3: this step advances the resident jobs with the
+        // least remaining work (cells x steps left, ties by arrival), as many
+        // as fit the cell budget, at least one; a job that does not fit is
+        // passed over for smaller ones behind it.  With StepMaxWait > 0, a
+        // job passed over by that many consecutive steps goes first (longest
+        // wait first), which bounds how long a large job can be held back.
+        // Every job still takes exactly its own steps, so results do not
+        // depend on the choice.
         if (by_steps) {
             order.clear();
-            for (size_t r = 0; r < resident.size(); r++)
-                order.push_back({work_of(resident[r].job, resident[r].left), (int64_t)r});
+            const int64_t max_wait = b->cfg.StepMaxWait;
+            for (size_t r = 0; r < resident.size(); r++) {
+                const Res &x = resident[r];
+                const bool starved = max_wait > 0 && x.idle >= max_wait;
+                order.push_back({starved ? -(double)x.idle : work_of(x.job, x.left), (int64_t)r});
+            }
             std::sort(order.begin(), order.end(), [&](const auto &p, const auto &q) {
                 return p.first != q.first ? p.first < q.first
                                           : resident[(size_t)p.second].job <
@@ -2007,16 +2019,24 @@ int af_serve(af_batch *b, const af_serve_job *jobs, int64_t n, int32_t kmax, dou
         if ((err = hipGraphLaunch(b->graph, b->stream)) != hipSuccess) break;
         if ((err = hipEventRecord(ev[step % R], b->stream)) != hipSuccess) break;
         std::vector<int64_t> done_now;
-        for (size_t r = 0; r < resident.size();)
-            if (by_steps && !b->step_mask[(size_t)resident[r].slot]) {
-                r++;  // not advanced by this step
-            } else if (--resident[r].left == 0) {
-                done_now.push_back(resident[r].job);
+        for (size_t r = 0; r < resident.size();) {
+            Res &x = resident[r];
+            if (by_steps && !b->step_mask[(size_t)x.slot]) {
+                x.idle++;  // not advanced by this step
+                r++;
+                continue;
+            }
+            x.idle = 0;
+            if (out[x.job].FirstStep < 0) out[x.job].FirstStep = step;
+            if (--x.left == 0) {
+                out[x.job].DoneStep = step;
+                done_now.push_back(x.job);
                 std::swap(resident[r], resident.back());
                 resident.pop_back();
             } else {
                 r++;
             }
+        }
         if ((err = retire(done_now)) != hipSuccess) break;
         pending.emplace_back(step, std::move(done_now));
         step++;

@@ -391,6 +391,11 @@ typedef struct af_batch_config {
                                   last-level cache across its passes
                                   (0 -> 224 MiB; -1 -> one group).  Results
                                   do not depend on it */
+    int32_t StepMaxWait;       /* policies 2, 3: a resident job passed over by
+                                  this many consecutive batched steps goes
+                                  first in the next one, longest wait first
+                                  (0 -> no bound: a large job may wait while
+                                  smaller ones keep it out of the budget) */
 } af_batch_config;
 
 typedef struct af_batch_info {
@@ -469,10 +474,12 @@ int af_batch_time_stage(af_batch *b, int stage, int reps, double *avg_ms);
 /* -------------------------------------------------------------- serving */
 /* A request loop over a batch (no reference counterpart).  Step-N jobs of
  * different sizes arrive over time; each is admitted as a slot as soon as
- * the pool can place it (first come, first served), gets a seeded
- * band-limited initial velocity on the device, and is stepped with every
- * other resident job by the batched step graph (continuous batching: the
- * resident set changes between steps, while earlier steps still run).  When
+ * the pool can place it (in the order of the batch's ServePolicy: first
+ * come first served, or least remaining work first), gets a seeded
+ * band-limited initial velocity on the device, and is stepped with the
+ * other resident jobs by the batched step graph (continuous batching: the
+ * resident set changes between steps, while earlier steps still run; under
+ * policies 2 and 3 each step advances a least-work subset of them).  When
  * a job's N steps are done its final MaxDivergence and KineticEnergy are
  * recorded and its block returns to the pool.  Every job's result equals
  * the same engine stepped alone (af_new, af_init_band_limited, af_step_n). */
@@ -497,6 +504,10 @@ typedef struct af_serve_result {
     int32_t Status;            /* AF_OK, AF_ERR_FAULT, or AF_ERR_OOM if the
                                   job can never fit the pool */
     int32_t Slot;
+    int64_t FirstStep;         /* index of the batched step that first advanced
+                                  the job (0-based; -1 if never) */
+    int64_t DoneStep;          /* index of the batched step that ran its last
+                                  step (-1 if never) */
 } af_serve_result;
 
 typedef struct af_serve_stats {

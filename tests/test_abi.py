@@ -188,7 +188,7 @@ def C_sizeof(t):
                                 dict(ServePolicy=1, PreemptRatio=0.5),
                                 dict(SnapshotBudget=-1),
                                 dict(ServePolicy=2, StepCellBudget=-5),
-                                dict(SolveGroupBytes=-2)])
+                                dict(SolveGroupBytes=-2), dict(StepMaxWait=-1)])
 def test_batch_serve_policy_is_validated(lib, kw):
     """Rejected before any device call, so this runs without a GPU."""
     with pytest.raises(af.AeroFluxError) as ei:

@@ -155,6 +155,8 @@ def test_least_work_stepping_finishes_short_jobs_first(af):
     res, st = af.Serve(b, jobs, realtime=False)
     assert st["Steps"] == 16 and st["MaxResident"] == 4
     assert res[3]["Done"] <= res[1]["Done"] <= res[2]["Done"] <= res[0]["Done"]
+    # batched step indices: 1 step at 0, 2 steps at 1-2, 5 at 3-7, 8 at 8-15
+    assert [(r["FirstStep"], r["DoneStep"]) for r in res] == [(8, 15), (1, 2), (3, 7), (0, 0)]
     # after the run every slot advances again
     s = b.Add(af.Config(NX=8, NY=8, NZ=8))
     b.StepN(1)
@@ -163,6 +165,30 @@ def test_least_work_stepping_finishes_short_jobs_first(af):
     for j, r in zip(jobs, res):
         assert r["Status"] == afm.AF_OK
         assert (r["MaxDivergence"], r["KineticEnergy"]) == alone(af, afm.PRECISION_FP64, j)
+
+
+def test_step_max_wait_bounds_how_long_a_job_is_passed_over(af):
+    """The same four jobs (8, 2, 5 and 1 steps, a budget of one job per
+    batched step) with StepMaxWait = 2: a job passed over by two consecutive
+    steps goes first in the next (longest wait first, ties by arrival), so
+    the 8-step job first advances at step 2 instead of 8 and the 5-step job
+    at step 3; the run still takes 16 batched steps, with every result equal
+    to the job run alone."""
+    jobs = [dict(NX=16, NY=16, NZ=16, H=1 / 16, Viscosity=1e-4, DT=0.004, Steps=s, Seed=30 + i,
+                 Arrival=0.0) for i, s in enumerate((8, 2, 5, 1))]
+    for policy in (afm.SERVE_LEAST_WORK_STEPS, afm.SERVE_LEAST_WORK):
+        b = af.Batch(af.BatchConfig(MemoryBudget=64 << 20, MaxSlots=8, ServePolicy=policy,
+                                    StepCellBudget=16 ** 3, StepMaxWait=2))
+        res, st = af.Serve(b, jobs, realtime=False)
+        b.Close()
+        assert st["Steps"] == 16 and st["MaxResident"] == 4
+        assert res[3]["FirstStep"] == res[3]["DoneStep"] == 0
+        assert res[0]["FirstStep"] == 2
+        assert all(0 <= r["FirstStep"] <= 3 for r in res)
+        assert max(r["DoneStep"] for r in res) == 15
+        for j, r in zip(jobs, res):
+            assert r["Status"] == afm.AF_OK
+            assert (r["MaxDivergence"], r["KineticEnergy"]) == alone(af, afm.PRECISION_FP64, j)
 
 
 def test_serve_progress_reports(af):
