@@ -29,6 +29,7 @@
 // axes).  The tiles cover faces with x < g+NX; the U faces of the wall
 // column x = g+NX are computed by extra workgroups from global memory.
 #include "af_sweep_lds.h"
+#include "af_sweep_stage.h"
 #include "af_xcd.h"
 
 #include <climits>
@@ -43,7 +44,7 @@ constexpr int BX = body::BX;
 
 // nb = (x-tiles, y-tiles, z-chunks); the z-chunks split each tile column's
 // face planes evenly (slds::zchunk).
-template <int AX, int BY, int D>
+template <int AX, int BY, int D, bool BC>
 __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int ncol)
 {
     using B = slds::Box<AX, BY, D>;
@@ -51,16 +52,64 @@ __global__ __launch_bounds__(BX *BY) void k_sweep_lds(SweepArgs a, int3 nb, int 
     const int ntile = nb.x * nb.y * nb.z;
     bool bad = false;
     if ((int)blockIdx.x >= ntile) {
-        slds::wall_face<AX, false>(
+        slds::wall_face<AX, BC>(
             a, ((int)blockIdx.x - ntile) * (BX * BY) + threadIdx.y * BX + threadIdx.x, bad);
     } else {
         int bx, by, bz, zs, zend;
         xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
         slds::zchunk(a.G, nb.z, bz, zs, zend);
         auto src_of = [&](int c) -> const DField & { return c == 0 ? a.U : (c == 1 ? a.V : a.W); };
-        slds::tile<AX, BY, D, false>(a, src_of, (slds::lds_t *)lds_mem, bx, by, zs, zend, bad);
+        slds::tile<AX, BY, D, BC>(a, src_of, (slds::lds_t *)lds_mem, bx, by, zs, zend, bad);
     }
     if (bad) *body::gp(a.flag) = 1;
+}
+
+// The same tiles with per-field staged boxes (af_sweep_stage.h).
+// Three 8-wave workgroups per CU fit the LDS of every axis; the register
+// budget for six waves per SIMD (80) is asked for explicitly.
+template <int AX, int BY, bool BC>
+__global__ __launch_bounds__(BX *BY) __attribute__((amdgpu_waves_per_eu(6))) void k_sweep_stg(
+    SweepArgs a, int3 nb, int ncol)
+{
+    using L = stg::Layout<AX, BY>;
+    __shared__ __attribute__((aligned(16))) double lds_mem[L::TOTAL];
+    const int ntile = nb.x * nb.y * nb.z;
+    bool bad = false;
+    if ((int)blockIdx.x >= ntile) {
+        slds::wall_face<AX, BC>(
+            a, ((int)blockIdx.x - ntile) * (BX * BY) + threadIdx.y * BX + threadIdx.x, bad);
+    } else {
+        int bx, by, bz, zs, zend;
+        xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+        slds::zchunk(a.G, nb.z, bz, zs, zend);
+        auto src_of = [&](int c) -> const DField & { return c == 0 ? a.U : (c == 1 ? a.V : a.W); };
+        stg::tile<AX, BY, BC>(a, src_of, (slds::lds_t *)lds_mem, bx, by, zs, zend, bad);
+    }
+    if (bad) *body::gp(a.flag) = 1;
+}
+
+template <int BY, int ZC>
+hipError_t launch_stg(int axis, const SweepArgs &a, hipStream_t s)
+{
+    int3 nb = slds::tiles<BY, ZC>(a.G);
+    if (a.zchunks > 0) nb.z = a.zchunks < a.G.NZ + 1 ? a.zchunks : a.G.NZ + 1;
+    const int ncol = slds::wall_blocks<BY>(a.G);
+    const dim3 grid((unsigned)(nb.x * nb.y * nb.z + ncol));
+    const dim3 block(BX, BY);
+    if (a.bc) {
+        switch (axis) {
+        case 0: hipLaunchKernelGGL((k_sweep_stg<0, BY, true>), grid, block, 0, s, a, nb, ncol); break;
+        case 1: hipLaunchKernelGGL((k_sweep_stg<1, BY, true>), grid, block, 0, s, a, nb, ncol); break;
+        default: hipLaunchKernelGGL((k_sweep_stg<2, BY, true>), grid, block, 0, s, a, nb, ncol); break;
+        }
+    } else {
+        switch (axis) {
+        case 0: hipLaunchKernelGGL((k_sweep_stg<0, BY, false>), grid, block, 0, s, a, nb, ncol); break;
+        case 1: hipLaunchKernelGGL((k_sweep_stg<1, BY, false>), grid, block, 0, s, a, nb, ncol); break;
+        default: hipLaunchKernelGGL((k_sweep_stg<2, BY, false>), grid, block, 0, s, a, nb, ncol); break;
+        }
+    }
+    return hipGetLastError();
 }
 
 template <int BY, int D, int ZC>
@@ -71,34 +120,48 @@ hipError_t launch_lds(int axis, const SweepArgs &a, hipStream_t s)
     const int ncol = slds::wall_blocks<BY>(a.G);
     const dim3 grid((unsigned)(nb.x * nb.y * nb.z + ncol));
     const dim3 block(BX, BY);
-    switch (axis) {
-    case 0: hipLaunchKernelGGL((k_sweep_lds<0, BY, D>), grid, block, 0, s, a, nb, ncol); break;
-    case 1: hipLaunchKernelGGL((k_sweep_lds<1, BY, D>), grid, block, 0, s, a, nb, ncol); break;
-    default: hipLaunchKernelGGL((k_sweep_lds<2, BY, D>), grid, block, 0, s, a, nb, ncol); break;
+    if (a.bc) {
+        switch (axis) {
+        case 0: hipLaunchKernelGGL((k_sweep_lds<0, BY, D, true>), grid, block, 0, s, a, nb, ncol); break;
+        case 1: hipLaunchKernelGGL((k_sweep_lds<1, BY, D, true>), grid, block, 0, s, a, nb, ncol); break;
+        default: hipLaunchKernelGGL((k_sweep_lds<2, BY, D, true>), grid, block, 0, s, a, nb, ncol); break;
+        }
+    } else {
+        switch (axis) {
+        case 0: hipLaunchKernelGGL((k_sweep_lds<0, BY, D, false>), grid, block, 0, s, a, nb, ncol); break;
+        case 1: hipLaunchKernelGGL((k_sweep_lds<1, BY, D, false>), grid, block, 0, s, a, nb, ncol); break;
+        default: hipLaunchKernelGGL((k_sweep_lds<2, BY, D, false>), grid, block, 0, s, a, nb, ncol); break;
+        }
     }
     return hipGetLastError();
 }
 
 // Resident workgroups per CU of variant (BY, D)'s X-sweep kernel, and the
 // device's CU count (0 on error).
-template <int BY, int D>
-int lds_slots(int *cus)
+template <class K>
+int kernel_slots(K kernel, int threads, int *cus)
 {
     int dev = 0, n = 0;
     *cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_sweep_lds<0, BY, D>, BX * BY, 0) !=
-            hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess)
         return 0;
     return n;
 }
 
+template <int BY, int D>
+int lds_slots(int *cus, bool staged)
+{
+    return staged ? kernel_slots(k_sweep_stg<0, BY, true>, BX * BY, cus)
+                  : kernel_slots(k_sweep_lds<0, BY, D, true>, BX * BY, cus);
+}
+
 template <int BY, int D, int ZC>
-int lds_chunks(const Geom &G)
+int lds_chunks(const Geom &G, bool staged = false)
 {
     int cus = 0;
-    const int occ = lds_slots<BY, D>(&cus);
+    const int occ = lds_slots<BY, D>(&cus, staged);
     const int planes = G.NZ + 1;
     const int3 nb = slds::tiles<BY, ZC>(G);
     if (occ <= 0 || cus < kXcds) return nb.z;
@@ -121,7 +184,7 @@ int lds_chunks(const Geom &G)
 
 }  // namespace
 
-int sweep_lds_variant_count() { return 4; }
+int sweep_lds_variant_count() { return 6; }
 
 // The z-chunk count per tile column of LDS-staged variant `variant` on grid
 // G whose tiles best fill whole rounds of the workgroups each XCD holds at
@@ -134,14 +197,16 @@ int sweep_lds_chunks(const Geom &G, int variant)
     case 0: return lds_chunks<4, 2, 16>(G);
     case 1: return lds_chunks<4, 2, 32>(G);
     case 2: return lds_chunks<8, 2, 16>(G);
-    default: return lds_chunks<8, 2, 32>(G);
+    case 3: return lds_chunks<8, 2, 32>(G);
+    case 4: return lds_chunks<8, 2, 16>(G, true);
+    default: return lds_chunks<8, 2, 32>(G, true);
     }
 }
 
 // The shape's own chunk count: z-chunks of at most ZC planes.
 int sweep_lds_default_chunks(const Geom &G, int variant)
 {
-    const int zc = variant == 1 || variant == 3 ? 32 : 16;
+    const int zc = variant == 1 || variant == 3 || variant == 5 ? 32 : 16;
     return (G.NZ + 1 + zc - 1) / zc;
 }
 
@@ -154,6 +219,8 @@ hipError_t launch_sweep_lds(int axis, const SweepArgs &a, int variant, hipStream
     case 1: return launch_lds<4, 2, 32>(axis, a, s);
     case 2: return launch_lds<8, 2, 16>(axis, a, s);
     case 3: return launch_lds<8, 2, 32>(axis, a, s);
+    case 4: return launch_stg<8, 16>(axis, a, s);
+    case 5: return launch_stg<8, 32>(axis, a, s);
     default: return hipErrorInvalidValue;
     }
 }

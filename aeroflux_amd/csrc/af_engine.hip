@@ -182,7 +182,9 @@ hipError_t enq_velocity_bc(af_engine *e, const Desc &d)
                               dfield(e, d.f[AF_FIELD_W]), e->G, e->stream);
 }
 
-hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
+// With bc the sweep also applies applyVelocityBC to its output
+// (body::store_face_bc), which the step runs after every sweep.
+hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt, bool bc = false)
 {
     SweepArgs a;
     a.U = dfield(e, d.f[AF_FIELD_U]);
@@ -200,6 +202,7 @@ hipError_t enq_sweep(af_engine *e, const Desc &d, int axis, double dt)
     a.shape = e->sweep_shape;
     a.slab = e->sweep_slab;
     a.zchunks = e->sweep_chunks;
+    a.bc = bc ? 1 : 0;
     return launch_sweep(axis, a, e->stream);
 }
 
@@ -360,10 +363,11 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 
 // project, project.go:12-34.  The gradient subtraction applies the closing
 // velocity BC itself (body::gradient_cell), which is the same values as the
-// separate launch.
-hipError_t enq_project(af_engine *e, const Desc &d)
+// separate launch.  bc_applied: the velocity already holds the opening BC
+// (the sweep before it applied the BC to its output).
+hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false)
 {
-    hipError_t err = enq_velocity_bc(e, d);
+    hipError_t err = bc_applied ? hipSuccess : enq_velocity_bc(e, d);
     if (e->direct.method == DIRECT_FFT) {
         // The direct solve's first pass computes the divergence itself (the
         // same values, stored to Div), which saves re-reading Div.
@@ -389,25 +393,26 @@ hipError_t enq_step(af_engine *e, Desc &d)
     static const int rev[3] = {AF_AXIS_Z, AF_AXIS_Y, AF_AXIS_X};
     const double half = e->cfg.DT * 0.5;
     hipError_t err = hipSuccess;
+    // Every sweep is followed by applyVelocityBC: the next sweep's, or the
+    // one that opens project (project.go:13).  Each sweep applies it to its
+    // own output (body::store_face_bc), so only the step's first BC is
+    // launched.  The BC writes only cells it never reads, so applying it a
+    // second time would leave the same values.  The projection ends with
+    // the BC (applied by the gradient subtraction) and nothing touches the
+    // velocity before the reverse sweeps' first BC, which is skipped too.
+    // Whole planes of V and W, ghosts included, are then written in one
+    // pass, which also keeps the sweep's row writes to whole cache lines.
+    err = enq_velocity_bc(e, d);
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
-        err = enq_velocity_bc(e, d);
-        if (err == hipSuccess) err = enq_sweep(e, d, fwd[k], half);
+        err = enq_sweep(e, d, fwd[k], half, true);
         commit(d);
     }
-    if (err == hipSuccess) err = enq_project(e, d);
-    // The projection ends with applyVelocityBC and nothing touches the
-    // velocity before the reverse sweeps' first applyVelocityBC.  The BC
-    // writes only cells it never reads, so applying it twice leaves the
-    // values of applying it once: the second application is skipped.  (The
-    // sweeps could apply the BC to their own output, as the batched sweeps
-    // do, but the extra registers cost the single engine's LDS-staged sweep
-    // more occupancy than the BC launches cost.)
+    if (err == hipSuccess) err = enq_project(e, d, true);
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
-        if (k > 0) err = enq_velocity_bc(e, d);
-        if (err == hipSuccess) err = enq_sweep(e, d, rev[k], half);
+        err = enq_sweep(e, d, rev[k], half, true);
         commit(d);
     }
-    if (err == hipSuccess) err = enq_project(e, d);
+    if (err == hipSuccess) err = enq_project(e, d, true);
     return err;
 }
 
@@ -552,8 +557,9 @@ int tune_blocked(af_engine *e, int k, Tile *out)
 // that best fills whole rounds of resident workgroups): every choice gives
 // the same bits (faces are independent), so the fastest on this grid is
 // kept.  Timed on the engine's own (zeroed) arena, X, Y and Z sweeps twice
-// each (the sweeps only write the Tmp fields, which are still zero
-// afterwards: advecting zeros gives +0); cached per grid for the process.
+// each, as the step launches them (with the velocity BC); the sweeps write
+// only the Tmp fields, whose BC ghosts then hold -0, so the Tmp fields are
+// zeroed again afterwards.  Cached per grid for the process.
 std::map<std::tuple<int, int, int, int>, std::tuple<int, int, int>> g_sweep_cache;
 
 constexpr int kPlaneOrder = 1 << 30;
@@ -605,11 +611,11 @@ int choose_sweep_shape(af_engine *e)
                 e->sweep_shape = s;
                 e->sweep_slab = sl;
                 e->sweep_chunks = chunks[ci];
-                err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5);
+                err = enq_sweep(e, e->cur, AF_AXIS_X, e->cfg.DT * 0.5, true);
                 if (err == hipSuccess) err = hipEventRecord(a, e->stream);
                 for (int r = 0; r < 2 && err == hipSuccess; r++)
                     for (int ax = 0; ax < 3 && err == hipSuccess; ax++)
-                        err = enq_sweep(e, e->cur, ax, e->cfg.DT * 0.5);
+                        err = enq_sweep(e, e->cur, ax, e->cfg.DT * 0.5, true);
                 if (err == hipSuccess) err = hipEventRecord(b, e->stream);
                 if (err == hipSuccess) err = hipEventSynchronize(b);
                 float ms = 0;
@@ -625,6 +631,11 @@ int choose_sweep_shape(af_engine *e)
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    for (int f = AF_FIELD_TMP0; f <= AF_FIELD_TMP2 && err == hipSuccess; f++) {
+        const DField t = dfield(e, e->cur.f[f]);
+        err = hipMemsetAsync(t.d, 0, (size_t)t.sz * t.nz * sizeof(double), e->stream);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
     if (err != hipSuccess) return hip_fail(err, "tune advection workgroup shape");
     e->sweep_shape = bs;
     e->sweep_slab = bl;
@@ -1352,8 +1363,12 @@ int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *z
     // Gathering shapes: rows x z per thread; LDS-staged: rows x the largest
     // z-chunk (the NZ+1 face planes split into chunks of sizes that differ
     // by at most one).
-    static const int kRows[] = {1, 1, 2, 2, 4, 4, 8, 16, 4, 4, 8, 8};
-    static const int kZ[] = {1, 2, 1, 2, 1, 2, 1, 1, 16, 32, 16, 32};
+    static const int kRows[] = {1, 1, 2, 2, 4, 4, 8, 16, 4, 4, 8, 8, 8, 8};
+    static const int kZ[] = {1, 2, 1, 2, 1, 2, 1, 1, 16, 32, 16, 32, 16, 32};
+    static_assert(sizeof(kRows) == sizeof(kZ), "one entry per shape");
+    if (e->sweep_shape < 0 || e->sweep_shape >= (int)(sizeof(kRows) / sizeof(kRows[0])) ||
+        e->sweep_shape >= sweep_shape_count())
+        return fail(AF_ERR_ARG, "aeroflux: sweep shape %d has no table entry", e->sweep_shape);
     if (shape) *shape = e->sweep_shape;
     if (rows) *rows = kRows[e->sweep_shape];
     if (zper) {
@@ -1433,9 +1448,9 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms)
                 return launch_rbgs_tb<double>(a, e->tile64.k, e->tile64.rows, false, e->stream);
             }
             return launch_rbgs(P, D, e->G, e->h2, AF_COLOR_RED, true, e->stream);
-        case 1: return enq_sweep(e, d, AF_AXIS_X, e->cfg.DT * 0.5);
-        case 4: return enq_sweep(e, d, AF_AXIS_Y, e->cfg.DT * 0.5);
-        case 5: return enq_sweep(e, d, AF_AXIS_Z, e->cfg.DT * 0.5);
+        case 1: return enq_sweep(e, d, AF_AXIS_X, e->cfg.DT * 0.5, true);
+        case 4: return enq_sweep(e, d, AF_AXIS_Y, e->cfg.DT * 0.5, true);
+        case 5: return enq_sweep(e, d, AF_AXIS_Z, e->cfg.DT * 0.5, true);
         case 2: return enq_divergence(e, d);
         case 3: return enq_gradient(e, d);
         case 6: return enq_solve(e, d);

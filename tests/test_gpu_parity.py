@@ -502,7 +502,7 @@ def test_mixed_precision_is_reproducible(af):
     assert outs[0] == outs[1]
 
 
-@pytest.mark.parametrize("shape", range(1, 13))
+@pytest.mark.parametrize("shape", range(1, 15))
 def test_step_bitwise_across_sweep_shapes(af, oracle_lib, shape):
     """Every advection workgroup shape gives the reference's bits."""
     cfg = dict(NX=20, NY=18, NZ=14, H=1 / 20, DT=0.01, Viscosity=1e-4, GaussSeidelIters=6)
@@ -547,7 +547,7 @@ def test_step_bitwise_across_pass_orders(af, oracle_lib, slab):
     assert mixed[0] == mixed[1], f"mixed pass slab {slab}"
 
 
-LDS_SHAPES = range(9, 13)  # the LDS-staged sweep variants (SweepShape 9..12)
+LDS_SHAPES = range(9, 15)  # the LDS-staged sweep variants (SweepShape 9..14)
 
 
 @pytest.mark.parametrize("shape", LDS_SHAPES)
