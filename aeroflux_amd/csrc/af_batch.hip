@@ -34,6 +34,7 @@
 #include "af_bodies.h"
 #include "af_host.h"
 #include "af_sweep_lds.h"
+#include "af_sweep_stage.h"
 #include "af_ic.h"
 #include "af_internal.h"
 #include "af_snapshot.h"
@@ -87,9 +88,11 @@ constexpr int kShapes = 8;
 constexpr int kShapeRows[kShapes] = {1, 1, 2, 2, 4, 4, 8, 16};
 constexpr int kShapeZ[kShapes] = {1, 2, 1, 2, 1, 2, 1, 1};
 
-// LDS-staged sweep variants (af_sweep_lds.h): rows per workgroup and
-// z-planes per chunk; staged reach two cells along x and y, one along z.
-constexpr int kLdsVariants = 4;
+// LDS-staged sweep variants: rows per workgroup and z-planes per chunk.
+// 0-3 stage one box of the three fields' union (af_sweep_lds.h: reach two
+// cells along x and y, one along z); 4-7 stage each field's own box
+// (af_sweep_stage.h).
+constexpr int kLdsVariants = 8;
 
 // A sweep plan: the gather shape for every slot, and the LDS-staged variant
 // (-1: none) for the slots lds_slot accepts, as one integer.
@@ -116,9 +119,9 @@ inline int64_t lds_items_t(const Geom &G)
 inline int64_t lds_items(const Geom &G, int v)
 {
     switch (v) {
-    case 0: return lds_items_t<4, 16>(G);
-    case 1: return lds_items_t<4, 32>(G);
-    case 2: return lds_items_t<8, 16>(G);
+    case 0: case 6: return lds_items_t<4, 16>(G);
+    case 1: case 7: return lds_items_t<4, 32>(G);
+    case 2: case 4: return lds_items_t<8, 16>(G);
     default: return lds_items_t<8, 32>(G);
     }
 }
@@ -340,6 +343,58 @@ __global__ __launch_bounds__(BX *LBY) void kb_sweep_lds(BCtl c, int sw)
         slds::zchunk(e.G, nb.z, rest / nb.y, zs, zend);
         slds::tile<AX, LBY, 2, true>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
                                      rest % nb.y, zs, zend, bad);
+    }
+    if (bad) *body::gp(sa.flag) = 1;
+}
+
+// The same work-list with per-field staged boxes (af_sweep_stage.h).  Six
+// waves per SIMD are asked for where the registers allow it (the 4-row Y
+// sweep stages two pairs of its velocity field per thread).
+template <int AX, int LBY, int ZC>
+__global__ __launch_bounds__(BX *LBY) __attribute__((amdgpu_waves_per_eu(
+    AX == 1 && LBY == 4 ? 4 : 6))) void kb_sweep_stg(
+    BCtl c, int sw)
+{
+    using L = stg::Layout<AX, LBY>;
+    __shared__ __attribute__((aligned(16))) double lds_mem[L::TOTAL];
+    const int *tab = c.tab;
+    const int nact = tab[0], stride = tab[1];
+    const int *act = tab + 2;
+    const int *first = tab + 2 + stride * (1 + K_LDS);
+    const int total = first[nact];
+    const int per = (total + XCDS - 1) / XCDS;
+    const int k = (int)(blockIdx.x / XCDS);
+    const int i = (int)(blockIdx.x % XCDS) * per + k;
+    if (k >= per || i >= total) return;
+    int a = 0, r = nact - 1;  // largest a with first[a] <= i
+    while (a < r) {
+        const int m = (a + r + 1) >> 1;
+        if (first[m] <= i) a = m; else r = m - 1;
+    }
+    const int s = act[a], t = i - first[a];
+    const BSlot &e = c.slots[s];
+    SweepArgs sa;
+    sa.U = velf(e, sw, 0); sa.V = velf(e, sw, 1); sa.W = velf(e, sw, 2);
+    sa.T0 = tmpf(e, sw, 0); sa.T1 = tmpf(e, sw, 1); sa.T2 = tmpf(e, sw, 2);
+    sa.G = e.G;
+    sa.dt = e.dt_half;
+    sa.invH = e.invH;
+    sa.lam = e.lam_half;
+    sa.flag = c.flags + s;
+    sa.bc = 1;
+    const int3 nb = slds::tiles<LBY, ZC>(e.G);
+    const int ntile = nb.x * nb.y * nb.z;
+    bool bad = false;
+    if (t >= ntile) {
+        slds::wall_face<AX, true>(sa, (t - ntile) * (BX * LBY) + threadIdx.y * BX + threadIdx.x,
+                                  bad);
+    } else {
+        const int rest = t / nb.x;
+        auto src_of = [&](int q) -> const DField & { return velf(e, sw, q); };
+        int zs, zend;
+        slds::zchunk(e.G, nb.z, rest / nb.y, zs, zend);
+        stg::tile<AX, LBY, true>(sa, src_of, (slds::lds_t *)lds_mem, t - rest * nb.x,
+                                 rest % nb.y, zs, zend, bad);
     }
     if (bad) *body::gp(sa.flag) = 1;
 }
@@ -787,11 +842,19 @@ hipError_t sweep_shape_launch(const af_batch *b, int nwg, int axis, int sw)
     return hipGetLastError();
 }
 
-template <int LBY, int ZC>
+template <int LBY, int ZC, bool OWN>
 hipError_t sweep_lds_launch(const af_batch *b, int axis, int sw)
 {
     const BCtl c = ctl(b);
     const dim3 grid(b->lds_grid), blk(BX, LBY);
+    if (OWN) {
+        switch (axis) {
+        case 0: hipLaunchKernelGGL((kb_sweep_stg<0, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
+        case 1: hipLaunchKernelGGL((kb_sweep_stg<1, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
+        default: hipLaunchKernelGGL((kb_sweep_stg<2, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
+        }
+        return hipGetLastError();
+    }
     switch (axis) {
     case 0: hipLaunchKernelGGL((kb_sweep_lds<0, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
     case 1: hipLaunchKernelGGL((kb_sweep_lds<1, LBY, ZC>), grid, blk, 0, b->stream, c, sw); break;
@@ -818,10 +881,14 @@ hipError_t enq_sweep(const af_batch *b, int plan, int nwg, int axis, int sw)
     if (err != hipSuccess) return err;
     switch (plan_lds(plan)) {
     case -1: return hipSuccess;
-    case 0: return sweep_lds_launch<4, 16>(b, axis, sw);
-    case 1: return sweep_lds_launch<4, 32>(b, axis, sw);
-    case 2: return sweep_lds_launch<8, 16>(b, axis, sw);
-    default: return sweep_lds_launch<8, 32>(b, axis, sw);
+    case 0: return sweep_lds_launch<4, 16, false>(b, axis, sw);
+    case 1: return sweep_lds_launch<4, 32, false>(b, axis, sw);
+    case 2: return sweep_lds_launch<8, 16, false>(b, axis, sw);
+    case 3: return sweep_lds_launch<8, 32, false>(b, axis, sw);
+    case 4: return sweep_lds_launch<8, 16, true>(b, axis, sw);
+    case 5: return sweep_lds_launch<8, 32, true>(b, axis, sw);
+    case 6: return sweep_lds_launch<4, 16, true>(b, axis, sw);
+    default: return sweep_lds_launch<4, 32, true>(b, axis, sw);
     }
 }
 

@@ -357,9 +357,12 @@ typedef struct af_batch_config {
     int32_t SweepStaged;       /* LDS-staged advection for slots whose x
                                   extent fills 64-wide tiles to >= 3/4:
                                   0 -> tuned; -1 -> off (every slot
-                                  gathers); k (1..4) -> variant k-1, i.e.
+                                  gathers); k (1..8) -> variant k-1, i.e.
                                   (rows, z-planes) of (4,16), (4,32), (8,16),
-                                  (8,32) per workgroup */
+                                  (8,32) per workgroup staging one box of
+                                  all three fields, then (8,16), (8,32),
+                                  (4,16), (4,32) staging each field's own
+                                  box */
     int32_t ServePolicy;       /* af_serve: 0 -> admission first come first
                                   served, every resident job advances each
                                   batched step; 1 -> least remaining work
