@@ -641,61 +641,117 @@ __device__ __forceinline__ int parity(int xi, int yi, int zi, int g)
 // One thread of a colour pass: cell k of rows (yi, zb .. zb+ZR-1).
 // MODE 0: normal pass.  MODE 1: first pass of a solve from zero (every
 // neighbour is +0).  MODE 2: second pass of a solve from zero (the cell's own
-// old value, used for mirrored ghosts, is +0).  The planes go in chunks of
+// old value, used for mirrored ghosts, is +0).
+//
+// The thread walks its planes in z.  Entry k of the other colour's rows
+// (yi, z) is the z-neighbour of this colour's entry k in rows (yi, z-1) and
+// (yi, z+1), and one of the two x-neighbours of entry k in row (yi, z) (x'
+// = 2k + x0 has neighbours x'-1 and x'+1, one of which is 2k or 2k+1, entry
+// k).  So that column of the other colour is read once per plane into a
+// register queue, and a plane needs four more loads: the other x-neighbour,
+// the two y-neighbours and the right-hand side.  The planes go in chunks of
 // CH: a chunk's loads are all issued, then its results formed, then stored,
 // so CH planes' loads are in flight together.  CH holds 32 bytes of results
-// per thread (8 FP32 or 4 FP64 planes): more raises the register count past
-// the point where fewer waves per SIMD cost more than the extra loads gain.
+// per thread (8 FP32 or 4 FP64 planes).
+//
+// Every neighbour value, wall substitution and the order of the sum are
+// those of gaussSeidelSweep (project.go:103-130) after applyPressureBC
+// (project.go:139-163): a neighbour outside the interior is replaced by the
+// cell's own old value (the Neumann mirror).  Out-of-range stand-in indices
+// keep every load inside the arrays.
+// What a thread's entry k of row yi is, for one parity x0 of the row (the
+// first x of this colour): the cell x' = x0 + 2k, or for a row whose last
+// entry has no cell of this colour (odd NX) a clamped in-range stand-in that
+// is computed and not stored.  Rows of successive planes alternate parity.
+struct SplitCol {
+    int off;            // the cell's entry in its plane: yi * NXH + (x' >> 1)
+    int xo;             // the x-neighbour entry other than k (k itself if both are k)
+    unsigned char wx;   // wall bits 0..3: x-, x+, y-, y+ outside the interior
+    bool st, live;      // stored; a cell of this colour within reach
+    bool xk0, xk1;      // the x- / x+ neighbour is entry k
+};
+
+__device__ __forceinline__ SplitCol split_col(int NX, int NXH, int k, int kq, int yi, int x0,
+                                              bool ylo, bool yhi)
+{
+    SplitCol c;
+    const int xr = x0 + 2 * k;
+    const int xi = xr < NX ? xr : (xr - 2 >= 0 ? xr - 2 : xr);
+    c.live = xi < NX;  // false when no cell of this colour is within reach
+    const int xs = c.live ? xi : 0;
+    c.off = yi * NXH + (xs >> 1);
+    c.st = xr < NX && c.live;
+    const bool xlo = xs == 0, xhi = xs == NX - 1;
+    c.wx = (unsigned char)(xlo | (xhi << 1) | (ylo << 2) | (yhi << 3));
+    // A wall cell reads an in-range stand-in for its outside neighbour (the
+    // value is replaced by the cell's own).
+    const int xm1 = xlo ? (xs + 1 < NX ? xs + 1 : xs) : xs - 1;
+    const int xp1 = xhi ? (xs > 0 ? xs - 1 : xs) : xs + 1;
+    const int km = xm1 >> 1, kp = xp1 >> 1;
+    c.xk0 = km == kq;
+    c.xk1 = kp == kq;
+    c.xo = yi * NXH + (c.xk0 ? kp : km);
+    return c;
+}
+
 template <typename T, int MODE, int ZR>
 __device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int color, int k, int yi,
                                                   int zb)
 {
     const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
-    if (yi >= NY) return;
+    if (yi >= NY || zb >= NZ) return;
     const auto cur = gp(color ? a.black : a.red);
     const auto oth = gp((const T *)(color ? a.red : a.black));
     const auto rhs = gp(color ? a.rhs_black : a.rhs_red);
+    const int P = NY * NXH;
     constexpr int CH = (int)(ZR * sizeof(T)) > 32 ? 32 / (int)sizeof(T) : ZR;
-    static_assert(ZR % CH == 0, "planes per thread must be a whole number of chunks");
+    static_assert(ZR % CH == 0 && CH % 2 == 0, "whole, even chunks of planes per thread");
+    const bool ylo = yi == 0, yhi = yi == NY - 1;
+    // Entry k of the other colour in plane zb-1 (zm) and zb (zc): the queue.
+    // Planes outside the interior are read as an in-range stand-in (the
+    // value is replaced by the wall's own value).  Lanes past the row's end
+    // (k >= NXH: the launch covers whole 64-lane rows) store nothing and
+    // read entry NXH-1.
+    const int kq = k < NXH ? k : NXH - 1;
+    const int col = yi * NXH + kq;
+    // The column's two parities: plane zb + r has the first for even r.
+    const int x0 = (color + yi + zb + 3 * a.g) & 1;
+    const SplitCol ca = split_col(NX, NXH, k, kq, yi, x0, ylo, yhi);
+    const SplitCol cb = split_col(NX, NXH, k, kq, yi, x0 ^ 1, ylo, yhi);
+    T zm = T(0), zc = T(0);
+    if (MODE != 1) {
+        zm = oth[(zb > 0 ? zb - 1 : zb) * P + col];
+        zc = oth[zb * P + col];
+    }
 #pragma unroll
-    for (int zc = zb; zc < zb + ZR; zc += CH) {
-        // Phase 1: every plane's index and its unconditional loads (the six
-        // other-colour neighbours, a neighbour outside the interior read from an
-        // in-range stand-in in the same row or column, and the right-hand
-        // side), so that all of them are in flight together.  A plane past the
-        // last (zi >= NZ) or a cell past the row's end (xi >= NX, odd NX) is
-        // computed on a clamped in-range cell of the same colour and not
-        // stored.
-        T nb[CH][6], rh[CH];
-        int at[CH], ii[CH];
+    for (int zc0 = zb; zc0 < zb + ZR; zc0 += CH) {
+        // Phase 1: every plane's index and its loads, all in flight together.
+        // A plane past the last (zr >= NZ) is computed on the last plane's
+        // indices and not stored.
+        T nq[CH], nx[CH], ny0[CH], ny1[CH], rh[CH], cw[CH];
+        int at[CH];
         unsigned char wall[CH];  // bit 0..5: x-, x+, y-, y+, z-, z+ outside the interior
-        bool live[CH];
 #pragma unroll
         for (int r = 0; r < CH; r++) {
-            const int zr = zc + r;
+            const SplitCol &cc = (r & 1) ? cb : ca;
+            const int zr = zc0 + r;
             const int zi = zr < NZ ? zr : NZ - 1;
-            const int x0 = (color + yi + zi + 3 * a.g) & 1;
-            const int xr = x0 + 2 * k;
-            const int xi = xr < NX ? xr : (xr - 2 >= 0 ? xr - 2 : xr);
-            live[r] = xi < NX;  // false when no cell of this colour is within reach
-            const int xs = live[r] ? xi : 0;
-            const int row = zi * NY + yi;
-            const int i = row * NXH + (xs >> 1);
-            ii[r] = i;
-            at[r] = (zr < NZ && xr < NX && live[r]) ? i : -1;
-            const bool xlo = xs == 0, xhi = xs == NX - 1, ylo = yi == 0, yhi = yi == NY - 1;
-            const bool zlo = zi == 0, zhi = zi == NZ - 1;
-            wall[r] = (unsigned char)(xlo | (xhi << 1) | (ylo << 2) | (yhi << 3) | (zlo << 4) |
-                                      (zhi << 5));
+            const int i = zi * P + cc.off;
+            at[r] = (zr < NZ && cc.st) ? i : -1;
+            wall[r] = (unsigned char)(cc.wx | ((zi == 0) << 4) | ((zi == NZ - 1) << 5));
+            // A wall cell's own old value (the Neumann mirror's stand-in for
+            // its outside neighbours); its entry is i, which is k except for
+            // a clamped stand-in cell (not stored).
+            cw[r] = T(0);
+            if (MODE == 0 && wall[r] && cc.live) cw[r] = cur[i];
             if (MODE != 1) {
-                const int xm1 = xlo ? (xs + 1 < NX ? xs + 1 : xs) : xs - 1;
-                const int xp1 = xhi ? (xs > 0 ? xs - 1 : xs) : xs + 1;
-                nb[r][0] = oth[row * NXH + (xm1 >> 1)];
-                nb[r][1] = oth[row * NXH + (xp1 >> 1)];
-                nb[r][2] = oth[ylo ? i : i - NXH];
-                nb[r][3] = oth[yhi ? i : i + NXH];
-                nb[r][4] = oth[zlo ? i : i - NY * NXH];
-                nb[r][5] = oth[zhi ? i : i + NY * NXH];
+                // The x-neighbour that is not entry k, the y-neighbours, and
+                // entry k of plane zr+1 (the queue's next value).
+                nx[r] = oth[zi * P + cc.xo];
+                ny0[r] = oth[ylo ? i : i - NXH];
+                ny1[r] = oth[yhi ? i : i + NXH];
+                const int zn = zr + 1 < NZ ? zr + 1 : NZ - 1;
+                nq[r] = oth[zn * P + col];
             }
             rh[r] = rhs[i];
         }
@@ -704,20 +760,26 @@ __device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int col
         T out[CH];
 #pragma unroll
         for (int r = 0; r < CH; r++) {
+            const SplitCol &cc = (r & 1) ? cb : ca;
             T sum;
             if (MODE == 1) {
                 sum = T(0) + T(0) + T(0) + T(0) + T(0) + T(0);
             } else {
-                T c = T(0);
-                if (MODE == 0 && wall[r] && live[r]) c = cur[ii[r]];
+                // Entry k of this plane (zc) and of the planes below (zm) and
+                // above (nq) are the queue.
+                const T c = cw[r];
                 const unsigned w = wall[r];
-                const T xm = (w & 1) ? c : nb[r][0];
-                const T xp = (w & 2) ? c : nb[r][1];
-                const T ym = (w & 4) ? c : nb[r][2];
-                const T yp = (w & 8) ? c : nb[r][3];
-                const T zm = (w & 16) ? c : nb[r][4];
-                const T zp = (w & 32) ? c : nb[r][5];
-                sum = xm + xp + ym + yp + zm + zp;
+                const T xa = cc.xk0 ? zc : nx[r];
+                const T xb = cc.xk1 ? zc : nx[r];
+                const T xm = (w & 1) ? c : xa;
+                const T xp = (w & 2) ? c : xb;
+                const T ym = (w & 4) ? c : ny0[r];
+                const T yp = (w & 8) ? c : ny1[r];
+                const T zmv = (w & 16) ? c : zm;
+                const T zpv = (w & 32) ? c : nq[r];
+                sum = xm + xp + ym + yp + zmv + zpv;
+                zm = zc;
+                zc = nq[r];
             }
             out[r] = relax<T>(sum, rh[r], a.h2);
         }

@@ -21,15 +21,15 @@ namespace {
 constexpr int BX = body::BX;
 constexpr int BY = 4;
 
-// Planes per thread: a pass over one plane per thread leaves each wave too
-// little work to keep the CUs full, so every thread updates zr<T>() planes,
-// all of whose loads are in flight together (32 bytes of results per thread:
-// 4 FP64 or 8 FP32 planes; measured at 512^3, 8 FP32 planes instead of 4
-// took the FP32 pass from 173 to 165 us, and 256^3 from 20.6 to 20.5 us).
+// Planes per thread: each thread walks zr<T>() planes of its column in z,
+// reading the other colour's column once per plane into a register queue
+// (body::split_pass_column); the queue's two prologue loads are paid once
+// per zr<T>() planes.  Measured at 256^3 (one pass, us): FP64 4/8/16
+// planes 30.8/29.7/29.6, FP32 8/16/32 planes 15.6/16.5/16.8.
 template <typename T>
 constexpr int zr()
 {
-    return sizeof(T) == 4 ? 8 : 4;
+    return 8;
 }
 
 template <typename T, int MODE>
