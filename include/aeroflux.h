@@ -308,8 +308,9 @@ int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles);
 int af_solver_tile(const af_engine *e, int32_t *k, int32_t *rows, int32_t *zchunk);
 
 /* Advection sweep workgroup shape in use: index, rows per workgroup and z
- * positions per thread (64 lanes along x; LDS-staged shapes 8-11: the
- * largest z-chunk a workgroup walks).  Chosen by timing every shape -- and
+ * positions per thread (64 lanes along x; LDS-staged shapes 8-13: the
+ * largest z-chunk a workgroup walks; 8-11 stage one box of the three
+ * fields, 12-13 each field's own box).  Chosen by timing every shape -- and
  * for the staged shapes two z-chunk counts -- on the engine's grid at New
  * unless SweepShape fixes it.  No reference counterpart. */
 int af_sweep_shape(const af_engine *e, int32_t *shape, int32_t *rows, int32_t *zper);
