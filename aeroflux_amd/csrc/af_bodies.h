@@ -604,6 +604,81 @@ __device__ __forceinline__ void gradient_column(const DField &U, const DField &V
     }
 }
 
+// gradient_column with P read from the colour-split solution (red, black:
+// interior cells, NXH = ceil(NX/2) entries per row) instead of the natural
+// layout: P's interior cells are the split entries, and the only ghosts the
+// gradient reads are first-layer face ghosts, which applyPressureBC
+// (project.go:139-163) fills with the interior cell beside them -- here the
+// split entry at the clamped coordinate.  So the values, and the bits, are
+// those of merging the solution into P, applying the pressure BC and then
+// gradient_column.
+template <bool BC, int ZG>
+__device__ __forceinline__ void gradient_column_split(const DField &U, const DField &V,
+                                                      const DField &W, const double *red,
+                                                      const double *black, const Geom &G,
+                                                      double scale, int x, int y, int z0)
+{
+    const int g = G.g;
+    if (x >= g + G.NX || y >= g + G.NY) return;
+    const int NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2;
+    const auto sr = gp(red), sb = gp(black);
+    // P at interior coordinates (xi, yi, zi), each clamped into the interior.
+    auto P = [&](int xi, int yi, int zi) -> double {
+        xi = xi < 0 ? 0 : (xi >= NX ? NX - 1 : xi);
+        yi = yi < 0 ? 0 : (yi >= NY ? NY - 1 : yi);
+        zi = zi < 0 ? 0 : (zi >= NZ ? NZ - 1 : zi);
+        const int i = (zi * NY + yi) * NXH + (xi >> 1);
+        return ((xi + yi + zi + 3 * g) & 1) ? sb[i] : sr[i];  // colour parity, red = 0
+    };
+    const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
+    const int zl = g + NZ;  // last position (W's wall face)
+    const int xi = x - g, yi = y - g;
+    double pc[ZG], px[ZG], py[ZG], pz[ZG], uu[ZG], vv[ZG], ww[ZG];
+#pragma unroll
+    for (int r = 0; r < ZG; r++) {
+        const int z = z0 + r < zl ? z0 + r : zl;
+        const int zi = z - g;
+        pc[r] = P(xi, yi, zi);
+        px[r] = P(xi - 1, yi, zi);
+        py[r] = P(xi, yi - 1, zi);
+        pz[r] = P(xi, yi, zi - 1);
+        uu[r] = u[idx(U, x, y, z)];  // z = g+NZ is a ghost plane of U and V: in range
+        vv[r] = v[idx(V, x, y, z)];
+        ww[r] = w[idx(W, x, y, z)];
+    }
+    double ou[ZG], ov[ZG], ow[ZG];
+#pragma unroll
+    for (int r = 0; r < ZG; r++) {
+        ou[r] = uu[r] - scale * (pc[r] - px[r]);
+        ov[r] = vv[r] - scale * (pc[r] - py[r]);
+        ow[r] = ww[r] - scale * (pc[r] - pz[r]);
+        asm volatile("" : "+v"(ou[r]), "+v"(ov[r]), "+v"(ow[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < ZG; r++) {
+        const int z = z0 + r;
+        if (z < zl) {
+            put_grad<BC, 0>(G, U, x, y, z, ou[r]);
+            put_grad<BC, 1>(G, V, x, y, z, ov[r]);
+        }
+        if (z <= zl) put_grad<BC, 2>(G, W, x, y, z, ow[r]);
+    }
+    if (x == g + NX - 1 || y == g + NY - 1) {
+        for (int r = 0; r < ZG; r++) {
+            const int z = z0 + r;
+            if (z >= zl) break;
+            const int zi = z - g;
+            const double c = P(xi, yi, zi);
+            if (x == g + NX - 1)
+                put_grad<BC, 0>(G, U, x + 1, y, z,
+                                u[idx(U, x, y, z) + 1] - scale * (P(xi + 1, yi, zi) - c));
+            if (y == g + NY - 1)
+                put_grad<BC, 1>(G, V, x, y + 1, z,
+                                v[idx(V, x, y, z) + V.sy] - scale * (P(xi, yi + 1, zi) - c));
+        }
+    }
+}
+
 // ---------------------------------------------------- colour-split layout
 //
 // A colour pass of gaussSeidelSweep (project.go:103-130) writes every cell of

@@ -301,7 +301,8 @@ SplitArgs<double> split_args(const af_engine *e)
 // the compacted colour arrays (the first from P == 0, which is the solve's
 // zeroing), and the result is merged back into P's interior before the final
 // ghost fill.  Bit-identical to the reference's sequence.
-hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D, bool rhs_ready)
+hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D, bool rhs_ready,
+                                bool merge = true)
 {
     hipError_t err = rhs_ready ? hipSuccess
                                : launch_to_split<double, double>(D.d, D.sy, D.sz, e->G, e->ds_red,
@@ -310,6 +311,7 @@ hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D, 
     const int passes = (int)(2 * e->cfg.GaussSeidelIters);
     for (int p = 0; p < passes && err == hipSuccess; p++)
         err = launch_split_pass<double>(a, p & 1, p == 0 ? 1 : (p == 1 ? 2 : 0), e->stream);
+    if (!merge) return err;  // the caller reads the solution from the split arrays
     if (err == hipSuccess)
         err = launch_from_split<double>(P.d, P.sy, P.sz, e->G, e->ps_red, e->ps_black, false,
                                         nullptr, e->stream);
@@ -364,10 +366,16 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 // project, project.go:12-34.  The gradient subtraction applies the closing
 // velocity BC itself (body::gradient_cell), which is the same values as the
 // separate launch.  bc_applied: the velocity already holds the opening BC
-// (the sweep before it applied the BC to its output).
-hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false)
+// (the sweep before it applied the BC to its output).  p_dead: nothing reads
+// P after this projection's gradient before the next solve overwrites it
+// (the step's first projection), so on the FP64 colour-split path the
+// solution is not merged into P: the gradient reads it from the split
+// arrays (launch_gradient_split, the same values).
+hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, bool p_dead = false)
 {
     hipError_t err = bc_applied ? hipSuccess : enq_velocity_bc(e, d);
+    const bool split_tail = p_dead && !e->direct.ready() &&
+                            e->cfg.Precision != AF_PRECISION_MIXED && e->tile64.k == 0;
     if (e->direct.method == DIRECT_FFT) {
         // The direct solve's first pass computes the divergence itself (the
         // same values, stored to Div), which saves re-reading Div.
@@ -379,8 +387,16 @@ hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false)
     } else {
         const bool fuse = fused_rhs(e);
         if (err == hipSuccess) err = enq_divergence(e, d, fuse);
-        if (err == hipSuccess) err = enq_solve(e, d, fuse);
+        if (err == hipSuccess && split_tail)
+            err = enq_solve_fp64_split(e, dfield(e, d.f[AF_FIELD_P]),
+                                       dfield(e, d.f[AF_FIELD_DIV]), fuse, false);
+        else if (err == hipSuccess)
+            err = enq_solve(e, d, fuse);
     }
+    if (err == hipSuccess && split_tail)
+        return launch_gradient_split(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
+                                     dfield(e, d.f[AF_FIELD_W]), e->ps_red, e->ps_black, e->G,
+                                     e->grad_scale, e->stream);
     if (err == hipSuccess) err = enq_gradient(e, d, true);
     return err;
 }
@@ -407,7 +423,8 @@ hipError_t enq_step(af_engine *e, Desc &d)
         err = enq_sweep(e, d, fwd[k], half, true);
         commit(d);
     }
-    if (err == hipSuccess) err = enq_project(e, d, true);
+    // P after the first projection is overwritten by the second solve.
+    if (err == hipSuccess) err = enq_project(e, d, true, true);
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
         err = enq_sweep(e, d, rev[k], half, true);
         commit(d);

@@ -63,6 +63,11 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s);
 // that follows subtractGradient in project), so that launch can be skipped.
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
                            hipStream_t s, bool velocity_bc = false);
+// subtractGradient with the velocity BC, P read from the FP64 colour-split
+// solution (red, black) instead of P: the values of merging it into P and
+// applying the pressure BC first.
+hipError_t launch_gradient_split(DField U, DField V, DField W, const double *red,
+                                 const double *black, Geom G, double scale, hipStream_t s);
 hipError_t launch_set_int(int *p, int v, hipStream_t s);
 
 // Reductions: two launches, fixed tree, no atomics.  `partials` must hold

@@ -151,6 +151,19 @@ __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField 
                                   g + by * BY + threadIdx.y, g + bz * GZ);
 }
 
+// The same with P read from the colour-split solution (body::gradient_column_split).
+__global__ __launch_bounds__(BX *BY) void k_gradient_split(DField U, DField V, DField W,
+                                                           const double *red, const double *black,
+                                                           Geom G, double scale, int3 nb)
+{
+    int bx, by, bz;
+    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+    const int g = G.g;
+    body::gradient_column_split<true, GZ>(U, V, W, red, black, G, scale,
+                                          g + bx * BX + threadIdx.x, g + by * BY + threadIdx.y,
+                                          g + bz * GZ);
+}
+
 __global__ void k_set_int(int *p, int v) { *p = v; }
 
 // ------------------------------------------------------------- reductions
@@ -330,6 +343,15 @@ hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, doubl
     else
         hipLaunchKernelGGL(k_gradient<false>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
                            nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_gradient_split(DField U, DField V, DField W, const double *red,
+                                 const double *black, Geom G, double scale, hipStream_t s)
+{
+    const int3 nb = nblocks(G.NX, G.NY, (G.NZ + 1 + GZ - 1) / GZ);  // z = g .. g+NZ
+    hipLaunchKernelGGL(k_gradient_split, grid1(nb), dim3(BX, BY), 0, s, U, V, W, red, black, G,
+                       scale, nb);
     return hipGetLastError();
 }
 
