@@ -100,13 +100,13 @@ inline int make_plan(int gather, int lds) { return gather + kShapes * (lds + 1);
 inline int plan_gather(int plan) { return plan % kShapes; }
 inline int plan_lds(int plan) { return plan / kShapes - 1; }
 
-// Slots whose x extent fills their 64-wide tiles to at least 3/4 take the
-// LDS-staged sweep; narrower ones keep the flattened-plane gather, which
-// leaves no lane idle.
+// Slots whose x extent fills their 64-wide tiles to at least 3/4 (48, 64,
+// 96, 128, 192, ...) take the LDS-staged sweep; narrower ones keep the
+// flattened-plane gather, which leaves no lane idle.
 inline bool lds_slot(const Geom &G)
 {
     const int tiles = (G.NX + BX - 1) / BX;
-    return G.NX >= BX && 4 * G.NX >= 3 * BX * tiles;
+    return 4 * G.NX >= 3 * BX * tiles;
 }
 
 template <int LBY, int ZC>

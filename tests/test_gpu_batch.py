@@ -173,7 +173,7 @@ def test_tuned_and_fixed_launch_shapes_agree(af):
 # LDS-staged sweep when the batch enables it; (33, 9, 14) and (70, 8, 6)
 # stay on the gather beside them.  (64, 7, 33, 2) has Ghost = 2.
 STAGED_GRIDS = [(64, 12, 10, 1), (96, 9, 14, 1), (128, 6, 5, 1), (64, 7, 33, 2), (33, 9, 14, 1),
-                (70, 8, 6, 2)]
+                (70, 8, 6, 2), (48, 10, 9, 1)]
 
 
 def staged_fields(i, shp, amp):
@@ -199,7 +199,7 @@ def test_staged_sweep_batch_equals_alone(af, variant, amp):
     b.StepN(steps)
     info = b.info()
     if variant:
-        assert info["SweepStaged"] == variant - 1 and info["StagedSlots"] == 4
+        assert info["SweepStaged"] == variant - 1 and info["StagedSlots"] == 5
     else:
         assert -1 <= info["SweepStaged"] < 8
     for i, (s, c, f) in enumerate(zip(slots, cfgs, fs)):
