@@ -418,11 +418,12 @@ __device__ __forceinline__ void sweep_column(const SweepArgs &a, int x, int y, i
 // ------------------------------------------------------------- projection
 
 // computeDivergence, project.go:42-62, interior cell (x, y, z).  Returns
-// false (writing nothing) outside the interior; else stores and returns the
-// value in `out`.
+// false (writing nothing) outside the interior; else stores the value to D
+// (unless store is false) and returns it in `out`.
 __device__ __forceinline__ bool divergence_cell(const DField &U, const DField &V, const DField &W,
                                                 const DField &D, const Geom &G, double scale,
-                                                int x, int y, int z, double &out)
+                                                int x, int y, int z, double &out,
+                                                bool store = true)
 {
     const int g = G.g;
     if (x >= g + G.NX || y >= g + G.NY) return false;
@@ -431,7 +432,7 @@ __device__ __forceinline__ bool divergence_cell(const DField &U, const DField &V
     const double dv = v[idx(V, x, y + 1, z)] - v[idx(V, x, y, z)];
     const double dw = w[idx(W, x, y, z + 1)] - w[idx(W, x, y, z)];
     out = scale * (du + dv + dw);
-    gp(D.d)[idx(D, x, y, z)] = out;
+    if (store) gp(D.d)[idx(D, x, y, z)] = out;
     return true;
 }
 

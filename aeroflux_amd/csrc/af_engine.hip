@@ -214,23 +214,29 @@ bool fused_rhs(const af_engine *e)
     return e->cfg.Precision == AF_PRECISION_MIXED || e->tile64.k == 0;
 }
 
-// computeDivergence; with `fuse` also the solve's split right-hand side.
-hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false)
+// computeDivergence; with `fuse` also the solve's split right-hand side,
+// and on the FP64 colour-split path the solve's first (red) pass, whose
+// values depend only on that right-hand side (k_divergence<3>).  div_dead
+// (FP64 colour-split path only): Div itself is not stored, because nothing
+// reads it before the next projection's divergence overwrites it.
+hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false, bool div_dead = false)
 {
     int split = 0;
     void *red = nullptr, *black = nullptr;
+    double *first = nullptr;
     if (fuse && e->cfg.Precision == AF_PRECISION_MIXED) {
         split = 2;
         red = e->mixed.r_red;
         black = e->mixed.r_black;
     } else if (fuse && e->tile64.k == 0) {
-        split = 1;
+        split = div_dead ? 4 : 3;
         red = e->ds_red;
         black = e->ds_black;
+        first = e->ps_red;
     }
     return launch_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
                              dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_DIV]), e->G,
-                             e->div_scale, e->stream, split, red, black, e->h2);
+                             e->div_scale, e->stream, split, red, black, e->h2, first);
 }
 
 // subtractGradient; with `velocity_bc` also the velocity BC that follows it
@@ -300,7 +306,10 @@ SplitArgs<double> split_args(const af_engine *e)
 // Colour-split form: Div is split once, the 2*GaussSeidelIters passes run on
 // the compacted colour arrays (the first from P == 0, which is the solve's
 // zeroing), and the result is merged back into P's interior before the final
-// ghost fill.  Bit-identical to the reference's sequence.
+// ghost fill.  Bit-identical to the reference's sequence.  rhs_ready: the
+// projection's divergence (k_divergence<3>) already wrote the split
+// right-hand side and the first red pass, so the passes start at the first
+// black one.
 hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D, bool rhs_ready,
                                 bool merge = true)
 {
@@ -309,7 +318,7 @@ hipError_t enq_solve_fp64_split(af_engine *e, const DField &P, const DField &D, 
                                                                  e->ds_black, e->stream);
     const SplitArgs<double> a = split_args(e);
     const int passes = (int)(2 * e->cfg.GaussSeidelIters);
-    for (int p = 0; p < passes && err == hipSuccess; p++)
+    for (int p = rhs_ready ? 1 : 0; p < passes && err == hipSuccess; p++)
         err = launch_split_pass<double>(a, p & 1, p == 0 ? 1 : (p == 1 ? 2 : 0), e->stream);
     if (!merge) return err;  // the caller reads the solution from the split arrays
     if (err == hipSuccess)
@@ -337,7 +346,8 @@ hipError_t enq_solve_fp64(af_engine *e, const Desc &d, bool rhs_ready)
 }
 
 // rhs_ready: the divergence of this projection already wrote the split
-// right-hand side (enq_divergence with fuse).
+// right-hand side (enq_divergence with fuse), and on the FP64 colour-split
+// path also the solve's first red pass.
 hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 {
     if (e->direct.ready()) {
@@ -386,7 +396,7 @@ hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, boo
                 dfield(e, d.f[AF_FIELD_W]), e->div_scale, e->G, e->stream);
     } else {
         const bool fuse = fused_rhs(e);
-        if (err == hipSuccess) err = enq_divergence(e, d, fuse);
+        if (err == hipSuccess) err = enq_divergence(e, d, fuse, split_tail);
         if (err == hipSuccess && split_tail)
             err = enq_solve_fp64_split(e, dfield(e, d.f[AF_FIELD_P]),
                                        dfield(e, d.f[AF_FIELD_DIV]), fuse, false);

@@ -38,25 +38,37 @@ __global__ __launch_bounds__(256) void k_velocity_bc(DField U, DField V, DField 
 // the colour-split layout of the FP64 solve, SPLIT 2 writes the FP32
 // right-hand side -(H^2 div) of the mixed-precision solve there: the same
 // values k_to_split / k_rhs32_split would form from Div, without reading it
-// back.
+// back.  SPLIT 3 is SPLIT 1 plus the solve's first red pass: the solve
+// starts from P == 0 (project.go:73-80), so every neighbour a red cell reads
+// in the first pass, and the cell's own value that stands in for a mirrored
+// ghost, is +0, and the cell's new value is relax(+0 + ... + +0, div) -- the
+// colour pass's own arithmetic (body::split_pass_column, MODE 1) on the
+// value just formed.  SPLIT 4 is SPLIT 3 without the store to Div, for a
+// projection whose Div is overwritten before anything reads it.
 template <int SPLIT>
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
                                                        Geom G, double scale, int3 nb, void *red,
-                                                       void *black, double h2)
+                                                       void *black, double h2, double *first)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
     const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, zi = bz;
     double v;
-    if (!body::divergence_cell(U, V, W, D, G, scale, g + xi, g + yi, g + zi, v) || SPLIT == 0)
+    if (!body::divergence_cell(U, V, W, D, G, scale, g + xi, g + yi, g + zi, v, SPLIT != 4) ||
+        SPLIT == 0)
         return;
     const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
-    void *dst = body::parity(xi, yi, zi, g) ? black : red;
-    if (SPLIT == 1)
+    const bool is_black = body::parity(xi, yi, zi, g);
+    void *dst = is_black ? black : red;
+    if (SPLIT != 2)
         body::gp(static_cast<double *>(dst))[i] = v;
     else
         body::gp(static_cast<float *>(dst))[i] = (float)(-(h2 * v));
+    if (SPLIT >= 3 && !is_black) {
+        const double zero = 0.0;
+        body::gp(first)[i] = body::relax<double>(zero + zero + zero + zero + zero + zero, v, h2);
+    }
 }
 
 // gaussSeidelSweep, project.go:103-130: one lane per cell of the colour.
@@ -281,21 +293,31 @@ hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t 
 }
 
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
-                             hipStream_t s, int split, void *red, void *black, double h2)
+                             hipStream_t s, int split, void *red, void *black, double h2,
+                             double *first)
 {
     const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    if (split >= 3 && !first) return hipErrorInvalidValue;
     switch (split) {
     case 0:
         hipLaunchKernelGGL(k_divergence<0>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2);
+                           nb, red, black, h2, first);
         break;
     case 1:
         hipLaunchKernelGGL(k_divergence<1>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2);
+                           nb, red, black, h2, first);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_divergence<2>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
+                           nb, red, black, h2, first);
+        break;
+    case 3:
+        hipLaunchKernelGGL(k_divergence<3>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
+                           nb, red, black, h2, first);
         break;
     default:
-        hipLaunchKernelGGL(k_divergence<2>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2);
+        hipLaunchKernelGGL(k_divergence<4>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
+                           nb, red, black, h2, first);
         break;
     }
     return hipGetLastError();

@@ -57,9 +57,12 @@ def schedule_bytes(kind: str, iters: int = 24) -> int:
     * fp64_split (batches), per projection: divergence 40 (U, V, W 24, Div 8,
       split right-hand side 8), first pass from zero 8, second pass 12,
       2*iters-2 passes of 12, merge into P 16, gradient 56 = 108 + 24*iters;
-    * fp64_split_engine (one engine): the same, except that the first
-      projection's gradient reads the split solution and skips the merge
-      (16 fewer per step);
+    * fp64_split_engine (one engine): the same, except that the divergence
+      also writes the first pass's red values (4 more, and no first-pass
+      kernel: 8 fewer, per projection), and in the first projection Div is
+      not stored (nothing reads it before the second projection rewrites
+      it: 8 fewer) and the gradient reads the split solution and skips the
+      merge (16 fewer);
     * fp64_arena (small grids): divergence 32, first pass 16, 2*iters-1
       passes of 24, gradient 56 = 80 + 48*iters;
     * mixed (one FP32 cycle), per projection: divergence 36 (FP32 rhs 4),
@@ -71,7 +74,7 @@ def schedule_bytes(kind: str, iters: int = 24) -> int:
     * direct (FFT passes): divergence folded into the first pass, five
       passes 80, gradient 56, per projection 32 + 80 + 56."""
     if kind == "fp64_split_engine":
-        return schedule_bytes("fp64_split", iters) - 16
+        return schedule_bytes("fp64_split", iters) - 16 - 8 - 2 * 4
     proj = {"fp64_split": 108 + 24 * iters, "fp64_arena": 80 + 48 * iters,
             "mixed": 102 + 12 * iters, "mixed_checked": 114 + 12 * iters,
             "direct": 32 + 80 + 56}[kind]
