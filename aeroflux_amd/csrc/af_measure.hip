@@ -1,7 +1,10 @@
 // af_measure.hip -- attainable streaming bandwidth of the access patterns the
 // step's kernels use (measurement only; no reference counterpart).  A
 // grid-stride loop over 16-byte elements, no reuse, HIP events around
-// back-to-back launches on a private stream.
+// back-to-back launches on a private stream.  The arrays hold pseudo-random
+// doubles in [-1, 1], like the fields the step moves: the memory system
+// moves all-zero data measurably faster (a 256^3 FP64 colour pass takes
+// about 29 us on zeroed arrays and 32.5 us on such values).
 #include <hip/hip_runtime.h>
 
 #include "../../include/aeroflux.h"
@@ -12,6 +15,20 @@ namespace {
 
 typedef double d2 __attribute__((ext_vector_type(2)));
 
+// Element i of an array: a hash of (i, seed) mapped to [-1, 1] in steps of
+// 1e-6.
+__global__ __launch_bounds__(256) void k_stream_fill(double *p, int64_t n, unsigned seed)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        unsigned long long h = (unsigned long long)(i + 1) * 0x9E3779B97F4A7C15ull ^ seed;
+        h ^= h >> 31;
+        h *= 0xBF58476D1CE4E5B9ull;
+        h ^= h >> 29;
+        p[i] = (double)(h % 2000001ull) * 1e-6 - 1.0;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_stream_read(const d2 *a, int64_t n2, double *sink)
 {
     double s = 0;
@@ -20,9 +37,8 @@ __global__ __launch_bounds__(256) void k_stream_read(const d2 *a, int64_t n2, do
         const d2 v = a[i];
         s += v.x + v.y;
     }
-    // The buffers are zero, so the sum is 0 and nothing is stored; the test
-    // keeps the loads live.
-    if (s != 0.0) sink[blockIdx.x] = s;
+    // One store per workgroup keeps the loads live.
+    sink[blockIdx.x] = s;
 }
 
 __global__ __launch_bounds__(256) void k_stream_copy(const d2 *a, d2 *b, int64_t n2)
@@ -72,7 +88,11 @@ extern "C" int af_measure_stream(int kind, int64_t n, int reps, double *gbps)
     hipError_t err = hipStreamCreate(&s);
     for (int i = 0; i < kArrays[kind] && err == hipSuccess; i++) {
         err = hipMalloc(&buf[i], bytes);
-        if (err == hipSuccess) err = hipMemsetAsync(buf[i], 0, bytes, s);
+        if (err == hipSuccess) {
+            hipLaunchKernelGGL(k_stream_fill, dim3(kGrid), dim3(256), 0, s,
+                               reinterpret_cast<double *>(buf[i]), n, 17u + (unsigned)i);
+            err = hipGetLastError();
+        }
     }
     if (err == hipSuccess) err = hipMalloc(&sink, kGrid * sizeof(double));
     if (err == hipSuccess) err = hipEventCreate(&e0);

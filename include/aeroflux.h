@@ -293,8 +293,9 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
  * pattern over arrays of n doubles each (n even), timed over `reps`
  * back-to-back launches after one warm-up: kind 0 = read one array, 1 =
  * copy, 2 = two reads and one write (a colour pass's pattern), 3 = four
- * reads and three writes (the gradient's).  Allocates and frees its arrays.
- * No reference counterpart (measurement only). */
+ * reads and three writes (the gradient's).  The arrays hold pseudo-random
+ * values in [-1, 1] (all-zero data moves measurably faster).  Allocates and
+ * frees its arrays.  No reference counterpart (measurement only). */
 int af_measure_stream(int kind, int64_t n, int reps, double *gbps);
 
 /* Mixed precision only: max|div| implied by the last FP64 residual check
