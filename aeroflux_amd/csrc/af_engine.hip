@@ -214,20 +214,20 @@ bool fused_rhs(const af_engine *e)
     return e->cfg.Precision == AF_PRECISION_MIXED || e->tile64.k == 0;
 }
 
-// computeDivergence; with `fuse` also the solve's split right-hand side,
-// and on the FP64 colour-split path the solve's first (red) pass, whose
-// values depend only on that right-hand side (k_divergence<3>).  div_dead
+// computeDivergence; with `fuse` also the solve's split right-hand side
+// and the solve's first (red) pass, whose values depend only on that
+// right-hand side (k_divergence<3>, or <2> for the mixed solve's FP32 pass).  div_dead
 // (FP64 colour-split path only): Div itself is not stored, because nothing
 // reads it before the next projection's divergence overwrites it.
 hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false, bool div_dead = false)
 {
     int split = 0;
-    void *red = nullptr, *black = nullptr;
-    double *first = nullptr;
+    void *red = nullptr, *black = nullptr, *first = nullptr;
     if (fuse && e->cfg.Precision == AF_PRECISION_MIXED) {
         split = 2;
         red = e->mixed.r_red;
         black = e->mixed.r_black;
+        first = e->mixed.e_red;
     } else if (fuse && e->tile64.k == 0) {
         split = div_dead ? 4 : 3;
         red = e->ds_red;

@@ -45,14 +45,15 @@ hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t 
 hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s);
 // split: 0 -> Div only; 1 -> also the FP64 colour-split copy (red, black
 // are double arrays); 2 -> also the FP32 split right-hand side -(h2 * div)
-// of the mixed-precision solve (float arrays); 3 -> as 1, and also the
-// solve's first colour pass from zero into `first` (the solution's red
-// array): each red cell's value depends only on its own right-hand side;
-// 4 -> as 3 without the store to D (a Div that nothing reads before the
-// next divergence overwrites it).
+// of the mixed-precision solve (float arrays) and its first FP32 red pass
+// from zero into `first` (float, the correction's red array); 3 -> as 1, and
+// also the solve's first colour pass from zero into `first` (double, the
+// solution's red array): each red cell's value depends only on its own
+// right-hand side; 4 -> as 3 without the store to D (a Div that nothing
+// reads before the next divergence overwrites it).
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
                              hipStream_t s, int split = 0, void *red = nullptr,
-                             void *black = nullptr, double h2 = 0, double *first = nullptr);
+                             void *black = nullptr, double h2 = 0, void *first = nullptr);
 // Red-black colour pass.  fused_bc: a neighbour outside the interior reads the
 // cell's own value (the Neumann mirror applied just before the pass).
 hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fused_bc,

@@ -44,11 +44,13 @@ __global__ __launch_bounds__(256) void k_velocity_bc(DField U, DField V, DField 
 // ghost, is +0, and the cell's new value is relax(+0 + ... + +0, div) -- the
 // colour pass's own arithmetic (body::split_pass_column, MODE 1) on the
 // value just formed.  SPLIT 4 is SPLIT 3 without the store to Div, for a
-// projection whose Div is overwritten before anything reads it.
+// projection whose Div is overwritten before anything reads it.  SPLIT 2
+// likewise writes the first FP32 red pass of the mixed solve's first cycle
+// (from e == 0) into `first` (the correction's red array).
 template <int SPLIT>
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
                                                        Geom G, double scale, int3 nb, void *red,
-                                                       void *black, double h2, double *first)
+                                                       void *black, double h2, void *first)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
@@ -61,13 +63,21 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DFiel
     const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
     const bool is_black = body::parity(xi, yi, zi, g);
     void *dst = is_black ? black : red;
-    if (SPLIT != 2)
+    if (SPLIT != 2) {
         body::gp(static_cast<double *>(dst))[i] = v;
-    else
-        body::gp(static_cast<float *>(dst))[i] = (float)(-(h2 * v));
-    if (SPLIT >= 3 && !is_black) {
-        const double zero = 0.0;
-        body::gp(first)[i] = body::relax<double>(zero + zero + zero + zero + zero + zero, v, h2);
+        if (SPLIT >= 3 && !is_black) {
+            const double zero = 0.0;
+            body::gp(static_cast<double *>(first))[i] =
+                body::relax<double>(zero + zero + zero + zero + zero + zero, v, h2);
+        }
+    } else {
+        const float rf = (float)(-(h2 * v));
+        body::gp(static_cast<float *>(dst))[i] = rf;
+        if (!is_black) {
+            const float zero = 0.0f;
+            body::gp(static_cast<float *>(first))[i] =
+                body::relax<float>(zero + zero + zero + zero + zero + zero, rf, 0.0);
+        }
     }
 }
 
@@ -294,10 +304,10 @@ hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t 
 
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
                              hipStream_t s, int split, void *red, void *black, double h2,
-                             double *first)
+                             void *first)
 {
     const int3 nb = nblocks(G.NX, G.NY, G.NZ);
-    if (split >= 3 && !first) return hipErrorInvalidValue;
+    if (split >= 2 && !first) return hipErrorInvalidValue;
     switch (split) {
     case 0:
         hipLaunchKernelGGL(k_divergence<0>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,

@@ -28,7 +28,9 @@ struct MixedCtx {
     int cycles;        // maximum cycles
     double target;     // max|div| target after the projection (0: one cycle)
     double res_to_div; // DT/(rho*H^2): max|div| = res_to_div * max|residual|
-    bool rhs_ready = false;  // the split right-hand side is already in r_red/r_black
+    // The split right-hand side is already in r_red/r_black, and the first
+    // cycle's first red pass in e_red (k_divergence<2>).
+    bool rhs_ready = false;
     int slab = 1 << 30;      // workgroup y-slab order of the FP32 passes (SplitArgs::slab)
 };
 

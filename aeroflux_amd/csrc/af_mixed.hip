@@ -219,8 +219,11 @@ __global__ void k_mixed_reset(int *done, int *cycles)
 
 __global__ void k_mixed_count(int *cycles) { *cycles = *cycles + 1; }
 
-// `passes` FP32 red-black passes (red first) on A e = r from e = 0.
-hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipStream_t s)
+// `passes` FP32 red-black passes (red first) on A e = r from e = 0; with
+// first_done the first red pass's values are already in e_red and the
+// passes start at the first black one.
+hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipStream_t s,
+                  bool first_done = false)
 {
     SplitArgs<float> a;
     a.red = m.e_red;
@@ -231,7 +234,7 @@ hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipS
     a.h2 = 0;
     a.done = gated ? m.done : nullptr;
     a.slab = c.slab;
-    for (int p = 0; p < passes; p++) {
+    for (int p = first_done ? 1 : 0; p < passes; p++) {
         hipError_t err = launch_split_pass<float>(a, p & 1, p == 0 ? 1 : (p == 1 ? 2 : 0), s);
         if (err != hipSuccess) return err;
     }
@@ -285,7 +288,7 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
         (err = launch_rhs32_split(c.D.d, c.D.sy, c.D.sz, G, c.h2, m.r_red, m.r_black, s)) !=
             hipSuccess)
         return err;
-    if ((err = smooth(m, c, 2 * c.sweeps, false, s)) != hipSuccess) return err;
+    if ((err = smooth(m, c, 2 * c.sweeps, false, s, c.rhs_ready)) != hipSuccess) return err;
     if (!check) {
         if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, false,
                                             nullptr, s)) != hipSuccess)
