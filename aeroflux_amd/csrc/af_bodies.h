@@ -182,6 +182,16 @@ __device__ __forceinline__ double2 ld2(const double *p)
     return make_double2(v.x, v.y);
 }
 
+// p[0] = a, p[1] = b in one 16-byte store (8-byte aligned).
+__device__ __forceinline__ void st2(double *p, double a, double b)
+{
+    typedef __attribute__((address_space(1))) dbl2_a8 gdbl2;
+    dbl2_a8 v;
+    v.x = a;
+    v.y = b;
+    *(gdbl2 *)p = v;
+}
+
 // Where a sweep's values come from.  A loader returns component c's value
 // at padded coordinates (x, y, z) (`one`) or the x-adjacent pair at x, x+1
 // (`pair`).  GlobalLoad reads the fields in HBM; the LDS-staged sweep
@@ -433,6 +443,31 @@ __device__ __forceinline__ bool divergence_cell(const DField &U, const DField &V
     const double dw = w[idx(W, x, y, z + 1)] - w[idx(W, x, y, z)];
     out = scale * (du + dv + dw);
     if (store) gp(D.d)[idx(D, x, y, z)] = out;
+    return true;
+}
+
+// divergence_cell for the cell pair x = 2k, 2k+1 of interior row (yi, zi)
+// (interior coordinates), with the same values: U's faces 2k .. 2k+2 and the
+// x-adjacent V and W faces as 16-byte pairs.  Returns false outside the
+// interior; the second value is set, not meaningful, when cell 2k+1 is
+// outside it (odd NX).  The last face read, U's 2k+2 or a V/W pair's second
+// face, is inside the padded row.
+__device__ __forceinline__ bool divergence_pair(const DField &U, const DField &V, const DField &W,
+                                                const Geom &G, double scale, int k, int yi, int zi,
+                                                double &d0, double &d1)
+{
+    const int g = G.g;
+    if (2 * k >= G.NX || yi >= G.NY) return false;
+    const int x = g + 2 * k, y = g + yi, z = g + zi;
+    const int iu = idx(U, x, y, z), iv = idx(V, x, y, z), iw = idx(W, x, y, z);
+    const double2 ua = ld2(U.d + iu), ub = ld2(U.d + iu + 1);
+    const double2 v0 = ld2(V.d + iv), v1 = ld2(V.d + iv + V.sy);
+    const double2 w0 = ld2(W.d + iw), w1 = ld2(W.d + iw + W.sz);
+    const double du0 = ua.y - ua.x, du1 = ub.y - ub.x;
+    const double dv0 = v1.x - v0.x, dv1 = v1.y - v0.y;
+    const double dw0 = w1.x - w0.x, dw1 = w1.y - w0.y;
+    d0 = scale * (du0 + dv0 + dw0);
+    d1 = scale * (du1 + dv1 + dw1);
     return true;
 }
 
@@ -871,28 +906,6 @@ __device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int col
     }
 }
 
-// Interior cell (xi, yi, zi) of a padded natural-layout field -> split.
-template <typename TS, typename TN>
-__device__ __forceinline__ void to_split_cell(const TN *nat, int sy, int sz, int g, int NX, int NY,
-                                              int NXH, TS *red, TS *black, int xi, int yi, int zi)
-{
-    if (xi >= NX || yi >= NY) return;
-    const TN v = gp(nat)[(xi + g) + (yi + g) * sy + (zi + g) * sz];
-    const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    if (parity(xi, yi, zi, g)) gp(black)[i] = (TS)v; else gp(red)[i] = (TS)v;
-}
-
-// FP32 right-hand side -(h2 * div) of the mixed-precision smoother, split.
-__device__ __forceinline__ void rhs32_cell(const double *div, int sy, int sz, int g, int NX, int NY,
-                                           int NXH, double h2, float *red, float *black, int xi,
-                                           int yi, int zi)
-{
-    if (xi >= NX || yi >= NY) return;
-    const float v = (float)(-(h2 * gp(div)[(xi + g) + (yi + g) * sy + (zi + g) * sz]));
-    const int i = (zi * NY + yi) * NXH + (xi >> 1);
-    if (parity(xi, yi, zi, g)) gp(black)[i] = v; else gp(red)[i] = v;
-}
-
 // Split -> interior cell of the FP64 natural-layout P.  ADD: P += split (the
 // correction of the mixed-precision solve); else P = split.
 template <typename TS, bool ADD>
@@ -905,6 +918,60 @@ __device__ __forceinline__ void from_split_cell(double *nat, int sy, int sz, int
     const double v = (double)(parity(xi, yi, zi, g) ? gp(black)[i] : gp(red)[i]);
     const int n = (xi + g) + (yi + g) * sy + (zi + g) * sz;
     gp(nat)[n] = ADD ? gp(nat)[n] + v : v;
+}
+
+// Split conversions for the cell pair x = 2k, 2k+1 of interior row
+// (yi, zi): entry k of both colour arrays (cell 2k has the row's parity,
+// the same for every pair of the row) and the two x-adjacent natural values
+// in one 16-byte access.  When the pair's second cell is outside the
+// interior (odd NX), the natural row still holds it (a ghost, read and not
+// used) and only the first cell is written.
+__device__ __forceinline__ void to_split_pair(const double *nat, int sy, int sz, int g, int NX,
+                                              int NY, int NXH, double *red, double *black, int k,
+                                              int yi, int zi)
+{
+    if (k >= NXH || yi >= NY) return;
+    const int x0 = 2 * k;
+    const double2 v = ld2(nat + (x0 + g) + (yi + g) * sy + (zi + g) * sz);
+    const int i = (zi * NY + yi) * NXH + k;
+    const bool b0 = parity(0, yi, zi, g);
+    gp(b0 ? black : red)[i] = v.x;
+    if (x0 + 1 < NX) gp(b0 ? red : black)[i] = v.y;
+}
+
+__device__ __forceinline__ void rhs32_pair(const double *div, int sy, int sz, int g, int NX,
+                                           int NY, int NXH, double h2, float *red, float *black,
+                                           int k, int yi, int zi)
+{
+    if (k >= NXH || yi >= NY) return;
+    const int x0 = 2 * k;
+    const double2 v = ld2(div + (x0 + g) + (yi + g) * sy + (zi + g) * sz);
+    const int i = (zi * NY + yi) * NXH + k;
+    const bool b0 = parity(0, yi, zi, g);
+    gp(b0 ? black : red)[i] = (float)(-(h2 * v.x));
+    if (x0 + 1 < NX) gp(b0 ? red : black)[i] = (float)(-(h2 * v.y));
+}
+
+template <typename TS, bool ADD>
+__device__ __forceinline__ void from_split_pair(double *nat, int sy, int sz, int g, int NX, int NY,
+                                                int NXH, const TS *red, const TS *black, int k,
+                                                int yi, int zi)
+{
+    if (k >= NXH || yi >= NY) return;
+    const int x0 = 2 * k;
+    const int i = (zi * NY + yi) * NXH + k;
+    const bool b0 = parity(0, yi, zi, g);
+    const double v0 = (double)gp(b0 ? black : red)[i];
+    const double v1 = (double)gp(b0 ? red : black)[i];
+    double *d = nat + (x0 + g) + (yi + g) * sy + (zi + g) * sz;
+    if (x0 + 1 < NX) {
+        double2 o = make_double2(0.0, 0.0);
+        if (ADD) o = ld2(d);
+        st2(d, ADD ? o.x + v0 : v0, ADD ? o.y + v1 : v1);
+    } else {
+        const int n = (int)(d - nat);
+        gp(nat)[n] = ADD ? gp(nat)[n] + v0 : v0;
+    }
 }
 
 // FP64 residual of the mixed-precision solve along interior column (xi, yi):

@@ -47,6 +47,10 @@ __global__ __launch_bounds__(256) void k_velocity_bc(DField U, DField V, DField 
 // projection whose Div is overwritten before anything reads it.  SPLIT 2
 // likewise writes the first FP32 red pass of the mixed solve's first cycle
 // (from e == 0) into `first` (the correction's red array).
+//
+// Each thread forms the cell pair x = 2k, 2k+1 of its row (body::
+// divergence_pair), one cell of each colour: Div is written as a 16-byte
+// pair and the split arrays take entry k of both colours.
 template <int SPLIT>
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
                                                        Geom G, double scale, int3 nb, void *red,
@@ -55,28 +59,40 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DFiel
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, zi = bz;
-    double v;
-    if (!body::divergence_cell(U, V, W, D, G, scale, g + xi, g + yi, g + zi, v, SPLIT != 4) ||
-        SPLIT == 0)
-        return;
-    const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
-    const bool is_black = body::parity(xi, yi, zi, g);
-    void *dst = is_black ? black : red;
+    const int k = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, zi = bz;
+    double v0, v1;
+    if (!body::divergence_pair(U, V, W, G, scale, k, yi, zi, v0, v1)) return;
+    const bool two = 2 * k + 1 < G.NX;
+    if (SPLIT != 4) {
+        double *d = D.d + body::idx(D, g + 2 * k, g + yi, g + zi);
+        if (two)
+            body::st2(d, v0, v1);
+        else
+            body::gp(D.d)[(int)(d - D.d)] = v0;
+    }
+    if (SPLIT == 0) return;
+    const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + k;
+    // Cell 2k has the row's colour; cell 2k+1 the other.
+    const bool b0 = body::parity(0, yi, zi, g);
+    void *dst0 = b0 ? black : red, *dst1 = b0 ? red : black;
+    const double vr = b0 ? v1 : v0;  // the red cell's value
+    const bool has_red = !b0 || two;
     if (SPLIT != 2) {
-        body::gp(static_cast<double *>(dst))[i] = v;
-        if (SPLIT >= 3 && !is_black) {
+        body::gp(static_cast<double *>(dst0))[i] = v0;
+        if (two) body::gp(static_cast<double *>(dst1))[i] = v1;
+        if (SPLIT >= 3 && has_red) {
             const double zero = 0.0;
             body::gp(static_cast<double *>(first))[i] =
-                body::relax<double>(zero + zero + zero + zero + zero + zero, v, h2);
+                body::relax<double>(zero + zero + zero + zero + zero + zero, vr, h2);
         }
     } else {
-        const float rf = (float)(-(h2 * v));
-        body::gp(static_cast<float *>(dst))[i] = rf;
-        if (!is_black) {
+        const float r0 = (float)(-(h2 * v0)), r1 = (float)(-(h2 * v1));
+        body::gp(static_cast<float *>(dst0))[i] = r0;
+        if (two) body::gp(static_cast<float *>(dst1))[i] = r1;
+        if (has_red) {
             const float zero = 0.0f;
             body::gp(static_cast<float *>(first))[i] =
-                body::relax<float>(zero + zero + zero + zero + zero + zero, rf, 0.0);
+                body::relax<float>(zero + zero + zero + zero + zero + zero, b0 ? r1 : r0, 0.0);
         }
     }
 }
@@ -306,7 +322,7 @@ hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, dou
                              hipStream_t s, int split, void *red, void *black, double h2,
                              void *first)
 {
-    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    const int3 nb = nblocks((G.NX + 1) / 2, G.NY, G.NZ);
     if (split >= 2 && !first) return hipErrorInvalidValue;
     switch (split) {
     case 0:

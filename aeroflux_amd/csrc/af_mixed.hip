@@ -53,11 +53,16 @@ __global__ __launch_bounds__(BX *BY) void k_residual(DField P, const double *__r
 // values and their operation order are those of the merge followed by
 // k_residual, so the results are the same bits; the pass moves 24 B per cell
 // (e 4, Div 8, P 8, r 4) instead of the 32 B of the two, and it issues all
-// of a thread's loads before the first wait.  A workgroup owns a 64 x 4
-// column of (x, y) and ZC planes of it; the z-neighbours of a plane are the
-// column's own values of the planes beside it.  (Walking whole columns
-// through z, as k_residual does, keeps too few loads in flight: measured
-// 189 us against 115 us at 256^3.)
+// of a thread's loads before the first wait.
+//
+// A thread owns split entry k of row (y, z) for ZC planes: the cell pair
+// x = 2k, 2k+1, one of each colour (cell 2k has the row's parity, so the
+// colour array of each cell is uniform across the wave).  Its own values,
+// its x-neighbours 2k-1 and 2k+2 (entries k-1 and k+1 of the other and the
+// same colour), its y-neighbours (entry k of both colours in rows y-1, y+1)
+// and its z-neighbours (the pair's own values in the planes beside it) are
+// whole loads per lane, and Div and P are read and written as 16-byte pairs.
+// A workgroup owns 64 x 4 entries of (x, y) and ZC planes.
 constexpr int ZC = 4;
 
 __global__ __launch_bounds__(BX *BY) void k_merge_residual(DField P, const double *__restrict__ div,
@@ -69,70 +74,94 @@ __global__ __launch_bounds__(BX *BY) void k_merge_residual(DField P, const doubl
     __shared__ double sh[BY];
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int xi = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, z0 = bz * ZC;
+    const int NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2, g = G.g;
+    const int k = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, z0 = bz * ZC;
     double m = 0;
-    if (xi < G.NX && yi < G.NY) {
-        const int g = G.g, NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2;
-        // The colour array is chosen per lane by address, not by a branch:
-        // neighbouring lanes alternate colours, and a branch would split
-        // every load in two dependent halves.
-        const auto e_at = [&](int x, int y, int z) -> double {
-            const int i = (z * NY + y) * NXH + (x >> 1);
-            const float *base = body::parity(x, y, z, g) ? e_black : e_red;
-            return (double)body::gp(base)[i];
+    if (k < NXH && yi < NY) {
+        const int x0 = 2 * k;
+        const bool two = x0 + 1 < NX;  // the pair's second cell exists
+        // Colour arrays of cell x0 (parity of the row) and x0+1 in plane z.
+        const auto arr = [&](int z, bool second) -> const float * {
+            return (body::parity(0, yi, z, g) ^ (int)second) ? e_black : e_red;
         };
-        const bool xlo = xi == 0, xhi = xi == NX - 1, ylo = yi == 0, yhi = yi == NY - 1;
-        // Phase 1: every load is unconditional (an outside neighbour reads an
-        // in-range stand-in that the mirror then replaces), so all of them
-        // are in flight together.  Own values of planes z0-1 .. z0+ZC are
-        // clamped into the grid; a clamped plane is only read where the
-        // mirror replaces it, or not stored.
-        const int xm1 = xlo ? xi : xi - 1, xp1 = xhi ? xi : xi + 1;
+        const bool xlo = x0 == 0;
+        const bool xhi0 = x0 == NX - 1, xhi1 = x0 + 1 == NX - 1;
+        const bool ylo = yi == 0, yhi = yi == NY - 1;
+        // In-range stand-ins for outside neighbours (the mirror replaces them).
+        const int km = k > 0 ? k - 1 : 0, kp = k + 1 < NXH ? k + 1 : NXH - 1;
         const int ym1 = ylo ? yi : yi - 1, yp1 = yhi ? yi : yi + 1;
-        double own[ZC + 2], xm[ZC], xp[ZC], ym[ZC], yp[ZC], dv[ZC];
+        // Phase 1: every load in flight.  Own values of planes z0-1 .. z0+ZC
+        // (clamped into the grid; a clamped plane is only read where the
+        // mirror replaces it, or not stored).
+        double a0[ZC + 2], a1[ZC + 2];
 #pragma unroll
         for (int r = 0; r < ZC + 2; r++) {
             int z = z0 - 1 + r;
             z = z < 0 ? 0 : (z >= NZ ? NZ - 1 : z);
-            own[r] = e_at(xi, yi, z);
+            const int i = (z * NY + yi) * NXH + k;
+            a0[r] = (double)body::gp(arr(z, false))[i];
+            a1[r] = (double)body::gp(arr(z, true))[i];
         }
-        const auto d = body::gp(div);
+        double xm0[ZC], xp1[ZC], ym0[ZC], ym1v[ZC], yp0[ZC], yp1v[ZC];
+        double2 dv[ZC];
 #pragma unroll
         for (int r = 0; r < ZC; r++) {
             const int z = (z0 + r) < NZ ? z0 + r : NZ - 1;
-            xm[r] = e_at(xm1, yi, z);
-            xp[r] = e_at(xp1, yi, z);
-            ym[r] = e_at(xi, ym1, z);
-            yp[r] = e_at(xi, yp1, z);
-            dv[r] = d[(xi + g) + (yi + g) * P.sy + (z + g) * P.sz];
+            const int row = z * NY;
+            const float *c0 = arr(z, false), *c1 = arr(z, true);
+            xm0[r] = (double)body::gp(c1)[(row + yi) * NXH + km];    // cell 2k-1
+            xp1[r] = (double)body::gp(c0)[(row + yi) * NXH + kp];    // cell 2k+2
+            ym0[r] = (double)body::gp(c1)[(row + ym1) * NXH + k];    // rows y+-1 flip colour
+            ym1v[r] = (double)body::gp(c0)[(row + ym1) * NXH + k];
+            yp0[r] = (double)body::gp(c1)[(row + yp1) * NXH + k];
+            yp1v[r] = (double)body::gp(c0)[(row + yp1) * NXH + k];
+            // x0 + 1 + g < NX + 2g: the pair's second value is in the row
+            // even when that cell is outside the interior.
+            dv[r] = body::ld2(div + (x0 + g) + (yi + g) * P.sy + (z + g) * P.sz);
         }
-        // Phase 2: results in registers.
-        double res[ZC];
+        // Phase 2: results in registers, in k_residual's order.
+        double res0[ZC], res1[ZC];
 #pragma unroll
         for (int r = 0; r < ZC; r++) {
             const int zi = z0 + r;
-            const double c = own[r + 1];
-            const double s = (((((xlo ? c : xm[r]) + (xhi ? c : xp[r])) + (ylo ? c : ym[r])) +
-                               (yhi ? c : yp[r])) +
-                              (zi == 0 ? c : own[r])) +
-                             (zi >= NZ - 1 ? c : own[r + 2]);
-            res[r] = (s - h2 * dv[r]) - 6.0 * c;
-            const double a = fabs(res[r]);
-            m = (zi < NZ && a > m) ? a : m;
+            const bool zlo = zi == 0, zhi = zi >= NZ - 1;
+            const double c = a0[r + 1], d = a1[r + 1];
+            const double s0 = (((((xlo ? c : xm0[r]) + (xhi0 ? c : d)) + (ylo ? c : ym0[r])) +
+                                (yhi ? c : yp0[r])) +
+                               (zlo ? c : a0[r])) +
+                              (zhi ? c : a0[r + 2]);
+            const double s1 = (((((c) + (xhi1 ? d : xp1[r])) + (ylo ? d : ym1v[r])) +
+                                (yhi ? d : yp1v[r])) +
+                               (zlo ? d : a1[r])) +
+                              (zhi ? d : a1[r + 2]);
+            res0[r] = (s0 - h2 * dv[r].x) - 6.0 * c;
+            res1[r] = (s1 - h2 * dv[r].y) - 6.0 * d;
+            const double f0 = fabs(res0[r]), f1 = fabs(res1[r]);
+            m = (zi < NZ && f0 > m) ? f0 : m;
+            m = (zi < NZ && two && f1 > m) ? f1 : m;
         }
 #pragma unroll
-        for (int r = 0; r < ZC; r++) asm volatile("" : "+v"(res[r]));
+        for (int r = 0; r < ZC; r++) asm volatile("" : "+v"(res0[r]), "+v"(res1[r]));
         // Phase 3: stores.
         const auto p = body::gp(P.d);
 #pragma unroll
         for (int r = 0; r < ZC; r++) {
             const int zi = z0 + r;
-            if (zi < NZ) {
-                p[(xi + g) + (yi + g) * P.sy + (zi + g) * P.sz] = own[r + 1];
-                const int si = (zi * NY + yi) * NXH + (xi >> 1);
-                float *rbase = body::parity(xi, yi, zi, g) ? r_black : r_red;
-                body::gp(rbase)[si] = (float)res[r];
+            if (zi >= NZ) break;
+            double *pp = P.d + (x0 + g) + (yi + g) * P.sy + (zi + g) * P.sz;
+            if (two) {
+                typedef __attribute__((address_space(1))) body::dbl2_a8 gdbl2;
+                body::dbl2_a8 v;
+                v.x = a0[r + 1];
+                v.y = a1[r + 1];
+                *(gdbl2 *)pp = v;
+            } else {
+                p[pp - P.d] = a0[r + 1];
             }
+            const int i = (zi * NY + yi) * NXH + k;
+            const bool black0 = body::parity(0, yi, zi, g);
+            body::gp(black0 ? r_black : r_red)[i] = (float)res0[r];
+            if (two) body::gp(black0 ? r_red : r_black)[i] = (float)res1[r];
         }
     }
     // Max over the workgroup: across each 64-lane wave by shuffles, then
@@ -247,14 +276,16 @@ hipError_t init_mixed(MixedState &m, Geom G, hipStream_t s)
 {
     const size_t n = (size_t)split_count(G);
     m.np = ((G.NX + BX - 1) / BX) * ((G.NY + BY - 1) / BY);
-    m.np_merge = m.np * ((G.NZ + ZC - 1) / ZC);
+    m.np_merge = (((G.NX + 1) / 2 + BX - 1) / BX) * ((G.NY + BY - 1) / BY) *
+                 ((G.NZ + ZC - 1) / ZC);
     hipError_t err;
     float **bufs[4] = {&m.e_red, &m.e_black, &m.r_red, &m.r_black};
     for (float **b : bufs) {
         if ((err = hipMalloc(b, n * sizeof(float))) != hipSuccess) return err;
         if ((err = hipMemsetAsync(*b, 0, n * sizeof(float), s)) != hipSuccess) return err;
     }
-    if ((err = hipMalloc(&m.partials, (size_t)m.np_merge * sizeof(double))) != hipSuccess)
+    const int np_max = m.np > m.np_merge ? m.np : m.np_merge;
+    if ((err = hipMalloc(&m.partials, (size_t)np_max * sizeof(double))) != hipSuccess)
         return err;
     if ((err = hipMalloc(&m.done, 2 * sizeof(int))) != hipSuccess) return err;
     if ((err = hipMalloc(&m.last_res, sizeof(double))) != hipSuccess) return err;
@@ -295,7 +326,7 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
             return err;
         hipLaunchKernelGGL(k_mixed_count, dim3(1), dim3(1), 0, s, m.cycles);
     } else {
-        const int3 nb = make_int3((G.NX + BX - 1) / BX, (G.NY + BY - 1) / BY,
+        const int3 nb = make_int3(((G.NX + 1) / 2 + BX - 1) / BX, (G.NY + BY - 1) / BY,
                                   (G.NZ + ZC - 1) / ZC);
         hipLaunchKernelGGL(k_merge_residual, dim3((unsigned)(nb.x * nb.y * nb.z)), blk, 0, s, c.P,
                            c.D.d, m.e_red, m.e_black, m.r_red, m.r_black, G, c.h2, m.partials,

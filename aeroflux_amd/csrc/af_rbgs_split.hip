@@ -43,16 +43,19 @@ __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color
 }
 
 // Natural interior -> split (both colours).  `nat` uses the padded layout.
-template <typename TS, typename TN>
-__global__ __launch_bounds__(BX *BY) void k_to_split(const TN *nat, int sy, int sz, int g, int NX,
-                                                     int NY, int NXH, TS *red, TS *black, int3 nb)
+// Each thread converts the cell pair of split entry bx * BX + threadIdx.x
+// (body::to_split_pair); the grid covers NXH entries per row.
+__global__ __launch_bounds__(BX *BY) void k_to_split(const double *nat, int sy, int sz, int g,
+                                                     int NX, int NY, int NXH, double *red,
+                                                     double *black, int3 nb)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    body::to_split_cell<TS, TN>(nat, sy, sz, g, NX, NY, NXH, red, black, bx * BX + threadIdx.x,
-                                by * BY + threadIdx.y, bz);
+    body::to_split_pair(nat, sy, sz, g, NX, NY, NXH, red, black, bx * BX + threadIdx.x,
+                        by * BY + threadIdx.y, bz);
 }
 
+// Split -> natural interior, a cell pair per thread (body::from_split_pair).
 template <typename TS, bool ADD>
 __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int sz, int g, int NX,
                                                        int NY, int NXH, const TS *red,
@@ -61,7 +64,7 @@ __global__ __launch_bounds__(BX *BY) void k_from_split(double *nat, int sy, int 
     if (done && *done) return;
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    body::from_split_cell<TS, ADD>(nat, sy, sz, g, NX, NY, NXH, red, black, bx * BX + threadIdx.x,
+    body::from_split_pair<TS, ADD>(nat, sy, sz, g, NX, NY, NXH, red, black, bx * BX + threadIdx.x,
                                    by * BY + threadIdx.y, bz);
 }
 
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(BX *BY) void k_from_split_gated(double *nat, int sy
     for (int l = blockIdx.x; l < total; l += gridDim.x) {
         const int bx = l % nb.x, r = l / nb.x;
         const int by = r % nb.y, bz = r / nb.y;
-        body::from_split_cell<TS, ADD>(nat, sy, sz, g, NX, NY, NXH, red, black,
+        body::from_split_pair<TS, ADD>(nat, sy, sz, g, NX, NY, NXH, red, black,
                                        bx * BX + threadIdx.x, by * BY + threadIdx.y, bz);
     }
 }
@@ -93,7 +96,7 @@ __global__ __launch_bounds__(BX *BY) void k_rhs32_split(const double *div, int s
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    body::rhs32_cell(div, sy, sz, g, NX, NY, NXH, h2, red, black, bx * BX + threadIdx.x,
+    body::rhs32_pair(div, sy, sz, g, NX, NY, NXH, h2, red, black, bx * BX + threadIdx.x,
                      by * BY + threadIdx.y, bz);
 }
 
@@ -121,10 +124,11 @@ hipError_t launch_split_pass(const SplitArgs<T> &a, int color, int mode, hipStre
 template <typename TS, typename TN>
 hipError_t launch_to_split(const TN *nat, int sy, int sz, Geom G, TS *red, TS *black, hipStream_t s)
 {
+    static_assert(sizeof(TS) == 8 && sizeof(TN) == 8, "FP64 natural -> FP64 split only");
     const int NXH = (G.NX + 1) / 2;
-    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
-    hipLaunchKernelGGL((k_to_split<TS, TN>), grid1(nb), dim3(BX, BY), 0, s, nat, sy, sz, G.g, G.NX,
-                       G.NY, NXH, red, black, nb);
+    const int3 nb = nblocks(NXH, G.NY, G.NZ);
+    hipLaunchKernelGGL(k_to_split, grid1(nb), dim3(BX, BY), 0, s, nat, sy, sz, G.g, G.NX, G.NY,
+                       NXH, red, black, nb);
     return hipGetLastError();
 }
 
@@ -133,7 +137,7 @@ hipError_t launch_from_split(double *nat, int sy, int sz, Geom G, const TS *red,
                              bool add, const int *done, hipStream_t s)
 {
     const int NXH = (G.NX + 1) / 2;
-    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    const int3 nb = nblocks(NXH, G.NY, G.NZ);
     const int total = nb.x * nb.y * nb.z;
     if (done) {
         const dim3 grid((unsigned)(total < kGatedGrid ? total : kGatedGrid));
@@ -156,7 +160,7 @@ hipError_t launch_rhs32_split(const double *div, int sy, int sz, Geom G, double 
                               float *black, hipStream_t s)
 {
     const int NXH = (G.NX + 1) / 2;
-    const int3 nb = nblocks(G.NX, G.NY, G.NZ);
+    const int3 nb = nblocks(NXH, G.NY, G.NZ);
     hipLaunchKernelGGL(k_rhs32_split, grid1(nb), dim3(BX, BY), 0, s, div, sy, sz, G.g, G.NX, G.NY,
                        NXH, h2, red, black, nb);
     return hipGetLastError();
