@@ -80,6 +80,8 @@ enum Kind {
     K_COL,    // 64 x 4 interior columns (z loop inside)
     K_ENG,    // one item per slot
     K_LDS,    // LDS-staged advection tiles (64 x rows x z-chunk) + wall blocks
+    K_PAIR,   // 256-entry tiles of the split layout's planes (a cell pair per
+              // thread), kMergeZ z-planes per item (merges, the fused close)
     K_COUNT
 };
 
@@ -150,7 +152,19 @@ struct BCtl {
 // Tiles of the plane-wise stages flatten each xy-plane over a workgroup's
 // threads, so short rows (small grids) keep every lane busy: a tile is
 // `threads` consecutive points of one plane (x fastest) for one z-block.
-inline int64_t plane_tiles(int64_t plane, int threads) { return (plane + threads - 1) / threads; }
+__host__ __device__ inline int64_t plane_tiles(int64_t plane, int threads)
+{
+    return (plane + threads - 1) / threads;
+}
+
+// K_PAIR items of a slot: 256-entry tiles of an NXH x NY split plane, for
+// each group of body::kMergeZ planes.  The fused close writes one residual
+// partial per item.
+__host__ __device__ inline int64_t pair_items(const Geom &G)
+{
+    return plane_tiles((int64_t)((G.NX + 1) / 2) * G.NY, BX * BY) *
+           ((G.NZ + body::kMergeZ - 1) / body::kMergeZ);
+}
 
 inline int64_t kind_items(int kind, const Geom &G, const DField &P, int plan, int precision)
 {
@@ -179,6 +193,7 @@ inline int64_t kind_items(int kind, const Geom &G, const DField &P, int plan, in
         return (body::pressure_shell_count(P.nx, P.ny, P.nz, G.g) + CHUNK - 1) / CHUNK;
     case K_COL: return plane_tiles((int64_t)G.NX * G.NY, BX * BY);
     case K_LDS: return staged ? lds_items(G, lds) : 0;
+    case K_PAIR: return pair_items(G);
     default: return 1;
     }
 }
@@ -402,7 +417,10 @@ __global__ __launch_bounds__(BX *LBY) __attribute__((amdgpu_waves_per_eu(
 
 // computeDivergence, project.go:42-62, also writing the solve's split
 // right-hand side: the FP64 divergence itself, or the mixed solve's FP32
-// -(H^2 div).  Same values as a separate split pass over Div.
+// -(H^2 div).  Same values as a separate split pass over Div.  It also
+// writes the solve's first red pass (the first cycle's, for the mixed
+// solve): from zero, every neighbour and the mirrored own value are +0, so
+// a red cell's value depends only on its right-hand side.
 template <bool MIXED>
 __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
 {
@@ -411,6 +429,8 @@ __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
         const Geom &G = e.G;
         int xi, yi;
         const int zb = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
+        // Plane by plane (measured on the 96 engines: issuing the CZ planes'
+        // loads together, body::divergence_column, was 3% slower).
         for (int r = 0; r < CZ; r++) {
             const int zi = zb * CZ + r;
             if (zi >= G.NZ) break;
@@ -422,11 +442,24 @@ __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
                 break;
             const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
             const bool black = body::parity(xi, yi, zi, G.g);
-            if (MIXED)
-                body::gp(static_cast<float *>(black ? e.d_black : e.d_red))[i] =
-                    (float)(-(e.h2 * v));
-            else
+            // The solve's first red pass from zero (s_red): relax(+0 + ... +
+            // +0, rhs), the colour pass's own MODE 1 arithmetic.
+            if (MIXED) {
+                const float rf = (float)(-(e.h2 * v));
+                body::gp(static_cast<float *>(black ? e.d_black : e.d_red))[i] = rf;
+                if (!black) {
+                    const float z = 0.0f;
+                    body::gp(static_cast<float *>(e.s_red))[i] =
+                        body::relax<float>(z + z + z + z + z + z, rf, 0.0);
+                }
+            } else {
                 body::gp(static_cast<double *>(black ? e.d_black : e.d_red))[i] = v;
+                if (!black) {
+                    const double z = 0.0;
+                    body::gp(static_cast<double *>(e.s_red))[i] =
+                        body::relax<double>(z + z + z + z + z + z, v, e.h2);
+                }
+            }
         }
     });
 }
@@ -458,20 +491,39 @@ __global__ __launch_bounds__(BX *BY) void kb_pass(BCtl c, int color, int gated)
 template <typename TS, bool ADD>
 __global__ __launch_bounds__(BX *BY) void kb_from_split(BCtl c, int gated)
 {
-    for_items<K_CELL>(c, [&](int s, int t) {
+    for_items<K_PAIR>(c, [&](int s, int t) {
         if (gated && c.done[s]) return;
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
         const DField &P = e.f[AF_FIELD_P];
-        int xi, yi;
-        const int zb = plane_point(t, G.NX, G.NX * G.NY, BX * BY, xi, yi);
-        for (int r = 0; r < CZ; r++) {
-            const int zi = zb * CZ + r;
-            if (zi >= G.NZ) break;
-            body::from_split_cell<TS, ADD>(P.d, P.sy, P.sz, G.g, G.NX, G.NY, (G.NX + 1) / 2,
-                                           static_cast<const TS *>(e.s_red),
-                                           static_cast<const TS *>(e.s_black), xi, yi, zi);
-        }
+        const int NXH = (G.NX + 1) / 2;
+        int k, yi;
+        const int zb = plane_point(t, NXH, NXH * G.NY, BX * BY, k, yi);
+        body::from_split_pairs<TS, ADD, body::kMergeZ>(
+            P.d, P.sy, P.sz, G.g, G.NX, G.NY, G.NZ, NXH, static_cast<const TS *>(e.s_red),
+            static_cast<const TS *>(e.s_black), k, yi, zb * body::kMergeZ);
+    });
+}
+
+// The mixed solve's first-cycle close (body::merge_residual_pair): P takes
+// the FP32 correction and the FP64 residual of that P goes to the split
+// right-hand side, with one max|r| partial per item.  The same bits as
+// kb_from_split<float, false> followed by kb_resid.
+__global__ __launch_bounds__(BX *BY) void kb_merge_resid(BCtl c)
+{
+    __shared__ double sh[BX * BY];
+    for_items<K_PAIR>(c, [&](int s, int t) {
+        const BSlot &e = c.slots[s];
+        const Geom &G = e.G;
+        const int NXH = (G.NX + 1) / 2;
+        int k, yi;
+        const int zb = plane_point(t, NXH, NXH * G.NY, BX * BY, k, yi);
+        const double m = body::merge_residual_pair(
+            e.f[AF_FIELD_P], e.f[AF_FIELD_DIV].d, static_cast<const float *>(e.s_red),
+            static_cast<const float *>(e.s_black), static_cast<float *>(e.d_red),
+            static_cast<float *>(e.d_black), G, e.h2, k, yi, zb * body::kMergeZ);
+        const double b = body::block_max256(m, sh);
+        if (threadIdx.x == 0 && threadIdx.y == 0) body::gp(e.partials)[t] = b;
     });
 }
 
@@ -522,13 +574,16 @@ __global__ __launch_bounds__(BX *BY) void kb_resid(BCtl c, int gated)
 
 // Mixed solve: fold a slot's partials (fixed order), record max|div| and
 // raise the slot's done flag once its target is met.
-__global__ __launch_bounds__(256) void kb_res_final(BCtl c, int gated)
+// pairs: the partials are kb_merge_resid's (one per K_PAIR item), else
+// kb_resid's (one per K_COL column tile).
+__global__ __launch_bounds__(256) void kb_res_final(BCtl c, int gated, int pairs)
 {
     __shared__ double sh[256];
     for_items<K_ENG>(c, [&](int s, int) {
         if (gated && c.done[s]) return;
         const BSlot &e = c.slots[s];
-        const int np = (e.G.NX * e.G.NY + BX * BY - 1) / (BX * BY);
+        const int np = pairs ? (int)pair_items(e.G)
+                             : (e.G.NX * e.G.NY + BX * BY - 1) / (BX * BY);
         double v = 0;
         const auto part = body::gp((const double *)e.partials);
         for (int k = threadIdx.x; k < np; k += 256) v = part[k] > v ? part[k] : v;
@@ -580,7 +635,8 @@ bool slot_layout(const af_config &c, int precision, SlotLayout *L)
     }
     Geom G{(int)c.NX, (int)c.NY, (int)c.NZ, (int)c.Ghost};
     L->partials_off = off;
-    off += round256((int64_t)reduce_partials_needed(G) * 8);
+    const int64_t np = std::max<int64_t>(reduce_partials_needed(G), pair_items(G));
+    off += round256(np * 8);
     L->out_off = off;
     off += 256;
     L->total = off;
@@ -951,21 +1007,25 @@ hipError_t enq_solve(const af_batch *b, int ngroups)
     for (int g = 0; g < ngroups && err == hipSuccess; g++) {
         const BCtl cg = ctl(b, ngroups > 1 ? g : -1);
         const int grp = cg.grp;
+        // The divergence already wrote the first red pass (kb_div): the
+        // passes start at the first black one.
         if (!mixed) {
             const int passes = (int)(2 * b->cfg.GaussSeidelIters);
-            for (int p = 0; p < passes && err == hipSuccess; p++)
+            for (int p = 1; p < passes && err == hipSuccess; p++)
                 err = enq_pass<double>(b, b->nwg_pass, p & 1, mode(p), 0, grp);
             if (err != hipSuccess) return err;
             hipLaunchKernelGGL((kb_from_split<double, false>), gp, blk, 0, b->stream, cg, 0);
             continue;
         }
-        for (int p = 0; p < 2 * b->cfg.MixedSweeps && err == hipSuccess; p++)
+        for (int p = 1; p < 2 * b->cfg.MixedSweeps && err == hipSuccess; p++)
             err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 0, grp);
         if (err != hipSuccess) return err;
-        hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, cg, 0);
-        if (!checked) continue;
-        hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, cg, 0);
-        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, cg, 0);
+        if (!checked) {
+            hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, cg, 0);
+            continue;
+        }
+        hipLaunchKernelGGL(kb_merge_resid, gp, blk, 0, b->stream, cg);
+        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, cg, 0, 1);
     }
     if (err != hipSuccess) return err;
     // The later correction cycles (a few passes, only for slots whose target
@@ -977,7 +1037,7 @@ hipError_t enq_solve(const af_batch *b, int ngroups)
         if (err != hipSuccess) return err;
         hipLaunchKernelGGL((kb_from_split<float, true>), gp, blk, 0, b->stream, c, 1);
         hipLaunchKernelGGL(kb_resid, gp, blk, 0, b->stream, c, 1);
-        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 1);
+        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 1, 0);
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     hipLaunchKernelGGL(kb_pbc, dim3(kSmallGrid), dim3(256), 0, b->stream, c);

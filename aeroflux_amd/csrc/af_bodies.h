@@ -974,6 +974,51 @@ __device__ __forceinline__ void from_split_pair(double *nat, int sy, int sz, int
     }
 }
 
+// from_split_pair for the planes z0 .. z0+NZC-1 (interior coordinates) of
+// one pair, every plane's loads issued before the first store.
+template <typename TS, bool ADD, int NZC>
+__device__ __forceinline__ void from_split_pairs(double *nat, int sy, int sz, int g, int NX, int NY,
+                                                 int NZ, int NXH, const TS *red, const TS *black,
+                                                 int k, int yi, int z0)
+{
+    if (k >= NXH || yi >= NY) return;
+    const int x0 = 2 * k;
+    const bool two = x0 + 1 < NX;
+    double v0[NZC], v1[NZC], o0[NZC], o1[NZC];
+#pragma unroll
+    for (int r = 0; r < NZC; r++) {
+        const int zi = z0 + r < NZ ? z0 + r : NZ - 1;
+        const int i = (zi * NY + yi) * NXH + k;
+        const bool b0 = parity(0, yi, zi, g);
+        v0[r] = (double)gp(b0 ? black : red)[i];
+        v1[r] = (double)gp(b0 ? red : black)[i];
+        o0[r] = o1[r] = 0.0;
+        if (ADD) {
+            const double2 o = ld2(nat + (x0 + g) + (yi + g) * sy + (zi + g) * sz);
+            o0[r] = o.x;
+            o1[r] = o.y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NZC; r++) {
+        if (ADD) {
+            v0[r] = o0[r] + v0[r];
+            v1[r] = o1[r] + v1[r];
+        }
+        asm volatile("" : "+v"(v0[r]), "+v"(v1[r]));
+    }
+#pragma unroll
+    for (int r = 0; r < NZC; r++) {
+        const int zi = z0 + r;
+        if (zi >= NZ) break;
+        double *d = nat + (x0 + g) + (yi + g) * sy + (zi + g) * sz;
+        if (two)
+            st2(d, v0[r], v1[r]);
+        else
+            gp(nat)[(int)(d - nat)] = v0[r];
+    }
+}
+
 // FP64 residual of the mixed-precision solve along interior column (xi, yi):
 // r = (sum p~ - H^2 div) - 6 p with the Neumann mirror fused (p~ = p for a
 // ghost neighbour).  Stores r in FP32 on the split layout and returns the
@@ -1007,6 +1052,119 @@ __device__ __forceinline__ double residual_column(const DField &P, const double 
         if ((x + y + z) & 1) rb[si] = (float)res; else rr[si] = (float)res;
         const double a = fabs(res);
         m = a > m ? a : m;
+    }
+    return m;
+}
+
+// The mixed solve's first-cycle close for cell pair x = 2k, 2k+1 of rows
+// (yi, z0 .. z0+kMergeZ-1): P's interior takes the FP32 correction e
+// (p = e, exact widening), and the FP64 residual of that p,
+// r = (sum p~ - H^2 div) - 6 p with the Neumann mirror fused, is formed from
+// the same e values (a neighbour's p is its e), stored in FP32 on the split
+// layout for the next cycle.  Returns the max |r| over the cells stored.
+// The values and their operation order are those of the merge
+// (from_split_pair) followed by residual_column, so the results are the
+// same bits; the close moves 24 B per cell (e 4, Div 8, P 8, r 4) instead of
+// the 32 B of the two, and a thread issues all of its loads before the
+// first wait.
+//
+// The pair holds one cell of each colour (cell 2k has the row's parity, so
+// the colour array of each cell is uniform across a row).  Its own values,
+// its x-neighbours 2k-1 and 2k+2 (entries k-1 and k+1 of the other and the
+// same colour), its y-neighbours (entry k of both colours in rows y-1, y+1)
+// and its z-neighbours (the pair's own values in the planes beside it) are
+// whole loads per lane, and Div and P are read and written as 16-byte pairs.
+// `div` shares P's layout.
+constexpr int kMergeZ = 4;
+
+__device__ __forceinline__ double merge_residual_pair(const DField &P, const double *div,
+                                                      const float *e_red, const float *e_black,
+                                                      float *r_red, float *r_black, const Geom &G,
+                                                      double h2, int k, int yi, int z0)
+{
+    constexpr int ZC = kMergeZ;
+    const int NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2, g = G.g;
+    double m = 0;
+    if (k >= NXH || yi >= NY || z0 >= NZ) return m;
+    const int x0 = 2 * k;
+    const bool two = x0 + 1 < NX;  // the pair's second cell exists
+    // Colour arrays of cell x0 (parity of the row) and x0+1 in plane z.
+    const auto arr = [&](int z, bool second) -> const float * {
+        return (parity(0, yi, z, g) ^ (int)second) ? e_black : e_red;
+    };
+    const bool xlo = x0 == 0;
+    const bool xhi0 = x0 == NX - 1, xhi1 = x0 + 1 == NX - 1;
+    const bool ylo = yi == 0, yhi = yi == NY - 1;
+    // In-range stand-ins for outside neighbours (the mirror replaces them).
+    const int km = k > 0 ? k - 1 : 0, kp = k + 1 < NXH ? k + 1 : NXH - 1;
+    const int ym1 = ylo ? yi : yi - 1, yp1 = yhi ? yi : yi + 1;
+    // Phase 1: every load in flight.  Own values of planes z0-1 .. z0+ZC
+    // (clamped into the grid; a clamped plane is only read where the
+    // mirror replaces it, or not stored).
+    double a0[ZC + 2], a1[ZC + 2];
+#pragma unroll
+    for (int r = 0; r < ZC + 2; r++) {
+        int z = z0 - 1 + r;
+        z = z < 0 ? 0 : (z >= NZ ? NZ - 1 : z);
+        const int i = (z * NY + yi) * NXH + k;
+        a0[r] = (double)gp(arr(z, false))[i];
+        a1[r] = (double)gp(arr(z, true))[i];
+    }
+    double xm0[ZC], xp1[ZC], ym0[ZC], ym1v[ZC], yp0[ZC], yp1v[ZC];
+    double2 dv[ZC];
+#pragma unroll
+    for (int r = 0; r < ZC; r++) {
+        const int z = (z0 + r) < NZ ? z0 + r : NZ - 1;
+        const int row = z * NY;
+        const float *c0 = arr(z, false), *c1 = arr(z, true);
+        xm0[r] = (double)gp(c1)[(row + yi) * NXH + km];    // cell 2k-1
+        xp1[r] = (double)gp(c0)[(row + yi) * NXH + kp];    // cell 2k+2
+        ym0[r] = (double)gp(c1)[(row + ym1) * NXH + k];    // rows y+-1 flip colour
+        ym1v[r] = (double)gp(c0)[(row + ym1) * NXH + k];
+        yp0[r] = (double)gp(c1)[(row + yp1) * NXH + k];
+        yp1v[r] = (double)gp(c0)[(row + yp1) * NXH + k];
+        // x0 + 1 + g < NX + 2g: the pair's second value is in the row
+        // even when that cell is outside the interior.
+        dv[r] = ld2(div + (x0 + g) + (yi + g) * P.sy + (z + g) * P.sz);
+    }
+    // Phase 2: results in registers, in residual_column's order.
+    double res0[ZC], res1[ZC];
+#pragma unroll
+    for (int r = 0; r < ZC; r++) {
+        const int zi = z0 + r;
+        const bool zlo = zi == 0, zhi = zi >= NZ - 1;
+        const double c = a0[r + 1], d = a1[r + 1];
+        const double s0 = (((((xlo ? c : xm0[r]) + (xhi0 ? c : d)) + (ylo ? c : ym0[r])) +
+                            (yhi ? c : yp0[r])) +
+                           (zlo ? c : a0[r])) +
+                          (zhi ? c : a0[r + 2]);
+        const double s1 = (((((c) + (xhi1 ? d : xp1[r])) + (ylo ? d : ym1v[r])) +
+                            (yhi ? d : yp1v[r])) +
+                           (zlo ? d : a1[r])) +
+                          (zhi ? d : a1[r + 2]);
+        res0[r] = (s0 - h2 * dv[r].x) - 6.0 * c;
+        res1[r] = (s1 - h2 * dv[r].y) - 6.0 * d;
+        const double f0 = fabs(res0[r]), f1 = fabs(res1[r]);
+        m = (zi < NZ && f0 > m) ? f0 : m;
+        m = (zi < NZ && two && f1 > m) ? f1 : m;
+    }
+#pragma unroll
+    for (int r = 0; r < ZC; r++) asm volatile("" : "+v"(res0[r]), "+v"(res1[r]));
+    // Phase 3: stores.
+    const auto p = gp(P.d);
+#pragma unroll
+    for (int r = 0; r < ZC; r++) {
+        const int zi = z0 + r;
+        if (zi >= NZ) break;
+        double *pp = P.d + (x0 + g) + (yi + g) * P.sy + (zi + g) * P.sz;
+        if (two)
+            st2(pp, a0[r + 1], a1[r + 1]);
+        else
+            p[pp - P.d] = a0[r + 1];
+        const int i = (zi * NY + yi) * NXH + k;
+        const bool black0 = parity(0, yi, zi, g);
+        gp(black0 ? r_black : r_red)[i] = (float)res0[r];
+        if (two) gp(black0 ? r_red : r_black)[i] = (float)res1[r];
     }
     return m;
 }

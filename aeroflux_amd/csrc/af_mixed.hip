@@ -45,25 +45,11 @@ __global__ __launch_bounds__(BX *BY) void k_residual(DField P, const double *__r
     if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = b;
 }
 
-// The first cycle's close in one pass: P's interior takes the FP32
-// correction (p = e, exact widening), and the FP64 residual of that p,
-// r = (sum p~ - H^2 div) - 6 p with the Neumann mirror fused, is formed from
-// the same e values (a neighbour's p is its e), stored in FP32 on the split
-// layout for the next cycle, and folded into a per-workgroup max|r|.  The
-// values and their operation order are those of the merge followed by
-// k_residual, so the results are the same bits; the pass moves 24 B per cell
-// (e 4, Div 8, P 8, r 4) instead of the 32 B of the two, and it issues all
-// of a thread's loads before the first wait.
-//
-// A thread owns split entry k of row (y, z) for ZC planes: the cell pair
-// x = 2k, 2k+1, one of each colour (cell 2k has the row's parity, so the
-// colour array of each cell is uniform across the wave).  Its own values,
-// its x-neighbours 2k-1 and 2k+2 (entries k-1 and k+1 of the other and the
-// same colour), its y-neighbours (entry k of both colours in rows y-1, y+1)
-// and its z-neighbours (the pair's own values in the planes beside it) are
-// whole loads per lane, and Div and P are read and written as 16-byte pairs.
-// A workgroup owns 64 x 4 entries of (x, y) and ZC planes.
-constexpr int ZC = 4;
+// The first cycle's close in one pass (body::merge_residual_pair): P's
+// interior takes the FP32 correction and the FP64 residual of that P is
+// stored in FP32 for the next cycle, with a per-workgroup max|r|.  A
+// workgroup owns 64 x 4 split entries of (x, y) and ZC planes.
+constexpr int ZC = body::kMergeZ;
 
 __global__ __launch_bounds__(BX *BY) void k_merge_residual(DField P, const double *__restrict__ div,
                                                            const float *__restrict__ e_red,
@@ -74,96 +60,8 @@ __global__ __launch_bounds__(BX *BY) void k_merge_residual(DField P, const doubl
     __shared__ double sh[BY];
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2, g = G.g;
-    const int k = bx * BX + threadIdx.x, yi = by * BY + threadIdx.y, z0 = bz * ZC;
-    double m = 0;
-    if (k < NXH && yi < NY) {
-        const int x0 = 2 * k;
-        const bool two = x0 + 1 < NX;  // the pair's second cell exists
-        // Colour arrays of cell x0 (parity of the row) and x0+1 in plane z.
-        const auto arr = [&](int z, bool second) -> const float * {
-            return (body::parity(0, yi, z, g) ^ (int)second) ? e_black : e_red;
-        };
-        const bool xlo = x0 == 0;
-        const bool xhi0 = x0 == NX - 1, xhi1 = x0 + 1 == NX - 1;
-        const bool ylo = yi == 0, yhi = yi == NY - 1;
-        // In-range stand-ins for outside neighbours (the mirror replaces them).
-        const int km = k > 0 ? k - 1 : 0, kp = k + 1 < NXH ? k + 1 : NXH - 1;
-        const int ym1 = ylo ? yi : yi - 1, yp1 = yhi ? yi : yi + 1;
-        // Phase 1: every load in flight.  Own values of planes z0-1 .. z0+ZC
-        // (clamped into the grid; a clamped plane is only read where the
-        // mirror replaces it, or not stored).
-        double a0[ZC + 2], a1[ZC + 2];
-#pragma unroll
-        for (int r = 0; r < ZC + 2; r++) {
-            int z = z0 - 1 + r;
-            z = z < 0 ? 0 : (z >= NZ ? NZ - 1 : z);
-            const int i = (z * NY + yi) * NXH + k;
-            a0[r] = (double)body::gp(arr(z, false))[i];
-            a1[r] = (double)body::gp(arr(z, true))[i];
-        }
-        double xm0[ZC], xp1[ZC], ym0[ZC], ym1v[ZC], yp0[ZC], yp1v[ZC];
-        double2 dv[ZC];
-#pragma unroll
-        for (int r = 0; r < ZC; r++) {
-            const int z = (z0 + r) < NZ ? z0 + r : NZ - 1;
-            const int row = z * NY;
-            const float *c0 = arr(z, false), *c1 = arr(z, true);
-            xm0[r] = (double)body::gp(c1)[(row + yi) * NXH + km];    // cell 2k-1
-            xp1[r] = (double)body::gp(c0)[(row + yi) * NXH + kp];    // cell 2k+2
-            ym0[r] = (double)body::gp(c1)[(row + ym1) * NXH + k];    // rows y+-1 flip colour
-            ym1v[r] = (double)body::gp(c0)[(row + ym1) * NXH + k];
-            yp0[r] = (double)body::gp(c1)[(row + yp1) * NXH + k];
-            yp1v[r] = (double)body::gp(c0)[(row + yp1) * NXH + k];
-            // x0 + 1 + g < NX + 2g: the pair's second value is in the row
-            // even when that cell is outside the interior.
-            dv[r] = body::ld2(div + (x0 + g) + (yi + g) * P.sy + (z + g) * P.sz);
-        }
-        // Phase 2: results in registers, in k_residual's order.
-        double res0[ZC], res1[ZC];
-#pragma unroll
-        for (int r = 0; r < ZC; r++) {
-            const int zi = z0 + r;
-            const bool zlo = zi == 0, zhi = zi >= NZ - 1;
-            const double c = a0[r + 1], d = a1[r + 1];
-            const double s0 = (((((xlo ? c : xm0[r]) + (xhi0 ? c : d)) + (ylo ? c : ym0[r])) +
-                                (yhi ? c : yp0[r])) +
-                               (zlo ? c : a0[r])) +
-                              (zhi ? c : a0[r + 2]);
-            const double s1 = (((((c) + (xhi1 ? d : xp1[r])) + (ylo ? d : ym1v[r])) +
-                                (yhi ? d : yp1v[r])) +
-                               (zlo ? d : a1[r])) +
-                              (zhi ? d : a1[r + 2]);
-            res0[r] = (s0 - h2 * dv[r].x) - 6.0 * c;
-            res1[r] = (s1 - h2 * dv[r].y) - 6.0 * d;
-            const double f0 = fabs(res0[r]), f1 = fabs(res1[r]);
-            m = (zi < NZ && f0 > m) ? f0 : m;
-            m = (zi < NZ && two && f1 > m) ? f1 : m;
-        }
-#pragma unroll
-        for (int r = 0; r < ZC; r++) asm volatile("" : "+v"(res0[r]), "+v"(res1[r]));
-        // Phase 3: stores.
-        const auto p = body::gp(P.d);
-#pragma unroll
-        for (int r = 0; r < ZC; r++) {
-            const int zi = z0 + r;
-            if (zi >= NZ) break;
-            double *pp = P.d + (x0 + g) + (yi + g) * P.sy + (zi + g) * P.sz;
-            if (two) {
-                typedef __attribute__((address_space(1))) body::dbl2_a8 gdbl2;
-                body::dbl2_a8 v;
-                v.x = a0[r + 1];
-                v.y = a1[r + 1];
-                *(gdbl2 *)pp = v;
-            } else {
-                p[pp - P.d] = a0[r + 1];
-            }
-            const int i = (zi * NY + yi) * NXH + k;
-            const bool black0 = body::parity(0, yi, zi, g);
-            body::gp(black0 ? r_black : r_red)[i] = (float)res0[r];
-            if (two) body::gp(black0 ? r_red : r_black)[i] = (float)res1[r];
-        }
-    }
+    double m = body::merge_residual_pair(P, div, e_red, e_black, r_red, r_black, G, h2,
+                                         bx * BX + threadIdx.x, by * BY + threadIdx.y, bz * ZC);
     // Max over the workgroup: across each 64-lane wave by shuffles, then
     // over the four waves (max is exact in any order).
 #pragma unroll

@@ -54,34 +54,28 @@ def schedule_bytes(kind: str, iters: int = 24) -> int:
     O(N^2) ghost fills not counted):
 
     * advection: 6 sweeps x (read U, V, W + write Tmp0..2) = 288;
-    * fp64_split (batches), per projection: divergence 40 (U, V, W 24, Div 8,
-      split right-hand side 8), first pass from zero 8, second pass 12,
-      2*iters-2 passes of 12, merge into P 16, gradient 56 = 108 + 24*iters;
-    * fp64_split_engine (one engine): the same, except that the divergence
-      also writes the first pass's red values (4 more, and no first-pass
-      kernel: 8 fewer, per projection), and in the first projection Div is
-      not stored (nothing reads it before the second projection rewrites
-      it: 8 fewer) and the gradient reads the split solution and skips the
-      merge (16 fewer);
+    * fp64_split (batches), per projection: divergence 44 (U, V, W 24, Div 8,
+      split right-hand side 8, and the solve's first red pass, which depends
+      only on the right-hand side, 4), second pass 12, 2*iters-2 passes of
+      12, merge into P 16, gradient 56 = 104 + 24*iters;
+    * fp64_split_engine (one engine): the same, except that in the first
+      projection Div is not stored (nothing reads it before the second
+      projection rewrites it: 8 fewer) and the gradient reads the split
+      solution and skips the merge (16 fewer);
     * fp64_arena (small grids): divergence 32, first pass 16, 2*iters-1
       passes of 24, gradient 56 = 80 + 48*iters;
-    * mixed (one FP32 cycle), per projection: divergence 36 (FP32 rhs 4),
-      first pass 4, second 6, 2*iters-2 passes of 6, merge 12, gradient 56
-      = 102 + 12*iters;
+    * mixed (one FP32 cycle), per projection: divergence 38 (FP32 rhs 4,
+      first FP32 red pass 2), second pass 6, 2*iters-2 passes of 6, merge
+      12, gradient 56 = 100 + 12*iters;
     * mixed_checked: the same with the first cycle closed by the fused merge
       and FP64 residual (24 instead of 12); later cycles, which run only when
       the check asks for them, are not counted (a lower bound);
-    * mixed_engine, mixed_checked_engine (one engine): the divergence also
-      writes the first FP32 pass's red values (2 more, and no first-pass
-      kernel: 4 fewer, per projection);
     * direct (FFT passes): divergence folded into the first pass, five
       passes 80, gradient 56, per projection 32 + 80 + 56."""
     if kind == "fp64_split_engine":
-        return schedule_bytes("fp64_split", iters) - 16 - 8 - 2 * 4
-    if kind.endswith("_engine"):
-        return schedule_bytes(kind[:-len("_engine")], iters) - 2 * 2
-    proj = {"fp64_split": 108 + 24 * iters, "fp64_arena": 80 + 48 * iters,
-            "mixed": 102 + 12 * iters, "mixed_checked": 114 + 12 * iters,
+        return schedule_bytes("fp64_split", iters) - 16 - 8
+    proj = {"fp64_split": 104 + 24 * iters, "fp64_arena": 80 + 48 * iters,
+            "mixed": 100 + 12 * iters, "mixed_checked": 112 + 12 * iters,
             "direct": 32 + 80 + 56}[kind]
     return 288 + 2 * proj
 
@@ -386,8 +380,7 @@ def big_leg(af, n: int, steps: int) -> dict:
         out[name] = {"value": n ** 3 * (steps - 1) / el, "unit": "cell-updates/s",
                      "ms_per_step": 1e3 * el / (steps - 1),
                      "final_max_divergence": e.MaxDivergence()}
-        kind = {"fp64": "fp64_split_engine", "mixed": "mixed_engine"}.get(
-            name, "mixed_checked_engine")
+        kind = {"fp64": "fp64_split_engine", "mixed": "mixed"}.get(name, "mixed_checked")
         out[name].update(bandwidth(out[name]["value"], schedule_bytes(kind), kind))
         if name == "mixed_to_target":
             out[name]["cycles_last_solve"] = e.mixed_stats()[1]
@@ -868,7 +861,7 @@ def main() -> int:
             "final_max_divergence_rel_diff": (dmx - d64) / d64,
             "cycles_last_solve": cyc,
             "max_rel_error_vs_fp64": errs,
-            **bandwidth(mval, schedule_bytes("mixed_engine", iters), "mixed_engine"),
+            **bandwidth(mval, schedule_bytes("mixed", iters), "mixed"),
         }
         # Iterated to the FP64 path's final max|div|: a short first cycle, then
         # correction cycles until the FP64 residual says the projection's
@@ -897,8 +890,7 @@ def main() -> int:
                         "still above the FP64 run's final value",
             "cycles_last_solve": tcyc,
             "max_rel_error_vs_fp64": terrs,
-            **bandwidth(tval, schedule_bytes("mixed_checked_engine", iters),
-                        "mixed_checked_engine"),
+            **bandwidth(tval, schedule_bytes("mixed_checked", iters), "mixed_checked"),
         }
     del fp64_fields
 
