@@ -284,7 +284,8 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
         t_vbc, t_pbc, t_merge = b.time_stage(4, 10), b.time_stage(7, 10), b.time_stage(8, 10)
         t_res = b.time_stage(9, 10) if prec == af.PRECISION_MIXED else 0.0
         t_div, t_grad = b.time_stage(2, 10), b.time_stage(3, 10)
-        npass = 2 * (bc_extra.get("MixedSweeps", 24) if prec == af.PRECISION_MIXED else 24)
+        # The divergence writes the first red pass; the solve launches the rest.
+        npass = 2 * (bc_extra.get("MixedSweeps", 24) if prec == af.PRECISION_MIXED else 24) - 1
         model = t_vbc + 2 * sum(sw) + 2 * (t_div + npass * t_pass + t_merge + t_pbc + t_grad)
         info = b.info()
         pass_bytes = (6 if prec == af.PRECISION_MIXED else 12) * cells
@@ -313,7 +314,7 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
                        "pressure_bc_ms": t_pbc, "residual_ms": t_res,
                        "one_cycle_step_model_ms": model,
                        "model": "1 velocity BC + 2 x (X+Y+Z sweeps, which apply the BC to "
-                                "their output) + 2 x (divergence + "
+                                "their output) + 2 x (divergence, which writes the first red pass, + "
                                 f"{npass} passes + merge + pressure BC + gradient, which applies the BC)"},
             "tuned": {"sweep_shape": info["SweepShape"], "sweep_staged": info["SweepStaged"],
                       "staged_slots": info["StagedSlots"],
