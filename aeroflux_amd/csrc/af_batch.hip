@@ -420,8 +420,10 @@ __global__ __launch_bounds__(BX *LBY) __attribute__((amdgpu_waves_per_eu(
 // -(H^2 div).  Same values as a separate split pass over Div.  It also
 // writes the solve's first red pass (the first cycle's, for the mixed
 // solve): from zero, every neighbour and the mirrored own value are +0, so
-// a red cell's value depends only on its right-hand side.
-template <bool MIXED>
+// a red cell's value depends only on its right-hand side.  DIV false skips
+// the store to Div (a Div nothing reads before the next projection's
+// divergence rewrites it).
+template <bool MIXED, bool DIV>
 __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
 {
     for_items<K_CELL>(c, [&](int s, int t) {
@@ -438,7 +440,7 @@ __global__ __launch_bounds__(BX *BY) void kb_div(BCtl c, int sw)
             // (false: x or y outside the interior, the same on every plane)
             if (!body::divergence_cell(velf(e, sw, 0), velf(e, sw, 1), velf(e, sw, 2),
                                        e.f[AF_FIELD_DIV], G, e.div_scale, xi + G.g, yi + G.g,
-                                       zi + G.g, v))
+                                       zi + G.g, v, DIV))
                 break;
             const int i = (zi * G.NY + yi) * ((G.NX + 1) / 2) + (xi >> 1);
             const bool black = body::parity(xi, yi, zi, G.g);
@@ -971,13 +973,18 @@ hipError_t enq_vbc(const af_batch *b, int sw)
     return hipGetLastError();
 }
 
-hipError_t enq_div(const af_batch *b, int sw)
+hipError_t enq_div(const af_batch *b, int sw, bool store_div = true)
 {
     const dim3 grid(b->nwg_other), blk(BX, BY);
-    if (b->cfg.Precision == AF_PRECISION_MIXED)
-        hipLaunchKernelGGL(kb_div<true>, grid, blk, 0, b->stream, ctl(b), sw);
+    const bool mixed = b->cfg.Precision == AF_PRECISION_MIXED;
+    if (mixed && store_div)
+        hipLaunchKernelGGL((kb_div<true, true>), grid, blk, 0, b->stream, ctl(b), sw);
+    else if (mixed)
+        hipLaunchKernelGGL((kb_div<true, false>), grid, blk, 0, b->stream, ctl(b), sw);
+    else if (store_div)
+        hipLaunchKernelGGL((kb_div<false, true>), grid, blk, 0, b->stream, ctl(b), sw);
     else
-        hipLaunchKernelGGL(kb_div<false>, grid, blk, 0, b->stream, ctl(b), sw);
+        hipLaunchKernelGGL((kb_div<false, false>), grid, blk, 0, b->stream, ctl(b), sw);
     return hipGetLastError();
 }
 
@@ -1044,13 +1051,20 @@ hipError_t enq_solve(const af_batch *b, int ngroups)
     return hipGetLastError();
 }
 
-// project, project.go:12-34.
 // project, project.go:12-34.  The velocity already holds the BC the
 // projection starts with (the sweep before it applied it), and the gradient
-// subtraction applies the closing one.
-hipError_t enq_project(const af_batch *b, int sw, int ngroups)
+// subtraction applies the closing one.  div_dead (the step's first
+// projection): nothing reads Div before the second projection's divergence
+// rewrites it, so unless the mixed solve is checked (its residual reads
+// Div), Div is not stored.  (The single engine's first projection also
+// leaves the solution in the split arrays for its gradient; the batched
+// gradient reading the split arrays, one position per work item, measured
+// 839 us (FP64) and 1008 us (FP32) against 585 us for kb_grad on the 96
+// engines, more than the merge and pressure BC it saves.)
+hipError_t enq_project(const af_batch *b, int sw, int ngroups, bool div_dead = false)
 {
-    hipError_t err = enq_div(b, sw);
+    const bool checked = b->cfg.Precision == AF_PRECISION_MIXED && b->cfg.MixedChecked != 0;
+    hipError_t err = enq_div(b, sw, !(div_dead && !checked));
     if (err == hipSuccess) err = enq_solve(b, ngroups);
     if (err == hipSuccess) err = enq_grad(b, sw);  // applies the closing velocity BC
     return err;
@@ -1070,7 +1084,7 @@ hipError_t enq_step(const af_batch *b, int ngroups)
         err = enq_sweep(b, b->shape, b->nwg_sweep, fwd[k], sw);
         sw ^= 1;
     }
-    if (err == hipSuccess) err = enq_project(b, sw, ngroups);
+    if (err == hipSuccess) err = enq_project(b, sw, ngroups, true);
     for (int k = 0; k < 3 && err == hipSuccess; k++) {
         err = enq_sweep(b, b->shape, b->nwg_sweep, rev[k], sw);
         sw ^= 1;

@@ -647,11 +647,12 @@ __device__ __forceinline__ void gradient_column(const DField &U, const DField &V
 // (project.go:139-163) fills with the interior cell beside them -- here the
 // split entry at the clamped coordinate.  So the values, and the bits, are
 // those of merging the solution into P, applying the pressure BC and then
-// gradient_column.
-template <bool BC, int ZG>
+// gradient_column.  TS = float reads the mixed solve's FP32 correction,
+// whose widening is exact: the values merging it into P would give.
+template <bool BC, int ZG, typename TS = double>
 __device__ __forceinline__ void gradient_column_split(const DField &U, const DField &V,
-                                                      const DField &W, const double *red,
-                                                      const double *black, const Geom &G,
+                                                      const DField &W, const TS *red,
+                                                      const TS *black, const Geom &G,
                                                       double scale, int x, int y, int z0)
 {
     const int g = G.g;
@@ -664,7 +665,7 @@ __device__ __forceinline__ void gradient_column_split(const DField &U, const DFi
         yi = yi < 0 ? 0 : (yi >= NY ? NY - 1 : yi);
         zi = zi < 0 ? 0 : (zi >= NZ ? NZ - 1 : zi);
         const int i = (zi * NY + yi) * NXH + (xi >> 1);
-        return ((xi + yi + zi + 3 * g) & 1) ? sb[i] : sr[i];  // colour parity, red = 0
+        return (double)(((xi + yi + zi + 3 * g) & 1) ? sb[i] : sr[i]);  // parity, red = 0
     };
     const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
     const int zl = g + NZ;  // last position (W's wall face)

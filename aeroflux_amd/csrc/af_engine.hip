@@ -216,9 +216,9 @@ bool fused_rhs(const af_engine *e)
 
 // computeDivergence; with `fuse` also the solve's split right-hand side
 // and the solve's first (red) pass, whose values depend only on that
-// right-hand side (k_divergence<3>, or <2> for the mixed solve's FP32 pass).  div_dead
-// (FP64 colour-split path only): Div itself is not stored, because nothing
-// reads it before the next projection's divergence overwrites it.
+// right-hand side (k_divergence<3>, or <2> for the mixed solve's FP32 pass).
+// div_dead (with fuse): Div itself is not stored, because nothing reads it
+// before the next projection's divergence overwrites it.
 hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false, bool div_dead = false)
 {
     int split = 0;
@@ -229,14 +229,15 @@ hipError_t enq_divergence(af_engine *e, const Desc &d, bool fuse = false, bool d
         black = e->mixed.r_black;
         first = e->mixed.e_red;
     } else if (fuse && e->tile64.k == 0) {
-        split = div_dead ? 4 : 3;
+        split = 3;
         red = e->ds_red;
         black = e->ds_black;
         first = e->ps_red;
     }
     return launch_divergence(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
                              dfield(e, d.f[AF_FIELD_W]), dfield(e, d.f[AF_FIELD_DIV]), e->G,
-                             e->div_scale, e->stream, split, red, black, e->h2, first);
+                             e->div_scale, e->stream, split, red, black, e->h2, first,
+                             !(div_dead && split >= 2));
 }
 
 // subtractGradient; with `velocity_bc` also the velocity BC that follows it
@@ -348,7 +349,7 @@ hipError_t enq_solve_fp64(af_engine *e, const Desc &d, bool rhs_ready)
 // rhs_ready: the divergence of this projection already wrote the split
 // right-hand side (enq_divergence with fuse), and on the FP64 colour-split
 // path also the solve's first red pass.
-hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
+hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false, bool merge = true)
 {
     if (e->direct.ready()) {
         int bs = 0;
@@ -367,6 +368,7 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
         c.target = e->cfg.MixedTarget;
         c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
         c.rhs_ready = rhs_ready;
+        c.merge = merge;
         c.slab = e->pass_slab;
         return enqueue_mixed_solve(e->mixed, c, e->stream);
     }
@@ -377,15 +379,18 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false)
 // velocity BC itself (body::gradient_cell), which is the same values as the
 // separate launch.  bc_applied: the velocity already holds the opening BC
 // (the sweep before it applied the BC to its output).  p_dead: nothing reads
-// P after this projection's gradient before the next solve overwrites it
-// (the step's first projection), so on the FP64 colour-split path the
-// solution is not merged into P: the gradient reads it from the split
-// arrays (launch_gradient_split, the same values).
+// P or Div after this projection's gradient before the next projection
+// overwrites them (the step's first projection), so on the FP64
+// colour-split path and on the one-cycle mixed path Div is not stored and
+// the solution is not merged into P: the gradient reads it from the split
+// arrays (launch_gradient_split; the FP64 solution, or the FP32 correction,
+// which is the whole of the one-cycle solution: the same values).
 hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, bool p_dead = false)
 {
     hipError_t err = bc_applied ? hipSuccess : enq_velocity_bc(e, d);
+    const bool mixed = e->cfg.Precision == AF_PRECISION_MIXED;
     const bool split_tail = p_dead && !e->direct.ready() &&
-                            e->cfg.Precision != AF_PRECISION_MIXED && e->tile64.k == 0;
+                            (mixed ? e->cfg.MixedTarget <= 0 : e->tile64.k == 0);
     if (e->direct.method == DIRECT_FFT) {
         // The direct solve's first pass computes the divergence itself (the
         // same values, stored to Div), which saves re-reading Div.
@@ -397,16 +402,22 @@ hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, boo
     } else {
         const bool fuse = fused_rhs(e);
         if (err == hipSuccess) err = enq_divergence(e, d, fuse, split_tail);
-        if (err == hipSuccess && split_tail)
+        if (err == hipSuccess && split_tail && !mixed)
             err = enq_solve_fp64_split(e, dfield(e, d.f[AF_FIELD_P]),
                                        dfield(e, d.f[AF_FIELD_DIV]), fuse, false);
         else if (err == hipSuccess)
-            err = enq_solve(e, d, fuse);
+            err = enq_solve(e, d, fuse, !split_tail);
     }
+    if (err == hipSuccess && split_tail && mixed)
+        return launch_gradient_split<float>(dfield(e, d.f[AF_FIELD_U]),
+                                            dfield(e, d.f[AF_FIELD_V]),
+                                            dfield(e, d.f[AF_FIELD_W]), e->mixed.e_red,
+                                            e->mixed.e_black, e->G, e->grad_scale, e->stream);
     if (err == hipSuccess && split_tail)
-        return launch_gradient_split(dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
-                                     dfield(e, d.f[AF_FIELD_W]), e->ps_red, e->ps_black, e->G,
-                                     e->grad_scale, e->stream);
+        return launch_gradient_split<double>(dfield(e, d.f[AF_FIELD_U]),
+                                             dfield(e, d.f[AF_FIELD_V]),
+                                             dfield(e, d.f[AF_FIELD_W]), e->ps_red, e->ps_black,
+                                             e->G, e->grad_scale, e->stream);
     if (err == hipSuccess) err = enq_gradient(e, d, true);
     return err;
 }

@@ -49,11 +49,12 @@ hipError_t launch_sweep(int axis, const SweepArgs &a, hipStream_t s);
 // from zero into `first` (float, the correction's red array); 3 -> as 1, and
 // also the solve's first colour pass from zero into `first` (double, the
 // solution's red array): each red cell's value depends only on its own
-// right-hand side; 4 -> as 3 without the store to D (a Div that nothing
-// reads before the next divergence overwrites it).
+// right-hand side.  store_div false (split 2 or 3 only) skips the store to
+// D, for a Div that nothing reads before the next divergence overwrites it.
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
                              hipStream_t s, int split = 0, void *red = nullptr,
-                             void *black = nullptr, double h2 = 0, void *first = nullptr);
+                             void *black = nullptr, double h2 = 0, void *first = nullptr,
+                             bool store_div = true);
 // Red-black colour pass.  fused_bc: a neighbour outside the interior reads the
 // cell's own value (the Neumann mirror applied just before the pass).
 hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fused_bc,
@@ -68,11 +69,13 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s);
 // that follows subtractGradient in project), so that launch can be skipped.
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
                            hipStream_t s, bool velocity_bc = false);
-// subtractGradient with the velocity BC, P read from the FP64 colour-split
-// solution (red, black) instead of P: the values of merging it into P and
-// applying the pressure BC first.
-hipError_t launch_gradient_split(DField U, DField V, DField W, const double *red,
-                                 const double *black, Geom G, double scale, hipStream_t s);
+// subtractGradient with the velocity BC, P read from the colour-split
+// solution (red, black: the FP64 solution, or the mixed solve's FP32
+// correction after its one cycle) instead of P: the values of merging it
+// into P and applying the pressure BC first.
+template <typename TS>
+hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
+                                 Geom G, double scale, hipStream_t s);
 hipError_t launch_set_int(int *p, int v, hipStream_t s);
 
 // Reductions: two launches, fixed tree, no atomics.  `partials` must hold

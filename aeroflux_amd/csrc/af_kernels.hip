@@ -43,15 +43,15 @@ __global__ __launch_bounds__(256) void k_velocity_bc(DField U, DField V, DField 
 // in the first pass, and the cell's own value that stands in for a mirrored
 // ghost, is +0, and the cell's new value is relax(+0 + ... + +0, div) -- the
 // colour pass's own arithmetic (body::split_pass_column, MODE 1) on the
-// value just formed.  SPLIT 4 is SPLIT 3 without the store to Div, for a
-// projection whose Div is overwritten before anything reads it.  SPLIT 2
-// likewise writes the first FP32 red pass of the mixed solve's first cycle
-// (from e == 0) into `first` (the correction's red array).
+// value just formed.  SPLIT 2 likewise writes the first FP32 red pass of
+// the mixed solve's first cycle (from e == 0) into `first` (the
+// correction's red array).  DIV false skips the store to Div, for a
+// projection whose Div is overwritten before anything reads it.
 //
 // Each thread forms the cell pair x = 2k, 2k+1 of its row (body::
 // divergence_pair), one cell of each colour: Div is written as a 16-byte
 // pair and the split arrays take entry k of both colours.
-template <int SPLIT>
+template <int SPLIT, bool DIV>
 __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DField W, DField D,
                                                        Geom G, double scale, int3 nb, void *red,
                                                        void *black, double h2, void *first)
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(BX *BY) void k_divergence(DField U, DField V, DFiel
     double v0, v1;
     if (!body::divergence_pair(U, V, W, G, scale, k, yi, zi, v0, v1)) return;
     const bool two = 2 * k + 1 < G.NX;
-    if (SPLIT != 4) {
+    if (DIV) {
         double *d = D.d + body::idx(D, g + 2 * k, g + yi, g + zi);
         if (two)
             body::st2(d, v0, v1);
@@ -190,16 +190,17 @@ __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField 
 }
 
 // The same with P read from the colour-split solution (body::gradient_column_split).
+template <typename TS>
 __global__ __launch_bounds__(BX *BY) void k_gradient_split(DField U, DField V, DField W,
-                                                           const double *red, const double *black,
-                                                           Geom G, double scale, int3 nb)
+                                                           const TS *red, const TS *black, Geom G,
+                                                           double scale, int3 nb)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     const int g = G.g;
-    body::gradient_column_split<true, GZ>(U, V, W, red, black, G, scale,
-                                          g + bx * BX + threadIdx.x, g + by * BY + threadIdx.y,
-                                          g + bz * GZ);
+    body::gradient_column_split<true, GZ, TS>(U, V, W, red, black, G, scale,
+                                              g + bx * BX + threadIdx.x,
+                                              g + by * BY + threadIdx.y, g + bz * GZ);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -320,32 +321,27 @@ hipError_t launch_velocity_bc(DField U, DField V, DField W, Geom G, hipStream_t 
 
 hipError_t launch_divergence(DField U, DField V, DField W, DField D, Geom G, double scale,
                              hipStream_t s, int split, void *red, void *black, double h2,
-                             void *first)
+                             void *first, bool store_div)
 {
     const int3 nb = nblocks((G.NX + 1) / 2, G.NY, G.NZ);
-    if (split >= 2 && !first) return hipErrorInvalidValue;
-    switch (split) {
-    case 0:
-        hipLaunchKernelGGL(k_divergence<0>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2, first);
-        break;
-    case 1:
-        hipLaunchKernelGGL(k_divergence<1>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2, first);
-        break;
-    case 2:
-        hipLaunchKernelGGL(k_divergence<2>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2, first);
-        break;
-    case 3:
-        hipLaunchKernelGGL(k_divergence<3>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2, first);
-        break;
-    default:
-        hipLaunchKernelGGL(k_divergence<4>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale,
-                           nb, red, black, h2, first);
-        break;
-    }
+    if (split < 0 || split > 3 || (split >= 2 && !first) || (!store_div && split < 2))
+        return hipErrorInvalidValue;
+#define AF_DIV_LAUNCH(S, DV)                                                                      \
+    hipLaunchKernelGGL((k_divergence<S, DV>), grid1(nb), dim3(BX, BY), 0, s, U, V, W, D, G, scale, \
+                       nb, red, black, h2, first)
+    if (split == 0)
+        AF_DIV_LAUNCH(0, true);
+    else if (split == 1)
+        AF_DIV_LAUNCH(1, true);
+    else if (split == 2 && store_div)
+        AF_DIV_LAUNCH(2, true);
+    else if (split == 2)
+        AF_DIV_LAUNCH(2, false);
+    else if (store_div)
+        AF_DIV_LAUNCH(3, true);
+    else
+        AF_DIV_LAUNCH(3, false);
+#undef AF_DIV_LAUNCH
     return hipGetLastError();
 }
 
@@ -394,14 +390,19 @@ hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_gradient_split(DField U, DField V, DField W, const double *red,
-                                 const double *black, Geom G, double scale, hipStream_t s)
+template <typename TS>
+hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
+                                 Geom G, double scale, hipStream_t s)
 {
     const int3 nb = nblocks(G.NX, G.NY, (G.NZ + 1 + GZ - 1) / GZ);  // z = g .. g+NZ
-    hipLaunchKernelGGL(k_gradient_split, grid1(nb), dim3(BX, BY), 0, s, U, V, W, red, black, G,
-                       scale, nb);
+    hipLaunchKernelGGL(k_gradient_split<TS>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, red, black,
+                       G, scale, nb);
     return hipGetLastError();
 }
+template hipError_t launch_gradient_split<double>(DField, DField, DField, const double *,
+                                                  const double *, Geom, double, hipStream_t);
+template hipError_t launch_gradient_split<float>(DField, DField, DField, const float *,
+                                                 const float *, Geom, double, hipStream_t);
 
 hipError_t launch_set_int(int *p, int v, hipStream_t s)
 {

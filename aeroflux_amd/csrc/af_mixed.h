@@ -31,6 +31,10 @@ struct MixedCtx {
     // The split right-hand side is already in r_red/r_black, and the first
     // cycle's first red pass in e_red (k_divergence<2>).
     bool rhs_ready = false;
+    // One cycle only (target 0): false leaves P unwritten (no merge, no
+    // pressure BC) for a caller that reads the correction e, the whole of
+    // the solution, from the split arrays.
+    bool merge = true;
     int slab = 1 << 30;      // workgroup y-slab order of the FP32 passes (SplitArgs::slab)
 };
 

@@ -209,6 +209,7 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
     const dim3 blk(BX, BY);
     const dim3 gcol((G.NX + BX - 1) / BX, (G.NY + BY - 1) / BY);
     const bool check = c.target > 0;
+    if (check && !c.merge) return hipErrorInvalidValue;
     hipError_t err;
 
     hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles);
@@ -219,8 +220,8 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
         return err;
     if ((err = smooth(m, c, 2 * c.sweeps, false, s, c.rhs_ready)) != hipSuccess) return err;
     if (!check) {
-        if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, false,
-                                            nullptr, s)) != hipSuccess)
+        if (c.merge && (err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red,
+                                                       m.e_black, false, nullptr, s)) != hipSuccess)
             return err;
         hipLaunchKernelGGL(k_mixed_count, dim3(1), dim3(1), 0, s, m.cycles);
     } else {
@@ -244,7 +245,7 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
     }
     if ((err = hipGetLastError()) != hipSuccess) return err;
     // Final Neumann fill of the FP64 pressure (project.go:98).
-    return launch_pressure_bc(c.P, G, G.g > 1, s);
+    return c.merge ? launch_pressure_bc(c.P, G, G.g > 1, s) : hipSuccess;
 }
 
 }  // namespace af

@@ -51,33 +51,40 @@ def reference_layout_bytes(iters: int) -> int:
 def schedule_bytes(kind: str, iters: int = 24) -> int:
     """Algorithmic bytes per interior cell per Step of the schedule this build
     runs (one read of each input and one write of each output per kernel;
-    O(N^2) ghost fills not counted):
+    O(N^2) ghost fills not counted), for a single engine and for a batch alike:
 
     * advection: 6 sweeps x (read U, V, W + write Tmp0..2) = 288;
-    * fp64_split (batches), per projection: divergence 44 (U, V, W 24, Div 8,
-      split right-hand side 8, and the solve's first red pass, which depends
-      only on the right-hand side, 4), second pass 12, 2*iters-2 passes of
-      12, merge into P 16, gradient 56 = 104 + 24*iters;
-    * fp64_split_engine (one engine): the same, except that in the first
+    * fp64_split, per projection: divergence 44 (U, V, W 24, Div 8, split
+      right-hand side 8, and the solve's first red pass, which depends only
+      on the right-hand side, 4), second pass 12, 2*iters-2 passes of 12,
+      merge into P 16, gradient 56 = 104 + 24*iters; in the first
       projection Div is not stored (nothing reads it before the second
-      projection rewrites it: 8 fewer) and the gradient reads the split
-      solution and skips the merge (16 fewer);
+      projection rewrites it: 8 fewer), and a single engine's gradient
+      reads the split solution, so its merge is skipped (16 fewer; a batch
+      keeps the merge: fp64_split_batch);
     * fp64_arena (small grids): divergence 32, first pass 16, 2*iters-1
       passes of 24, gradient 56 = 80 + 48*iters;
     * mixed (one FP32 cycle), per projection: divergence 38 (FP32 rhs 4,
       first FP32 red pass 2), second pass 6, 2*iters-2 passes of 6, merge
-      12, gradient 56 = 100 + 12*iters;
-    * mixed_checked: the same with the first cycle closed by the fused merge
-      and FP64 residual (24 instead of 12); later cycles, which run only when
-      the check asks for them, are not counted (a lower bound);
+      12, gradient 56 = 100 + 12*iters; the first projection stores no Div
+      (8 fewer), and a single engine's also skips the merge (12 fewer) and
+      its gradient reads the FP32 correction instead of P (4 fewer; a batch
+      keeps both: mixed_batch);
+    * mixed_checked: the one-cycle schedule with the first cycle closed by
+      the fused merge and FP64 residual (24 instead of 12), in both
+      projections (the check reads Div and may add later cycles to P); later
+      cycles, which run only when the check asks for them, are not counted
+      (a lower bound);
     * direct (FFT passes): divergence folded into the first pass, five
       passes 80, gradient 56, per projection 32 + 80 + 56."""
-    if kind == "fp64_split_engine":
-        return schedule_bytes("fp64_split", iters) - 16 - 8
+    batch = kind.endswith("_batch")
+    base = kind[:-len("_batch")] if batch else kind
     proj = {"fp64_split": 104 + 24 * iters, "fp64_arena": 80 + 48 * iters,
             "mixed": 100 + 12 * iters, "mixed_checked": 112 + 12 * iters,
-            "direct": 32 + 80 + 56}[kind]
-    return 288 + 2 * proj
+            "direct": 32 + 80 + 56}[base]
+    first_saved = {"fp64_split": 8 + (0 if batch else 16),
+                   "mixed": 8 + (0 if batch else 12 + 4)}.get(base, 0)
+    return 288 + 2 * proj - first_saved
 
 
 def bandwidth(value: float, bytes_per_cell: int, kind: str) -> dict:
@@ -89,7 +96,7 @@ def bandwidth(value: float, bytes_per_cell: int, kind: str) -> dict:
 
 
 def fp64_kind(e) -> str:
-    return "fp64_split_engine" if e.solver_tile()[0] == 0 else "fp64_arena"
+    return "fp64_split" if e.solver_tile()[0] == 0 else "fp64_arena"
 
 
 def hip_sync() -> None:
@@ -286,6 +293,8 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
         t_div, t_grad = b.time_stage(2, 10), b.time_stage(3, 10)
         # The divergence writes the first red pass; the solve launches the rest.
         npass = 2 * (bc_extra.get("MixedSweeps", 24) if prec == af.PRECISION_MIXED else 24) - 1
+        bkind = ("fp64_split_batch" if prec == af.PRECISION_FP64 else
+                 ("mixed_checked" if bc_extra.get("MixedChecked") else "mixed_batch"))
         model = t_vbc + 2 * sum(sw) + 2 * (t_div + npass * t_pass + t_merge + t_pbc + t_grad)
         info = b.info()
         pass_bytes = (6 if prec == af.PRECISION_MIXED else 12) * cells
@@ -295,11 +304,7 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
             "ms_per_step": 1e3 * el / steps,
             "speedup_vs_plain_fp64": cells * steps / el / plain if plain else None,
             **bandwidth(cells * steps / el,
-                        schedule_bytes("fp64_split" if prec == af.PRECISION_FP64 else
-                                       ("mixed_checked" if bc_extra.get("MixedChecked") else
-                                        "mixed")),
-                        "fp64_split" if prec == af.PRECISION_FP64 else
-                        ("mixed_checked" if bc_extra.get("MixedChecked") else "mixed")),
+                        schedule_bytes(bkind), bkind),
             "pass_roofline": {"bound": "hbm", "achieved": pass_bytes / (t_pass * 1e-3) / 1e9,
                               "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                               "frac": pass_bytes / (t_pass * 1e-3) / 1e9 / HBM_PEAK_GBPS,
@@ -381,7 +386,7 @@ def big_leg(af, n: int, steps: int) -> dict:
         out[name] = {"value": n ** 3 * (steps - 1) / el, "unit": "cell-updates/s",
                      "ms_per_step": 1e3 * el / (steps - 1),
                      "final_max_divergence": e.MaxDivergence()}
-        kind = {"fp64": "fp64_split_engine", "mixed": "mixed"}.get(name, "mixed_checked")
+        kind = {"fp64": "fp64_split", "mixed": "mixed"}.get(name, "mixed_checked")
         out[name].update(bandwidth(out[name]["value"], schedule_bytes(kind), kind))
         if name == "mixed_to_target":
             out[name]["cycles_last_solve"] = e.mixed_stats()[1]
@@ -486,7 +491,7 @@ def serve_summary(jobs, res, st) -> dict:
         "peak_pool_bytes": st["PeakBytes"],
         "preemptions": st["Preemptions"],
         **bandwidth(st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0,
-                    schedule_bytes("mixed"), "mixed"),
+                    schedule_bytes("mixed_batch"), "mixed_batch"),
     }
 
 
