@@ -74,7 +74,8 @@ enum Kind {
     K_SWEEP,  // advection tiles (64 x rows x z-per-thread) of the union box
     K_CELL,   // 256-point plane tiles of CZ interior z-planes
     K_GRAD,   // 256-point tiles of one plane, over NZ + 1 planes (measured: more
-              // planes per item made the gradient slower)
+              // planes per item made the gradient slower, and four with every
+              // load in flight first, body::gradient_column, gained nothing)
     K_PASS,   // colour-pass tiles of the split layout (64 x 4 x pass_planes)
     K_PBC,    // pressure ghost-shell cells, CHUNK per item
     K_COL,    // 64 x 4 interior columns (z loop inside)
