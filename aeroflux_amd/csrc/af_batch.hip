@@ -75,7 +75,7 @@ enum Kind {
     K_CELL,   // 256-point plane tiles of CZ interior z-planes
     K_GRAD,   // 256-point tiles of one plane, over NZ + 1 planes (measured: more
               // planes per item made the gradient slower, and four with every
-              // load in flight first, body::gradient_column, gained nothing)
+              // load in flight before the first wait gained nothing)
     K_PASS,   // colour-pass tiles of the split layout (64 x 4 x pass_planes)
     K_PBC,    // pressure ghost-shell cells, CHUNK per item
     K_COL,    // 64 x 4 interior columns (z loop inside)

@@ -353,16 +353,25 @@ __device__ __forceinline__ double advect_value(const SweepArgs &a, const L &ld, 
 // that BC launch is skipped.  Without a.bc it is the plain store of
 // advectDiffuseAxis.
 template <int C>
-__device__ __forceinline__ void store_face_bc(const Geom &G, const DField &T, int x, int y, int z,
-                                              double v)
+__device__ __forceinline__ double face_bc_value(const Geom &G, int x, int y, int z, double v)
+{
+    const int g = G.g;
+    const int c[3] = {x, y, z};
+    const int N[3] = {G.NX, G.NY, G.NZ};
+    return (c[C] == g || c[C] == g + N[C]) ? 0.0 : v;
+}
+
+// The ghost cells whose base is face (x, y, z) of component C, given the
+// value v the face stores (see store_face_bc).
+template <int C>
+__device__ __forceinline__ void face_bc_ghosts(const Geom &G, const DField &T, int x, int y, int z,
+                                               double v)
 {
     const auto t = gp(T.d);
     const int i = idx(T, x, y, z);
     const int g = G.g;
     const int c[3] = {x, y, z};
     const int N[3] = {G.NX, G.NY, G.NZ};
-    if (c[C] == g || c[C] == g + N[C]) v = 0.0;
-    t[i] = v;
     constexpr int A = C == 0 ? 1 : 0;   // the two tangential axes
     constexpr int Bt = C == 2 ? 1 : 2;
     const int sa = A == 0 ? 1 : T.sy;
@@ -379,6 +388,15 @@ __device__ __forceinline__ void store_face_bc(const Geom &G, const DField &T, in
     if (alo && bhi) t[i - sa + sb] = v;
     if (ahi && blo) t[i + sa - sb] = v;
     if (ahi && bhi) t[i + sa + sb] = v;
+}
+
+template <int C>
+__device__ __forceinline__ void store_face_bc(const Geom &G, const DField &T, int x, int y, int z,
+                                              double v)
+{
+    v = face_bc_value<C>(G, x, y, z, v);
+    gp(T.d)[idx(T, x, y, z)] = v;
+    face_bc_ghosts<C>(G, T, x, y, z, v);
 }
 
 template <int C>
@@ -579,139 +597,219 @@ __device__ __forceinline__ void gradient_cell(const DField &U, const DField &V, 
     put_grad<BC, 2>(G, W, x, y, z, w[iw] - scale * (p[ip] - p[ip - P.sz]));
 }
 
-// gradient_cell for the ZG positions z0 .. z0+ZG-1 of column (x, y), with
-// the same values.  Every position's loads are issued before the first
-// wait and every result is held in a register before the first store, so
-// ZG positions' loads are in flight together.  A position past the last
+// Faces (x, y, z) and (x+1, y, z) of component C of T: with BC, the wall
+// value and the ghosts each face is the base of, as store_face_bc; the two
+// values in one 16-byte store, or the first alone when `two` is false.
+template <bool BC, int C>
+__device__ __forceinline__ void put_pair(const Geom &G, const DField &T, int x, int y, int z,
+                                         double v0, double v1, bool two)
+{
+    if (BC) {
+        v0 = face_bc_value<C>(G, x, y, z, v0);
+        v1 = face_bc_value<C>(G, x + 1, y, z, v1);
+    }
+    double *d = T.d + idx(T, x, y, z);
+    if (two)
+        st2(d, v0, v1);
+    else
+        gp(T.d)[(int)(d - T.d)] = v0;
+    if (BC) {
+        face_bc_ghosts<C>(G, T, x, y, z, v0);
+        if (two) face_bc_ghosts<C>(G, T, x + 1, y, z, v1);
+    }
+}
+
+// gradient_cell for the cell pair x = 2k, 2k+1 of row yi (interior
+// coordinates) at positions z0 .. z0+ZG-1 (padded; g+NZ is W's wall face),
+// with the same values: P, U, V and W read as x-adjacent 16-byte pairs (P's
+// x-neighbour of cell 2k as one more value), the faces written by put_pair.
+// Every position's loads are issued before the first wait and every result
+// is held in a register before the first store.  A position past the last
 // (z > g+NZ) loads from the last and stores nothing.  The wall faces of the
 // last column and row are updated afterwards, position by position.
 template <bool BC, int ZG>
-__device__ __forceinline__ void gradient_column(const DField &U, const DField &V, const DField &W,
-                                                const DField &P, const Geom &G, double scale,
-                                                int x, int y, int z0)
+__device__ __forceinline__ void gradient_pair(const DField &U, const DField &V, const DField &W,
+                                              const DField &P, const Geom &G, double scale, int k,
+                                              int yi, int z0)
 {
-    const int g = G.g;
-    if (x >= g + G.NX || y >= g + G.NY) return;
-    const auto p = gp((const double *)P.d);
-    const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
+    const int g = G.g, NX = G.NX, NY = G.NY, NXH = (NX + 1) / 2;
+    if (k >= NXH || yi >= NY) return;
+    const int x0 = 2 * k;
+    const bool two = x0 + 1 < NX;
+    const int x = g + x0, y = g + yi;
     const int zl = g + G.NZ;  // last position (W's wall face)
-    double pc[ZG], px[ZG], py[ZG], pz[ZG], uu[ZG], vv[ZG], ww[ZG];
+    const auto p = gp((const double *)P.d);
+    double2 pc[ZG], py[ZG], pz[ZG], uu[ZG], vv[ZG], ww[ZG];
+    double px[ZG];
 #pragma unroll
     for (int r = 0; r < ZG; r++) {
         const int z = z0 + r < zl ? z0 + r : zl;
         const int ip = idx(P, x, y, z);
-        pc[r] = p[ip];
+        // x0 + 1 + g is inside every padded row, and z = g+NZ is a ghost
+        // plane of P, U and V: every pair is in range.
+        pc[r] = ld2(P.d + ip);
         px[r] = p[ip - 1];
-        py[r] = p[ip - P.sy];
-        pz[r] = p[ip - P.sz];
-        uu[r] = u[idx(U, x, y, z)];  // z = g+NZ is a ghost plane of U and V: in range
-        vv[r] = v[idx(V, x, y, z)];
-        ww[r] = w[idx(W, x, y, z)];
+        py[r] = ld2(P.d + ip - P.sy);
+        pz[r] = ld2(P.d + ip - P.sz);
+        uu[r] = ld2(U.d + idx(U, x, y, z));
+        vv[r] = ld2(V.d + idx(V, x, y, z));
+        ww[r] = ld2(W.d + idx(W, x, y, z));
     }
-    double ou[ZG], ov[ZG], ow[ZG];
+    double ou0[ZG], ou1[ZG], ov0[ZG], ov1[ZG], ow0[ZG], ow1[ZG];
 #pragma unroll
     for (int r = 0; r < ZG; r++) {
-        ou[r] = uu[r] - scale * (pc[r] - px[r]);
-        ov[r] = vv[r] - scale * (pc[r] - py[r]);
-        ow[r] = ww[r] - scale * (pc[r] - pz[r]);
-        asm volatile("" : "+v"(ou[r]), "+v"(ov[r]), "+v"(ow[r]));
+        ou0[r] = uu[r].x - scale * (pc[r].x - px[r]);
+        ou1[r] = uu[r].y - scale * (pc[r].y - pc[r].x);
+        ov0[r] = vv[r].x - scale * (pc[r].x - py[r].x);
+        ov1[r] = vv[r].y - scale * (pc[r].y - py[r].y);
+        ow0[r] = ww[r].x - scale * (pc[r].x - pz[r].x);
+        ow1[r] = ww[r].y - scale * (pc[r].y - pz[r].y);
+        asm volatile("" : "+v"(ou0[r]), "+v"(ou1[r]), "+v"(ov0[r]), "+v"(ov1[r]), "+v"(ow0[r]),
+                     "+v"(ow1[r]));
     }
 #pragma unroll
     for (int r = 0; r < ZG; r++) {
         const int z = z0 + r;
         if (z < zl) {
-            put_grad<BC, 0>(G, U, x, y, z, ou[r]);
-            put_grad<BC, 1>(G, V, x, y, z, ov[r]);
+            put_pair<BC, 0>(G, U, x, y, z, ou0[r], ou1[r], two);
+            put_pair<BC, 1>(G, V, x, y, z, ov0[r], ov1[r], two);
         }
-        if (z <= zl) put_grad<BC, 2>(G, W, x, y, z, ow[r]);
+        if (z <= zl) put_pair<BC, 2>(G, W, x, y, z, ow0[r], ow1[r], two);
     }
-    if (x == g + G.NX - 1 || y == g + G.NY - 1) {
+    // The wall faces of the last column (U at x = g+NX) and row (V at
+    // y = g+NY), as gradient_cell forms them.
+    const int xl = two ? x + 1 : x;
+    const bool lastx = xl == g + NX - 1, lasty = y == g + NY - 1;
+    if (lastx || lasty) {
+        const auto u = gp(U.d), v = gp(V.d);
         for (int r = 0; r < ZG; r++) {
             const int z = z0 + r;
             if (z >= zl) break;
-            const int ip = idx(P, x, y, z);
-            if (x == g + G.NX - 1)
-                put_grad<BC, 0>(G, U, x + 1, y, z,
-                                u[idx(U, x, y, z) + 1] - scale * (p[ip + 1] - p[ip]));
-            if (y == g + G.NY - 1)
-                put_grad<BC, 1>(G, V, x, y + 1, z,
-                                v[idx(V, x, y, z) + V.sy] - scale * (p[ip + P.sy] - p[ip]));
+            if (lastx) {
+                const int ip = idx(P, xl, y, z);
+                put_grad<BC, 0>(G, U, xl + 1, y, z,
+                                u[idx(U, xl, y, z) + 1] - scale * (p[ip + 1] - p[ip]));
+            }
+            if (lasty) {
+                for (int c = 0; c < (two ? 2 : 1); c++) {
+                    const int ip = idx(P, x + c, y, z);
+                    put_grad<BC, 1>(G, V, x + c, y + 1, z,
+                                    v[idx(V, x + c, y, z) + V.sy] - scale * (p[ip + P.sy] - p[ip]));
+                }
+            }
         }
     }
 }
 
-// gradient_column with P read from the colour-split solution (red, black:
-// interior cells, NXH = ceil(NX/2) entries per row) instead of the natural
-// layout: P's interior cells are the split entries, and the only ghosts the
-// gradient reads are first-layer face ghosts, which applyPressureBC
-// (project.go:139-163) fills with the interior cell beside them -- here the
-// split entry at the clamped coordinate.  So the values, and the bits, are
-// those of merging the solution into P, applying the pressure BC and then
-// gradient_column.  TS = float reads the mixed solve's FP32 correction,
-// whose widening is exact: the values merging it into P would give.
-template <bool BC, int ZG, typename TS = double>
-__device__ __forceinline__ void gradient_column_split(const DField &U, const DField &V,
-                                                      const DField &W, const TS *red,
-                                                      const TS *black, const Geom &G,
-                                                      double scale, int x, int y, int z0)
+// gradient_pair<true, ZG> with P read from the colour-split solution (red,
+// black: interior cells, NXH = ceil(NX/2) entries per row) instead of the
+// natural layout: P's interior cells are the split entries, and the only
+// ghosts the gradient reads are first-layer face ghosts, which
+// applyPressureBC (project.go:139-163) fills with the interior cell beside
+// them -- here the split entry at the clamped coordinate.  So the values,
+// and the bits, are those of merging the solution into P, applying the
+// pressure BC and then gradient_pair.  TS = float reads the mixed solve's
+// FP32 correction, whose widening is exact: the values merging it into P
+// would give.
+//
+// A thread owns the cell pair x = 2k, 2k+1 of row yi (interior
+// coordinates) at positions z0 .. z0+ZG-1 (padded; g+NZ is W's wall face).  Cell 2k has the row's colour and
+// 2k+1 the other, so each P value the pair needs is entry k (or k-1 for
+// cell 2k-1) of a colour array chosen per row and plane, uniform across a
+// wave: the pair's own values, the row below and the plane below (the
+// clamped coordinate where that is outside the interior, as applyPressureBC
+// fills the ghost).  U, V and W are read and written as 16-byte pairs; a
+// wall face's +0 and the ghosts whose base is a face are written as
+// store_face_bc does.
+template <int ZG, typename TS>
+__device__ __forceinline__ void gradient_pair_split(const DField &U, const DField &V,
+                                                    const DField &W, const TS *red,
+                                                    const TS *black, const Geom &G, double scale,
+                                                    int k, int yi, int z0)
 {
-    const int g = G.g;
-    if (x >= g + G.NX || y >= g + G.NY) return;
-    const int NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2;
-    const auto sr = gp(red), sb = gp(black);
-    // P at interior coordinates (xi, yi, zi), each clamped into the interior.
-    auto P = [&](int xi, int yi, int zi) -> double {
-        xi = xi < 0 ? 0 : (xi >= NX ? NX - 1 : xi);
-        yi = yi < 0 ? 0 : (yi >= NY ? NY - 1 : yi);
-        zi = zi < 0 ? 0 : (zi >= NZ ? NZ - 1 : zi);
-        const int i = (zi * NY + yi) * NXH + (xi >> 1);
-        return (double)(((xi + yi + zi + 3 * g) & 1) ? sb[i] : sr[i]);  // parity, red = 0
-    };
-    const auto u = gp(U.d), v = gp(V.d), w = gp(W.d);
+    const int g = G.g, NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2;
+    if (k >= NXH || yi >= NY) return;
+    const int x0 = 2 * k;
+    const bool two = x0 + 1 < NX;
     const int zl = g + NZ;  // last position (W's wall face)
-    const int xi = x - g, yi = y - g;
-    double pc[ZG], px[ZG], py[ZG], pz[ZG], uu[ZG], vv[ZG], ww[ZG];
+    const int km = k > 0 ? k - 1 : 0;
+    const int ym = yi > 0 ? yi - 1 : 0;
+    const auto sr = gp(red), sb = gp(black);
+    // Entry kk of row yy, plane zz, in the array of cell 2kk's colour there
+    // (second false) or of cell 2kk+1's (second true).
+    const auto at = [&](int zz, int yy, int kk, bool second) -> double {
+        const bool b = ((yy + zz + 3 * g) & 1) ^ (int)second;
+        const int i = (zz * NY + yy) * NXH + kk;
+        return (double)(b ? sb[i] : sr[i]);
+    };
+    double pc0[ZG], pc1[ZG], px0[ZG], py0[ZG], py1[ZG], pz0[ZG], pz1[ZG];
+    double2 uu[ZG], vv[ZG], ww[ZG];
 #pragma unroll
     for (int r = 0; r < ZG; r++) {
         const int z = z0 + r < zl ? z0 + r : zl;
         const int zi = z - g;
-        pc[r] = P(xi, yi, zi);
-        px[r] = P(xi - 1, yi, zi);
-        py[r] = P(xi, yi - 1, zi);
-        pz[r] = P(xi, yi, zi - 1);
-        uu[r] = u[idx(U, x, y, z)];  // z = g+NZ is a ghost plane of U and V: in range
-        vv[r] = v[idx(V, x, y, z)];
-        ww[r] = w[idx(W, x, y, z)];
+        const int zc = zi < NZ ? zi : NZ - 1;                    // P's plane, clamped
+        const int zm = zi - 1 < 0 ? 0 : (zi - 1 < NZ ? zi - 1 : NZ - 1);
+        pc0[r] = at(zc, yi, k, false);
+        pc1[r] = at(zc, yi, k, true);
+        // Cell 2k-1 is the second cell of entry k-1; at k == 0 the clamped
+        // neighbour is cell 0 itself.
+        px0[r] = k > 0 ? at(zc, yi, km, true) : pc0[r];
+        // The row and plane below (their colours follow from their parity);
+        // a clamped one is the pair's own row or plane.
+        py0[r] = yi > 0 ? at(zc, ym, k, false) : pc0[r];
+        py1[r] = yi > 0 ? at(zc, ym, k, true) : pc1[r];
+        pz0[r] = zm != zc ? at(zm, yi, k, false) : pc0[r];
+        pz1[r] = zm != zc ? at(zm, yi, k, true) : pc1[r];
+        // z = g+NZ is a ghost plane of U and V, and x0 + 1 + g is inside
+        // every padded row: in range.
+        uu[r] = ld2(U.d + idx(U, g + x0, g + yi, z));
+        vv[r] = ld2(V.d + idx(V, g + x0, g + yi, z));
+        ww[r] = ld2(W.d + idx(W, g + x0, g + yi, z));
     }
-    double ou[ZG], ov[ZG], ow[ZG];
+    double ou0[ZG], ou1[ZG], ov0[ZG], ov1[ZG], ow0[ZG], ow1[ZG];
 #pragma unroll
     for (int r = 0; r < ZG; r++) {
-        ou[r] = uu[r] - scale * (pc[r] - px[r]);
-        ov[r] = vv[r] - scale * (pc[r] - py[r]);
-        ow[r] = ww[r] - scale * (pc[r] - pz[r]);
-        asm volatile("" : "+v"(ou[r]), "+v"(ov[r]), "+v"(ow[r]));
+        ou0[r] = uu[r].x - scale * (pc0[r] - px0[r]);
+        ou1[r] = uu[r].y - scale * (pc1[r] - pc0[r]);
+        ov0[r] = vv[r].x - scale * (pc0[r] - py0[r]);
+        ov1[r] = vv[r].y - scale * (pc1[r] - py1[r]);
+        ow0[r] = ww[r].x - scale * (pc0[r] - pz0[r]);
+        ow1[r] = ww[r].y - scale * (pc1[r] - pz1[r]);
+        asm volatile("" : "+v"(ou0[r]), "+v"(ou1[r]), "+v"(ov0[r]), "+v"(ov1[r]), "+v"(ow0[r]),
+                     "+v"(ow1[r]));
     }
+    const int x = g + x0, y = g + yi;
 #pragma unroll
     for (int r = 0; r < ZG; r++) {
         const int z = z0 + r;
         if (z < zl) {
-            put_grad<BC, 0>(G, U, x, y, z, ou[r]);
-            put_grad<BC, 1>(G, V, x, y, z, ov[r]);
+            put_pair<true, 0>(G, U, x, y, z, ou0[r], ou1[r], two);
+            put_pair<true, 1>(G, V, x, y, z, ov0[r], ov1[r], two);
         }
-        if (z <= zl) put_grad<BC, 2>(G, W, x, y, z, ow[r]);
+        if (z <= zl) put_pair<true, 2>(G, W, x, y, z, ow0[r], ow1[r], two);
     }
-    if (x == g + NX - 1 || y == g + NY - 1) {
+    // The wall faces of the last column (U at x = g+NX) and row (V at
+    // y = g+NY), from the last cell of the pair.
+    const int xl = two ? x + 1 : x;
+    const bool lastx = xl == g + NX - 1, lasty = y == g + NY - 1;
+    if (lastx || lasty) {
+        const auto u = gp(U.d), v = gp(V.d);
         for (int r = 0; r < ZG; r++) {
             const int z = z0 + r;
             if (z >= zl) break;
-            const int zi = z - g;
-            const double c = P(xi, yi, zi);
-            if (x == g + NX - 1)
-                put_grad<BC, 0>(G, U, x + 1, y, z,
-                                u[idx(U, x, y, z) + 1] - scale * (P(xi + 1, yi, zi) - c));
-            if (y == g + NY - 1)
-                put_grad<BC, 1>(G, V, x, y + 1, z,
-                                v[idx(V, x, y, z) + V.sy] - scale * (P(xi, yi + 1, zi) - c));
+            if (lastx) {
+                const double c = two ? pc1[r] : pc0[r];
+                // P beyond the last column is the clamped cell itself.
+                store_face_bc<0>(G, U, xl + 1, y, z, u[idx(U, xl, y, z) + 1] - scale * (c - c));
+            }
+            if (lasty) {
+                store_face_bc<1>(G, V, x, y + 1, z,
+                                 v[idx(V, x, y, z) + V.sy] - scale * (pc0[r] - pc0[r]));
+                if (two)
+                    store_face_bc<1>(G, V, x + 1, y + 1, z,
+                                     v[idx(V, x + 1, y, z) + V.sy] - scale * (pc1[r] - pc1[r]));
+            }
         }
     }
 }

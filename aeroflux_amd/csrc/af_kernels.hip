@@ -173,9 +173,9 @@ __global__ __launch_bounds__(256) void k_zero(double *d, int64_t n)
     if (i < n) d[i] = 0;
 }
 
-// subtractGradient, project.go:172-205, one thread per interior column and
-// GZ consecutive z positions (BC: with the velocity BC that follows it
-// applied, body::gradient_column).
+// subtractGradient, project.go:172-205, one thread per interior cell pair
+// (x = 2k, 2k+1) and GZ consecutive z positions (BC: with the velocity BC
+// that follows it applied, body::gradient_pair).
 constexpr int GZ = 4;
 
 template <bool BC>
@@ -184,12 +184,12 @@ __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField 
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int g = G.g;
-    body::gradient_column<BC, GZ>(U, V, W, P, G, scale, g + bx * BX + threadIdx.x,
-                                  g + by * BY + threadIdx.y, g + bz * GZ);
+    body::gradient_pair<BC, GZ>(U, V, W, P, G, scale, bx * BX + threadIdx.x,
+                                by * BY + threadIdx.y, G.g + bz * GZ);
 }
 
-// The same with P read from the colour-split solution (body::gradient_column_split).
+// The same with P read from the colour-split solution, a cell pair per
+// thread (body::gradient_pair_split).
 template <typename TS>
 __global__ __launch_bounds__(BX *BY) void k_gradient_split(DField U, DField V, DField W,
                                                            const TS *red, const TS *black, Geom G,
@@ -197,10 +197,8 @@ __global__ __launch_bounds__(BX *BY) void k_gradient_split(DField U, DField V, D
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    const int g = G.g;
-    body::gradient_column_split<true, GZ, TS>(U, V, W, red, black, G, scale,
-                                              g + bx * BX + threadIdx.x,
-                                              g + by * BY + threadIdx.y, g + bz * GZ);
+    body::gradient_pair_split<GZ, TS>(U, V, W, red, black, G, scale, bx * BX + threadIdx.x,
+                                      by * BY + threadIdx.y, G.g + bz * GZ);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -380,7 +378,8 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s)
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
                            hipStream_t s, bool velocity_bc)
 {
-    const int3 nb = nblocks(G.NX, G.NY, (G.NZ + 1 + GZ - 1) / GZ);  // z = g .. g+NZ
+    // A thread per cell pair (NXH entries per row); z = g .. g+NZ.
+    const int3 nb = nblocks((G.NX + 1) / 2, G.NY, (G.NZ + 1 + GZ - 1) / GZ);
     if (velocity_bc)
         hipLaunchKernelGGL(k_gradient<true>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
                            nb);
@@ -394,7 +393,8 @@ template <typename TS>
 hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
                                  Geom G, double scale, hipStream_t s)
 {
-    const int3 nb = nblocks(G.NX, G.NY, (G.NZ + 1 + GZ - 1) / GZ);  // z = g .. g+NZ
+    // A thread per cell pair (NXH entries per row); z = g .. g+NZ.
+    const int3 nb = nblocks((G.NX + 1) / 2, G.NY, (G.NZ + 1 + GZ - 1) / GZ);
     hipLaunchKernelGGL(k_gradient_split<TS>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, red, black,
                        G, scale, nb);
     return hipGetLastError();
