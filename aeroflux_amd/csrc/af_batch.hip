@@ -74,8 +74,10 @@ enum Kind {
     K_SWEEP,  // advection tiles (64 x rows x z-per-thread) of the union box
     K_CELL,   // 256-point plane tiles of CZ interior z-planes
     K_GRAD,   // 256-point tiles of one plane, over NZ + 1 planes (measured: more
-              // planes per item made the gradient slower, and four with every
-              // load in flight before the first wait gained nothing)
+              // planes per item made the gradient slower; four with every load
+              // in flight before the first wait gained nothing, and a cell
+              // pair per thread over 4 planes, body::gradient_pair, was 4-8%
+              // slower on the 96 engines)
     K_PASS,   // colour-pass tiles of the split layout (64 x 4 x pass_planes)
     K_PBC,    // pressure ghost-shell cells, CHUNK per item
     K_COL,    // 64 x 4 interior columns (z loop inside)
