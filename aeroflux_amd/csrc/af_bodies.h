@@ -713,7 +713,9 @@ __device__ __forceinline__ void gradient_pair(const DField &U, const DField &V, 
 // would give.
 //
 // A thread owns the cell pair x = 2k, 2k+1 of row yi (interior
-// coordinates) at positions z0 .. z0+ZG-1 (padded; g+NZ is W's wall face).  Cell 2k has the row's colour and
+// coordinates) at positions z0 .. z0+ZG-1 (padded; g+NZ is W's wall face).
+// With Pout it also writes the pair's P values to Pout's interior (the
+// merge; Pout's ghosts are left to applyPressureBC).  Cell 2k has the row's colour and
 // 2k+1 the other, so each P value the pair needs is entry k (or k-1 for
 // cell 2k-1) of a colour array chosen per row and plane, uniform across a
 // wave: the pair's own values, the row below and the plane below (the
@@ -725,7 +727,8 @@ template <int ZG, typename TS>
 __device__ __forceinline__ void gradient_pair_split(const DField &U, const DField &V,
                                                     const DField &W, const TS *red,
                                                     const TS *black, const Geom &G, double scale,
-                                                    int k, int yi, int z0)
+                                                    int k, int yi, int z0,
+                                                    const DField *Pout = nullptr)
 {
     const int g = G.g, NX = G.NX, NY = G.NY, NZ = G.NZ, NXH = (NX + 1) / 2;
     if (k >= NXH || yi >= NY) return;
@@ -786,6 +789,9 @@ __device__ __forceinline__ void gradient_pair_split(const DField &U, const DFiel
         if (z < zl) {
             put_pair<true, 0>(G, U, x, y, z, ou0[r], ou1[r], two);
             put_pair<true, 1>(G, V, x, y, z, ov0[r], ov1[r], two);
+            // The merge of the solution into P's interior (the values
+            // from_split_pair stores), for a caller that keeps P.
+            if (Pout) put_pair<false, 0>(G, *Pout, x, y, z, pc0[r], pc1[r], two);
         }
         if (z <= zl) put_pair<true, 2>(G, W, x, y, z, ow0[r], ow1[r], two);
     }

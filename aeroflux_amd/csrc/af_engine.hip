@@ -376,48 +376,54 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false, bool m
 }
 
 // project, project.go:12-34.  The gradient subtraction applies the closing
-// velocity BC itself (body::gradient_cell), which is the same values as the
+// velocity BC itself (body::gradient_pair), which is the same values as the
 // separate launch.  bc_applied: the velocity already holds the opening BC
-// (the sweep before it applied the BC to its output).  p_dead: nothing reads
-// P or Div after this projection's gradient before the next projection
-// overwrites them (the step's first projection), so on the FP64
-// colour-split path and on the one-cycle mixed path Div is not stored and
-// the solution is not merged into P: the gradient reads it from the split
+// (the sweep before it applied the BC to its output).
+//
+// On the FP64 colour-split path and on the one-cycle mixed path the solution
+// is not merged into P by the solve: the gradient reads it from the split
 // arrays (launch_gradient_split; the FP64 solution, or the FP32 correction,
-// which is the whole of the one-cycle solution: the same values).
+// which is the whole of the one-cycle solution: the same values) and writes
+// P's interior as it goes, and P's ghost fill follows.  p_dead: nothing
+// reads P or Div after this projection's gradient before the next
+// projection overwrites them (the step's first projection), so Div is not
+// stored and P not written at all.
 hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, bool p_dead = false)
 {
     hipError_t err = bc_applied ? hipSuccess : enq_velocity_bc(e, d);
     const bool mixed = e->cfg.Precision == AF_PRECISION_MIXED;
-    const bool split_tail = p_dead && !e->direct.ready() &&
-                            (mixed ? e->cfg.MixedTarget <= 0 : e->tile64.k == 0);
+    const bool split_path =
+        !e->direct.ready() && (mixed ? e->cfg.MixedTarget <= 0 : e->tile64.k == 0);
+    const DField P = dfield(e, d.f[AF_FIELD_P]);
     if (e->direct.method == DIRECT_FFT) {
         // The direct solve's first pass computes the divergence itself (the
         // same values, stored to Div), which saves re-reading Div.
         if (err == hipSuccess)
             err = enqueue_direct_project_solve(
-                e->direct, dfield(e, d.f[AF_FIELD_P]), dfield(e, d.f[AF_FIELD_DIV]),
-                dfield(e, d.f[AF_FIELD_U]), dfield(e, d.f[AF_FIELD_V]),
-                dfield(e, d.f[AF_FIELD_W]), e->div_scale, e->G, e->stream);
+                e->direct, P, dfield(e, d.f[AF_FIELD_DIV]), dfield(e, d.f[AF_FIELD_U]),
+                dfield(e, d.f[AF_FIELD_V]), dfield(e, d.f[AF_FIELD_W]), e->div_scale, e->G,
+                e->stream);
     } else {
         const bool fuse = fused_rhs(e);
-        if (err == hipSuccess) err = enq_divergence(e, d, fuse, split_tail);
-        if (err == hipSuccess && split_tail && !mixed)
-            err = enq_solve_fp64_split(e, dfield(e, d.f[AF_FIELD_P]),
-                                       dfield(e, d.f[AF_FIELD_DIV]), fuse, false);
+        if (err == hipSuccess) err = enq_divergence(e, d, fuse, split_path && p_dead);
+        if (err == hipSuccess && split_path && !mixed)
+            err = enq_solve_fp64_split(e, P, dfield(e, d.f[AF_FIELD_DIV]), fuse, false);
         else if (err == hipSuccess)
-            err = enq_solve(e, d, fuse, !split_tail);
+            err = enq_solve(e, d, fuse, !split_path);
     }
-    if (err == hipSuccess && split_tail && mixed)
-        return launch_gradient_split<float>(dfield(e, d.f[AF_FIELD_U]),
-                                            dfield(e, d.f[AF_FIELD_V]),
-                                            dfield(e, d.f[AF_FIELD_W]), e->mixed.e_red,
-                                            e->mixed.e_black, e->G, e->grad_scale, e->stream);
-    if (err == hipSuccess && split_tail)
-        return launch_gradient_split<double>(dfield(e, d.f[AF_FIELD_U]),
-                                             dfield(e, d.f[AF_FIELD_V]),
-                                             dfield(e, d.f[AF_FIELD_W]), e->ps_red, e->ps_black,
-                                             e->G, e->grad_scale, e->stream);
+    if (err == hipSuccess && split_path) {
+        const DField U = dfield(e, d.f[AF_FIELD_U]), V = dfield(e, d.f[AF_FIELD_V]),
+                     W = dfield(e, d.f[AF_FIELD_W]);
+        const DField *Pout = p_dead ? nullptr : &P;
+        err = mixed ? launch_gradient_split<float>(U, V, W, e->mixed.e_red, e->mixed.e_black, e->G,
+                                                   e->grad_scale, e->stream, Pout)
+                    : launch_gradient_split<double>(U, V, W, e->ps_red, e->ps_black, e->G,
+                                                    e->grad_scale, e->stream, Pout);
+        // The solve's final applyPressureBC (project.go:98), on the P the
+        // gradient wrote.
+        if (err == hipSuccess && !p_dead) err = launch_pressure_bc(P, e->G, e->G.g > 1, e->stream);
+        return err;
+    }
     if (err == hipSuccess) err = enq_gradient(e, d, true);
     return err;
 }

@@ -72,10 +72,12 @@ hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, doubl
 // subtractGradient with the velocity BC, P read from the colour-split
 // solution (red, black: the FP64 solution, or the mixed solve's FP32
 // correction after its one cycle) instead of P: the values of merging it
-// into P and applying the pressure BC first.
+// into P and applying the pressure BC first.  With P, the kernel also
+// writes the solution to P's interior (the merge; P's ghost fill is the
+// caller's).
 template <typename TS>
 hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
-                                 Geom G, double scale, hipStream_t s);
+                                 Geom G, double scale, hipStream_t s, const DField *P = nullptr);
 hipError_t launch_set_int(int *p, int v, hipStream_t s);
 
 // Reductions: two launches, fixed tree, no atomics.  `partials` must hold

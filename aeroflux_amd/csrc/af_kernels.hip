@@ -190,15 +190,16 @@ __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField 
 
 // The same with P read from the colour-split solution, a cell pair per
 // thread (body::gradient_pair_split).
-template <typename TS>
+template <typename TS, bool MERGE>
 __global__ __launch_bounds__(BX *BY) void k_gradient_split(DField U, DField V, DField W,
                                                            const TS *red, const TS *black, Geom G,
-                                                           double scale, int3 nb)
+                                                           double scale, int3 nb, DField P)
 {
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     body::gradient_pair_split<GZ, TS>(U, V, W, red, black, G, scale, bx * BX + threadIdx.x,
-                                      by * BY + threadIdx.y, G.g + bz * GZ);
+                                      by * BY + threadIdx.y, G.g + bz * GZ,
+                                      MERGE ? &P : nullptr);
 }
 
 __global__ void k_set_int(int *p, int v) { *p = v; }
@@ -391,18 +392,24 @@ hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, doubl
 
 template <typename TS>
 hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
-                                 Geom G, double scale, hipStream_t s)
+                                 Geom G, double scale, hipStream_t s, const DField *P)
 {
     // A thread per cell pair (NXH entries per row); z = g .. g+NZ.
     const int3 nb = nblocks((G.NX + 1) / 2, G.NY, (G.NZ + 1 + GZ - 1) / GZ);
-    hipLaunchKernelGGL(k_gradient_split<TS>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, red, black,
-                       G, scale, nb);
+    if (P)
+        hipLaunchKernelGGL((k_gradient_split<TS, true>), grid1(nb), dim3(BX, BY), 0, s, U, V, W,
+                           red, black, G, scale, nb, *P);
+    else
+        hipLaunchKernelGGL((k_gradient_split<TS, false>), grid1(nb), dim3(BX, BY), 0, s, U, V, W,
+                           red, black, G, scale, nb, DField{});
     return hipGetLastError();
 }
 template hipError_t launch_gradient_split<double>(DField, DField, DField, const double *,
-                                                  const double *, Geom, double, hipStream_t);
+                                                  const double *, Geom, double, hipStream_t,
+                                                  const DField *);
 template hipError_t launch_gradient_split<float>(DField, DField, DField, const float *,
-                                                 const float *, Geom, double, hipStream_t);
+                                                 const float *, Geom, double, hipStream_t,
+                                                 const DField *);
 
 hipError_t launch_set_int(int *p, int v, hipStream_t s)
 {

@@ -59,17 +59,20 @@ def schedule_bytes(kind: str, iters: int = 24) -> int:
       on the right-hand side, 4), second pass 12, 2*iters-2 passes of 12,
       merge into P 16, gradient 56 = 104 + 24*iters; in the first
       projection Div is not stored (nothing reads it before the second
-      projection rewrites it: 8 fewer), and a single engine's gradient
-      reads the split solution, so its merge is skipped (16 fewer; a batch
-      keeps the merge: fp64_split_batch);
+      projection rewrites it: 8 fewer); a single engine's gradients read the
+      split solution, so the first projection skips the merge (16 fewer)
+      and the second writes P from the gradient (8 fewer: the merge's read
+      of the split solution); a batch keeps the merges (fp64_split_batch);
     * fp64_arena (small grids): divergence 32, first pass 16, 2*iters-1
       passes of 24, gradient 56 = 80 + 48*iters;
     * mixed (one FP32 cycle), per projection: divergence 38 (FP32 rhs 4,
       first FP32 red pass 2), second pass 6, 2*iters-2 passes of 6, merge
       12, gradient 56 = 100 + 12*iters; the first projection stores no Div
-      (8 fewer), and a single engine's also skips the merge (12 fewer) and
-      its gradient reads the FP32 correction instead of P (4 fewer; a batch
-      keeps both: mixed_batch);
+      (8 fewer); a single engine's gradients read the FP32 correction
+      instead of P (4 fewer each), the first projection skips the merge (12
+      fewer) and the second writes P from the gradient instead (4 fewer: the
+      merge's read of the correction); a batch keeps the merges
+      (mixed_batch);
     * mixed_checked: the one-cycle schedule with the first cycle closed by
       the fused merge and FP64 residual (24 instead of 12), in both
       projections (the check reads Div and may add later cycles to P); later
@@ -82,9 +85,9 @@ def schedule_bytes(kind: str, iters: int = 24) -> int:
     proj = {"fp64_split": 104 + 24 * iters, "fp64_arena": 80 + 48 * iters,
             "mixed": 100 + 12 * iters, "mixed_checked": 112 + 12 * iters,
             "direct": 32 + 80 + 56}[base]
-    first_saved = {"fp64_split": 8 + (0 if batch else 16),
-                   "mixed": 8 + (0 if batch else 12 + 4)}.get(base, 0)
-    return 288 + 2 * proj - first_saved
+    saved = {"fp64_split": 8 + (0 if batch else 16 + 8),
+             "mixed": 8 + (0 if batch else (12 + 4) + (4 + 4))}.get(base, 0)
+    return 288 + 2 * proj - saved
 
 
 def bandwidth(value: float, bytes_per_cell: int, kind: str) -> dict:
