@@ -514,16 +514,20 @@ __global__ __launch_bounds__(BX *BY) void kb_from_split(BCtl c, int gated)
 // the FP32 correction and the FP64 residual of that P goes to the split
 // right-hand side, with one max|r| partial per item.  The same bits as
 // kb_from_split<float, false> followed by kb_resid.
-__global__ __launch_bounds__(BX *BY) void kb_merge_resid(BCtl c)
+// SR: also store the residual as the next cycle's FP32 right-hand side (the
+// start of a defect-correction cycle); else P = e and the max only.
+template <bool SR>
+__global__ __launch_bounds__(BX *BY) void kb_merge_resid(BCtl c, int gated)
 {
     __shared__ double sh[BX * BY];
     for_items<K_PAIR>(c, [&](int s, int t) {
+        if (gated && c.done[s]) return;
         const BSlot &e = c.slots[s];
         const Geom &G = e.G;
         const int NXH = (G.NX + 1) / 2;
         int k, yi;
         const int zb = plane_point(t, NXH, NXH * G.NY, BX * BY, k, yi);
-        const double m = body::merge_residual_pair(
+        const double m = body::merge_residual_pair<true, SR>(
             e.f[AF_FIELD_P], e.f[AF_FIELD_DIV].d, static_cast<const float *>(e.s_red),
             static_cast<const float *>(e.s_black), static_cast<float *>(e.d_red),
             static_cast<float *>(e.d_black), G, e.h2, k, yi, zb * body::kMergeZ);
@@ -1003,14 +1007,21 @@ hipError_t enq_grad(const af_batch *b, int sw)
 // table's solve groups, `ngroups` of them): all of one group's passes, then
 // the next group's, and a group's split arrays stay in the last-level cache
 // from one pass to the next instead of being streamed from HBM for every pass.
-hipError_t enq_solve(const af_batch *b, int ngroups)
+//
+// checked: the mixed solve iterates to each slot's MixedTarget, as the
+// single engine's checked solve does (af_mixed.hip): the first cycle's
+// close checks the FP64 residual of p = e; a slot that misses its target
+// continues its FP32 iteration by MixedFixSweeps pairs and checks again,
+// then runs FP64 defect-correction cycles.  Each slot takes the engine's
+// operations in its order, so its values are the engine's.
+hipError_t enq_solve(const af_batch *b, int ngroups, bool checked)
 {
     const BCtl c = ctl(b);
     const dim3 gp(b->nwg_other), blk(BX, BY);
     hipError_t err = hipSuccess;
     auto mode = [](int p) { return p == 0 ? 1 : (p == 1 ? 2 : 0); };
     const bool mixed = b->cfg.Precision == AF_PRECISION_MIXED;
-    const bool checked = mixed && b->cfg.MixedChecked != 0;
+    checked = checked && mixed;
     if (mixed)
         hipLaunchKernelGGL(kb_mixed_reset, dim3(kSmallGrid), dim3(256), 0, b->stream, c,
                            checked ? 1 : 0);
@@ -1034,14 +1045,24 @@ hipError_t enq_solve(const af_batch *b, int ngroups)
             hipLaunchKernelGGL((kb_from_split<float, false>), gp, blk, 0, b->stream, cg, 0);
             continue;
         }
-        hipLaunchKernelGGL(kb_merge_resid, gp, blk, 0, b->stream, cg);
+        hipLaunchKernelGGL((kb_merge_resid<false>), gp, blk, 0, b->stream, cg, 0);
         hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, cg, 0, 1);
     }
     if (err != hipSuccess) return err;
-    // The later correction cycles (a few passes, only for slots whose target
-    // is not met yet) run over every slot at once: each slot still takes the
-    // same operations in the same order.
-    for (int cyc = 1; checked && cyc < b->cfg.MixedMaxCycles; cyc++) {
+    // The later cycles (a few passes, only for slots whose target is not met
+    // yet) run over every slot at once: each slot still takes the same
+    // operations in the same order.  First the FP32 iteration continued.
+    if (checked && b->cfg.MixedMaxCycles > 1) {
+        for (int p = 0; p < 2 * b->cfg.MixedFixSweeps && err == hipSuccess; p++)
+            err = enq_pass<float>(b, b->nwg_pass, p & 1, 0, 1);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL((kb_merge_resid<false>), gp, blk, 0, b->stream, c, 1);
+        hipLaunchKernelGGL(kb_res_final, dim3(kSmallGrid), dim3(256), 0, b->stream, c, 1, 1);
+    }
+    // Then defect correction: the residual of P = e as the right-hand side.
+    if (checked && b->cfg.MixedMaxCycles > 2)
+        hipLaunchKernelGGL((kb_merge_resid<true>), gp, blk, 0, b->stream, c, 1);
+    for (int cyc = 2; checked && cyc < b->cfg.MixedMaxCycles; cyc++) {
         for (int p = 0; p < 2 * b->cfg.MixedFixSweeps && err == hipSuccess; p++)
             err = enq_pass<float>(b, b->nwg_pass, p & 1, mode(p), 1);
         if (err != hipSuccess) return err;
@@ -1058,17 +1079,17 @@ hipError_t enq_solve(const af_batch *b, int ngroups)
 // projection starts with (the sweep before it applied it), and the gradient
 // subtraction applies the closing one.  div_dead (the step's first
 // projection): nothing reads Div before the second projection's divergence
-// rewrites it, so unless the mixed solve is checked (its residual reads
-// Div), Div is not stored.  (The single engine's first projection also
+// rewrites it, so Div is not stored, and a checked mixed solve runs its one
+// cycle unchecked there (the second projection, whose divergence the step
+// leaves, is checked), as the single engine's does.  (The single engine's first projection also
 // leaves the solution in the split arrays for its gradient; the batched
 // gradient reading the split arrays, one position per work item, measured
 // 839 us (FP64) and 1008 us (FP32) against 585 us for kb_grad on the 96
 // engines, more than the merge and pressure BC it saves.)
 hipError_t enq_project(const af_batch *b, int sw, int ngroups, bool div_dead = false)
 {
-    const bool checked = b->cfg.Precision == AF_PRECISION_MIXED && b->cfg.MixedChecked != 0;
-    hipError_t err = enq_div(b, sw, !(div_dead && !checked));
-    if (err == hipSuccess) err = enq_solve(b, ngroups);
+    hipError_t err = enq_div(b, sw, !div_dead);
+    if (err == hipSuccess) err = enq_solve(b, ngroups, b->cfg.MixedChecked != 0 && !div_dead);
     if (err == hipSuccess) err = enq_grad(b, sw);  // applies the closing velocity BC
     return err;
 }

@@ -1171,7 +1171,8 @@ __device__ __forceinline__ double residual_column(const DField &P, const double 
 // (from_split_pair) followed by residual_column, so the results are the
 // same bits; the close moves 24 B per cell (e 4, Div 8, P 8, r 4) instead of
 // the 32 B of the two, and a thread issues all of its loads before the
-// first wait.
+// first wait.  SP / SR: store P / r (the mixed solve's read-only check of
+// p = e stores neither and moves 12 B per cell, e and Div).
 //
 // The pair holds one cell of each colour (cell 2k has the row's parity, so
 // the colour array of each cell is uniform across a row).  Its own values,
@@ -1182,6 +1183,7 @@ __device__ __forceinline__ double residual_column(const DField &P, const double 
 // `div` shares P's layout.
 constexpr int kMergeZ = 4;
 
+template <bool SP = true, bool SR = true>
 __device__ __forceinline__ double merge_residual_pair(const DField &P, const double *div,
                                                       const float *e_red, const float *e_black,
                                                       float *r_red, float *r_black, const Geom &G,
@@ -1261,15 +1263,19 @@ __device__ __forceinline__ double merge_residual_pair(const DField &P, const dou
     for (int r = 0; r < ZC; r++) {
         const int zi = z0 + r;
         if (zi >= NZ) break;
-        double *pp = P.d + (x0 + g) + (yi + g) * P.sy + (zi + g) * P.sz;
-        if (two)
-            st2(pp, a0[r + 1], a1[r + 1]);
-        else
-            p[pp - P.d] = a0[r + 1];
-        const int i = (zi * NY + yi) * NXH + k;
-        const bool black0 = parity(0, yi, zi, g);
-        gp(black0 ? r_black : r_red)[i] = (float)res0[r];
-        if (two) gp(black0 ? r_red : r_black)[i] = (float)res1[r];
+        if (SP) {
+            double *pp = P.d + (x0 + g) + (yi + g) * P.sy + (zi + g) * P.sz;
+            if (two)
+                st2(pp, a0[r + 1], a1[r + 1]);
+            else
+                p[pp - P.d] = a0[r + 1];
+        }
+        if (SR) {
+            const int i = (zi * NY + yi) * NXH + k;
+            const bool black0 = parity(0, yi, zi, g);
+            gp(black0 ? r_black : r_red)[i] = (float)res0[r];
+            if (two) gp(black0 ? r_red : r_black)[i] = (float)res1[r];
+        }
     }
     return m;
 }

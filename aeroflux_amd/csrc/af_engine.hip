@@ -349,7 +349,10 @@ hipError_t enq_solve_fp64(af_engine *e, const Desc &d, bool rhs_ready)
 // rhs_ready: the divergence of this projection already wrote the split
 // right-hand side (enq_divergence with fuse), and on the FP64 colour-split
 // path also the solve's first red pass.
-hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false, bool merge = true)
+// check: a mixed-precision solve iterates to MixedTarget (when set); else
+// it runs one cycle.
+hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false, bool merge = true,
+                     bool check = true)
 {
     if (e->direct.ready()) {
         int bs = 0;
@@ -365,7 +368,7 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false, bool m
         c.sweeps = e->cfg.MixedSweeps;
         c.fix_sweeps = e->cfg.MixedFixSweeps;
         c.cycles = e->cfg.MixedMaxCycles;
-        c.target = e->cfg.MixedTarget;
+        c.target = check ? e->cfg.MixedTarget : 0;
         c.res_to_div = e->cfg.DT / (e->cfg.Density * e->h2);
         c.rhs_ready = rhs_ready;
         c.merge = merge;
@@ -380,20 +383,26 @@ hipError_t enq_solve(af_engine *e, const Desc &d, bool rhs_ready = false, bool m
 // separate launch.  bc_applied: the velocity already holds the opening BC
 // (the sweep before it applied the BC to its output).
 //
-// On the FP64 colour-split path and on the one-cycle mixed path the solution
-// is not merged into P by the solve: the gradient reads it from the split
-// arrays (launch_gradient_split; the FP64 solution, or the FP32 correction,
-// which is the whole of the one-cycle solution: the same values) and writes
-// P's interior as it goes, and P's ghost fill follows.  p_dead: nothing
-// reads P or Div after this projection's gradient before the next
-// projection overwrites them (the step's first projection), so Div is not
-// stored and P not written at all.
+// On the FP64 colour-split path and on the mixed path the solution is not
+// merged into P by the solve: the gradient reads it from the split arrays
+// (launch_gradient_split; the FP64 solution, or the FP32 correction, which
+// is the whole of a mixed solution unless defect-correction cycles ran: the
+// same values) and writes P's interior as it goes, and P's ghost fill
+// follows.  A mixed solve with a max|div| target (MixedTarget) is checked
+// on the projection whose divergence the step leaves; when it needed
+// defect-correction cycles the solution is in P instead, and the gradient
+// reading P runs in place of the split one (both launched, each gated on
+// the solve's device flag).  p_dead: nothing reads P or Div after this
+// projection's gradient before the next projection overwrites them (the
+// step's first projection), so Div is not stored, P not written at all, and
+// the mixed solve runs its one cycle unchecked (the second projection
+// removes what divergence it leaves, and is checked).
 hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, bool p_dead = false)
 {
     hipError_t err = bc_applied ? hipSuccess : enq_velocity_bc(e, d);
     const bool mixed = e->cfg.Precision == AF_PRECISION_MIXED;
-    const bool split_path =
-        !e->direct.ready() && (mixed ? e->cfg.MixedTarget <= 0 : e->tile64.k == 0);
+    const bool split_path = !e->direct.ready() && (mixed || e->tile64.k == 0);
+    const bool checked = mixed && e->cfg.MixedTarget > 0 && !p_dead;
     const DField P = dfield(e, d.f[AF_FIELD_P]);
     if (e->direct.method == DIRECT_FFT) {
         // The direct solve's first pass computes the divergence itself (the
@@ -409,19 +418,23 @@ hipError_t enq_project(af_engine *e, const Desc &d, bool bc_applied = false, boo
         if (err == hipSuccess && split_path && !mixed)
             err = enq_solve_fp64_split(e, P, dfield(e, d.f[AF_FIELD_DIV]), fuse, false);
         else if (err == hipSuccess)
-            err = enq_solve(e, d, fuse, !split_path);
+            err = enq_solve(e, d, fuse, !split_path, !p_dead);
     }
     if (err == hipSuccess && split_path) {
         const DField U = dfield(e, d.f[AF_FIELD_U]), V = dfield(e, d.f[AF_FIELD_V]),
                      W = dfield(e, d.f[AF_FIELD_W]);
         const DField *Pout = p_dead ? nullptr : &P;
+        const int *merged = checked ? e->mixed.merged : nullptr;
         err = mixed ? launch_gradient_split<float>(U, V, W, e->mixed.e_red, e->mixed.e_black, e->G,
-                                                   e->grad_scale, e->stream, Pout)
+                                                   e->grad_scale, e->stream, Pout, merged, 0)
                     : launch_gradient_split<double>(U, V, W, e->ps_red, e->ps_black, e->G,
                                                     e->grad_scale, e->stream, Pout);
+        if (err == hipSuccess && checked)
+            err = launch_gradient(U, V, W, P, e->G, e->grad_scale, e->stream, true, merged, 1);
         // The solve's final applyPressureBC (project.go:98), on the P the
         // gradient wrote.
-        if (err == hipSuccess && !p_dead) err = launch_pressure_bc(P, e->G, e->G.g > 1, e->stream);
+        if (err == hipSuccess && !p_dead)
+            err = launch_pressure_bc(P, e->G, e->G.g > 1, e->stream, merged, 0);
         return err;
     }
     if (err == hipSuccess) err = enq_gradient(e, d, true);

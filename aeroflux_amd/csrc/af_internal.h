@@ -62,13 +62,16 @@ hipError_t launch_rbgs(DField P, DField D, Geom G, double h2, int color, bool fu
 // First red pass of a solve from P == 0: writes every interior cell.
 hipError_t launch_rbgs_first(DField P, DField D, Geom G, double h2, hipStream_t s);
 // Pressure ghost fill (Neumann mirror); outer layers (ghost > 1) are zeroed
-// when zero_outer is set.
-hipError_t launch_pressure_bc(DField P, Geom G, bool zero_outer, hipStream_t s);
+// when zero_outer is set.  Gated launches (gate != nullptr, here and in the
+// gradients below) run only when *gate == run_if, read on the device.
+hipError_t launch_pressure_bc(DField P, Geom G, bool zero_outer, hipStream_t s,
+                              const int *gate = nullptr, int run_if = 1);
 hipError_t launch_zero(double *d, int64_t n, hipStream_t s);
 // velocity_bc: also apply applyVelocityBC to the updated velocity (the BC
 // that follows subtractGradient in project), so that launch can be skipped.
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
-                           hipStream_t s, bool velocity_bc = false);
+                           hipStream_t s, bool velocity_bc = false, const int *gate = nullptr,
+                           int run_if = 1);
 // subtractGradient with the velocity BC, P read from the colour-split
 // solution (red, black: the FP64 solution, or the mixed solve's FP32
 // correction after its one cycle) instead of P: the values of merging it
@@ -77,7 +80,8 @@ hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, doubl
 // caller's).
 template <typename TS>
 hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
-                                 Geom G, double scale, hipStream_t s, const DField *P = nullptr);
+                                 Geom G, double scale, hipStream_t s, const DField *P = nullptr,
+                                 const int *gate = nullptr, int run_if = 1);
 hipError_t launch_set_int(int *p, int v, hipStream_t s);
 
 // Reductions: two launches, fixed tree, no atomics.  `partials` must hold

@@ -159,9 +159,11 @@ __global__ __launch_bounds__(BX *BY) void k_rbgs_first(DField P, DField D, Geom 
 }
 
 // applyPressureBC, project.go:139-163: one thread per ghost-shell cell.
+// A gated launch (gate != nullptr) runs only when *gate == run_if.
 __global__ __launch_bounds__(256) void k_pressure_bc(DField P, Geom G, int64_t count,
-                                                     bool zero_outer)
+                                                     bool zero_outer, const int *gate, int run_if)
 {
+    if (gate && *gate != run_if) return;
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     body::pressure_bc_cell(P, G, k, zero_outer);
@@ -180,8 +182,10 @@ constexpr int GZ = 4;
 
 template <bool BC>
 __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField W, DField P,
-                                                     Geom G, double scale, int3 nb)
+                                                     Geom G, double scale, int3 nb,
+                                                     const int *gate, int run_if)
 {
+    if (gate && *gate != run_if) return;
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     body::gradient_pair<BC, GZ>(U, V, W, P, G, scale, bx * BX + threadIdx.x,
@@ -193,8 +197,10 @@ __global__ __launch_bounds__(BX *BY) void k_gradient(DField U, DField V, DField 
 template <typename TS, bool MERGE>
 __global__ __launch_bounds__(BX *BY) void k_gradient_split(DField U, DField V, DField W,
                                                            const TS *red, const TS *black, Geom G,
-                                                           double scale, int3 nb, DField P)
+                                                           double scale, int3 nb, DField P,
+                                                           const int *gate, int run_if)
 {
+    if (gate && *gate != run_if) return;
     int bx, by, bz;
     xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
     body::gradient_pair_split<GZ, TS>(U, V, W, red, black, G, scale, bx * BX + threadIdx.x,
@@ -362,11 +368,12 @@ hipError_t launch_rbgs_first(DField P, DField D, Geom G, double h2, hipStream_t 
     return hipGetLastError();
 }
 
-hipError_t launch_pressure_bc(DField P, Geom G, bool zero_outer, hipStream_t s)
+hipError_t launch_pressure_bc(DField P, Geom G, bool zero_outer, hipStream_t s, const int *gate,
+                              int run_if)
 {
     const int64_t n = body::pressure_shell_count(P.nx, P.ny, P.nz, G.g);
     hipLaunchKernelGGL(k_pressure_bc, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P, G, n,
-                       zero_outer);
+                       zero_outer, gate, run_if);
     return hipGetLastError();
 }
 
@@ -377,39 +384,40 @@ hipError_t launch_zero(double *d, int64_t n, hipStream_t s)
 }
 
 hipError_t launch_gradient(DField U, DField V, DField W, DField P, Geom G, double scale,
-                           hipStream_t s, bool velocity_bc)
+                           hipStream_t s, bool velocity_bc, const int *gate, int run_if)
 {
     // A thread per cell pair (NXH entries per row); z = g .. g+NZ.
     const int3 nb = nblocks((G.NX + 1) / 2, G.NY, (G.NZ + 1 + GZ - 1) / GZ);
     if (velocity_bc)
         hipLaunchKernelGGL(k_gradient<true>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
-                           nb);
+                           nb, gate, run_if);
     else
         hipLaunchKernelGGL(k_gradient<false>, grid1(nb), dim3(BX, BY), 0, s, U, V, W, P, G, scale,
-                           nb);
+                           nb, gate, run_if);
     return hipGetLastError();
 }
 
 template <typename TS>
 hipError_t launch_gradient_split(DField U, DField V, DField W, const TS *red, const TS *black,
-                                 Geom G, double scale, hipStream_t s, const DField *P)
+                                 Geom G, double scale, hipStream_t s, const DField *P,
+                                 const int *gate, int run_if)
 {
     // A thread per cell pair (NXH entries per row); z = g .. g+NZ.
     const int3 nb = nblocks((G.NX + 1) / 2, G.NY, (G.NZ + 1 + GZ - 1) / GZ);
     if (P)
         hipLaunchKernelGGL((k_gradient_split<TS, true>), grid1(nb), dim3(BX, BY), 0, s, U, V, W,
-                           red, black, G, scale, nb, *P);
+                           red, black, G, scale, nb, *P, gate, run_if);
     else
         hipLaunchKernelGGL((k_gradient_split<TS, false>), grid1(nb), dim3(BX, BY), 0, s, U, V, W,
-                           red, black, G, scale, nb, DField{});
+                           red, black, G, scale, nb, DField{}, gate, run_if);
     return hipGetLastError();
 }
 template hipError_t launch_gradient_split<double>(DField, DField, DField, const double *,
                                                   const double *, Geom, double, hipStream_t,
-                                                  const DField *);
+                                                  const DField *, const int *, int);
 template hipError_t launch_gradient_split<float>(DField, DField, DField, const float *,
                                                  const float *, Geom, double, hipStream_t,
-                                                 const DField *);
+                                                 const DField *, const int *, int);
 
 hipError_t launch_set_int(int *p, int v, hipStream_t s)
 {

@@ -15,6 +15,9 @@ struct MixedState {
     int *done = nullptr;                         // device: set once the target is met
     double *last_res = nullptr;                  // device: last max|div| implied by the residual
     int *cycles = nullptr;                       // device: cycles run since the last reset
+    // device: a defect-correction cycle merged the solution into P (else
+    // the whole solution is the correction e in the split arrays)
+    int *merged = nullptr;
     int np = 0;        // residual partials (one per 64 x 4 column)
     int np_merge = 0;  // first-cycle merge + residual partials (column x z-chunk)
 };
@@ -26,14 +29,19 @@ struct MixedCtx {
     int sweeps;        // FP32 sweep pairs in the first cycle
     int fix_sweeps;    // FP32 sweep pairs in each later cycle
     int cycles;        // maximum cycles
-    double target;     // max|div| target after the projection (0: one cycle)
+    // max|div| target after the projection (0: one cycle).  A checked solve
+    // runs the first cycle's sweeps, then checks the FP64 residual of p = e;
+    // while the target is not met it continues the FP32 iteration by
+    // fix_sweeps pairs once, then runs FP64 defect-correction cycles.
+    double target;
     double res_to_div; // DT/(rho*H^2): max|div| = res_to_div * max|residual|
     // The split right-hand side is already in r_red/r_black, and the first
     // cycle's first red pass in e_red (k_divergence<2>).
     bool rhs_ready = false;
-    // One cycle only (target 0): false leaves P unwritten (no merge, no
-    // pressure BC) for a caller that reads the correction e, the whole of
-    // the solution, from the split arrays.
+    // false leaves P unwritten (no merge, no pressure BC) for a caller that
+    // reads the correction e, the whole of the solution, from the split
+    // arrays -- unless a checked solve ran a defect-correction cycle, which
+    // merges the solution into P and fills P's ghosts (`merged` raised).
     bool merge = true;
     int slab = 1 << 30;      // workgroup y-slab order of the FP32 passes (SplitArgs::slab)
 };

@@ -15,6 +15,16 @@
 // max is the post-projection max|div|: cycles stop once it reaches the
 // target.  Cycles after the target is met run as empty launches (a device
 // flag), which keeps the whole solve capturable in one graph.
+//
+// A checked solve's first check only reads: the FP64 residual of p = e
+// (exact widening) from the FP32 correction and Div, 12 B per cell, so a
+// solve that meets its target at once leaves the solution in the split
+// arrays for the gradient to read, as an unchecked one does.  A solve that
+// misses it continues the same FP32 iteration by a few sweep pairs (the
+// right-hand side is untouched) and checks again; only then do FP64
+// defect-correction cycles follow, which merge p = e into P, store the
+// residual as the next right-hand side, smooth a fresh correction and add
+// it, and raise `merged` so that the caller reads P instead.
 #include "af_bodies.h"
 #include "af_mixed.h"
 #include "af_xcd.h"
@@ -45,37 +55,75 @@ __global__ __launch_bounds__(BX *BY) void k_residual(DField P, const double *__r
     if (threadIdx.x == 0 && threadIdx.y == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = b;
 }
 
-// The first cycle's close in one pass (body::merge_residual_pair): P's
-// interior takes the FP32 correction and the FP64 residual of that P is
-// stored in FP32 for the next cycle, with a per-workgroup max|r|.  A
-// workgroup owns 64 x 4 split entries of (x, y) and ZC planes.
+// The FP64 residual of p = e in one pass (body::merge_residual_pair), with a
+// per-workgroup max|r|: SP also writes e into P's interior, SR stores the
+// residual in FP32 as the next cycle's right-hand side; the check stores
+// neither.  A workgroup owns 64 x 4 split entries of (x, y) and ZC planes.
+// GATED: skipped once the target is met; a fixed grid of workgroups strides
+// over the tiles, so a skipped launch dispatches few workgroups.
 constexpr int ZC = body::kMergeZ;
+constexpr int kGatedCloseGrid = 2048;
 
+template <bool SP, bool SR, bool GATED>
 __global__ __launch_bounds__(BX *BY) void k_merge_residual(DField P, const double *__restrict__ div,
                                                            const float *__restrict__ e_red,
                                                            const float *__restrict__ e_black,
                                                            float *r_red, float *r_black, Geom G,
-                                                           double h2, double *partials, int3 nb)
+                                                           double h2, double *partials, int3 nb,
+                                                           const int *done)
 {
     __shared__ double sh[BY];
-    int bx, by, bz;
-    xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
-    double m = body::merge_residual_pair(P, div, e_red, e_black, r_red, r_black, G, h2,
-                                         bx * BX + threadIdx.x, by * BY + threadIdx.y, bz * ZC);
-    // Max over the workgroup: across each 64-lane wave by shuffles, then
-    // over the four waves (max is exact in any order).
+    if (GATED && *done) return;  // uniform across the launch
+    const int total = GATED ? nb.x * nb.y * nb.z : 1;
+    for (int l = GATED ? (int)blockIdx.x : 0; l < total; l += GATED ? (int)gridDim.x : 1) {
+        int bx, by, bz;
+        if (GATED) {
+            bx = l % nb.x;
+            const int r = l / nb.x;
+            by = r % nb.y;
+            bz = r / nb.y;
+        } else {
+            xcd_block3(nb.x, nb.y, nb.z, bx, by, bz);
+        }
+        double m = body::merge_residual_pair<SP, SR>(P, div, e_red, e_black, r_red, r_black, G, h2,
+                                                     bx * BX + threadIdx.x, by * BY + threadIdx.y,
+                                                     bz * ZC);
+        // Max over the workgroup: across each 64-lane wave by shuffles, then
+        // over the four waves (max is exact in any order).
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(m, off, 64);
-        m = o > m ? o : m;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(m, off, 64);
+            m = o > m ? o : m;
+        }
+        if (threadIdx.x == 0) sh[threadIdx.y] = m;
+        __syncthreads();
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            double b = sh[0];
+            for (int w = 1; w < BY; w++) b = sh[w] > b ? sh[w] : b;
+            partials[(bz * nb.y + by) * nb.x + bx] = b;
+        }
+        __syncthreads();  // sh is rewritten by the next tile
     }
-    if (threadIdx.x == 0) sh[threadIdx.y] = m;
-    __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
-        double b = sh[0];
-        for (int w = 1; w < BY; w++) b = sh[w] > b ? sh[w] : b;
-        partials[(bz * nb.y + by) * nb.x + bx] = b;
-    }
+}
+
+template <bool SP, bool SR, bool GATED>
+void launch_close(MixedState &m, const MixedCtx &c, hipStream_t s)
+{
+    const Geom &G = c.G;
+    const int3 nb = make_int3(((G.NX + 1) / 2 + BX - 1) / BX, (G.NY + BY - 1) / BY,
+                              (G.NZ + ZC - 1) / ZC);
+    const int total = nb.x * nb.y * nb.z;
+    const int grid = GATED ? (total < kGatedCloseGrid ? total : kGatedCloseGrid) : total;
+    hipLaunchKernelGGL((k_merge_residual<SP, SR, GATED>), dim3((unsigned)grid), dim3(BX, BY), 0,
+                       s, c.P, c.D.d, m.e_red, m.e_black, m.r_red, m.r_black, G, c.h2,
+                       m.partials, nb, m.done);
+}
+
+// A defect-correction cycle is about to merge the solution into P: raised
+// unless the target is already met.
+__global__ void k_mark_merged(const int *done, int *merged)
+{
+    if (!*done) *merged = 1;
 }
 
 // k_res_final over many partials: 1024 lanes each fold a fixed stride of
@@ -84,9 +132,10 @@ constexpr int RF = 1024;
 
 __global__ __launch_bounds__(RF) void k_res_final_wide(const double *partials, int n,
                                                        double res_to_div, double target, int *done,
-                                                       double *last, int *cycles)
+                                                       double *last, int *cycles, int check_done)
 {
     __shared__ double sh[RF];
+    if (check_done && *done) return;
     const int t = threadIdx.x;
     double v = 0;
     for (int k0 = t; k0 < n; k0 += 8 * RF) {
@@ -138,19 +187,21 @@ __global__ __launch_bounds__(RB) void k_res_final(const double *partials, int n,
     }
 }
 
-__global__ void k_mixed_reset(int *done, int *cycles)
+__global__ void k_mixed_reset(int *done, int *cycles, int *merged)
 {
     *done = 0;
     *cycles = 0;
+    *merged = 0;
 }
 
 __global__ void k_mixed_count(int *cycles) { *cycles = *cycles + 1; }
 
 // `passes` FP32 red-black passes (red first) on A e = r from e = 0; with
 // first_done the first red pass's values are already in e_red and the
-// passes start at the first black one.
+// passes start at the first black one; with cont they continue from the
+// e the arrays hold.
 hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipStream_t s,
-                  bool first_done = false)
+                  bool first_done = false, bool cont = false)
 {
     SplitArgs<float> a;
     a.red = m.e_red;
@@ -162,7 +213,8 @@ hipError_t smooth(MixedState &m, const MixedCtx &c, int passes, bool gated, hipS
     a.done = gated ? m.done : nullptr;
     a.slab = c.slab;
     for (int p = first_done ? 1 : 0; p < passes; p++) {
-        hipError_t err = launch_split_pass<float>(a, p & 1, p == 0 ? 1 : (p == 1 ? 2 : 0), s);
+        const int mode = cont ? 0 : (p == 0 ? 1 : (p == 1 ? 2 : 0));
+        hipError_t err = launch_split_pass<float>(a, p & 1, mode, s);
         if (err != hipSuccess) return err;
     }
     return hipSuccess;
@@ -185,10 +237,11 @@ hipError_t init_mixed(MixedState &m, Geom G, hipStream_t s)
     const int np_max = m.np > m.np_merge ? m.np : m.np_merge;
     if ((err = hipMalloc(&m.partials, (size_t)np_max * sizeof(double))) != hipSuccess)
         return err;
-    if ((err = hipMalloc(&m.done, 2 * sizeof(int))) != hipSuccess) return err;
+    if ((err = hipMalloc(&m.done, 3 * sizeof(int))) != hipSuccess) return err;
     if ((err = hipMalloc(&m.last_res, sizeof(double))) != hipSuccess) return err;
     m.cycles = m.done + 1;
-    if ((err = hipMemsetAsync(m.done, 0, 2 * sizeof(int), s)) != hipSuccess) return err;
+    m.merged = m.done + 2;
+    if ((err = hipMemsetAsync(m.done, 0, 3 * sizeof(int), s)) != hipSuccess) return err;
     return hipMemsetAsync(m.last_res, 0, sizeof(double), s);
 }
 
@@ -209,10 +262,9 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
     const dim3 blk(BX, BY);
     const dim3 gcol((G.NX + BX - 1) / BX, (G.NY + BY - 1) / BY);
     const bool check = c.target > 0;
-    if (check && !c.merge) return hipErrorInvalidValue;
     hipError_t err;
 
-    hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles);
+    hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles, m.merged);
     // Cycle 0: p = 0, r = b = -(H^2 div).
     if (!c.rhs_ready &&
         (err = launch_rhs32_split(c.D.d, c.D.sy, c.D.sz, G, c.h2, m.r_red, m.r_black, s)) !=
@@ -224,15 +276,27 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
                                                        m.e_black, false, nullptr, s)) != hipSuccess)
             return err;
         hipLaunchKernelGGL(k_mixed_count, dim3(1), dim3(1), 0, s, m.cycles);
-    } else {
-        const int3 nb = make_int3(((G.NX + 1) / 2 + BX - 1) / BX, (G.NY + BY - 1) / BY,
-                                  (G.NZ + ZC - 1) / ZC);
-        hipLaunchKernelGGL(k_merge_residual, dim3((unsigned)(nb.x * nb.y * nb.z)), blk, 0, s, c.P,
-                           c.D.d, m.e_red, m.e_black, m.r_red, m.r_black, G, c.h2, m.partials,
-                           nb);
+        if ((err = hipGetLastError()) != hipSuccess) return err;
+        // Final Neumann fill of the FP64 pressure (project.go:98).
+        return c.merge ? launch_pressure_bc(c.P, G, G.g > 1, s) : hipSuccess;
+    }
+    // The check: FP64 residual of p = e, read-only.
+    launch_close<false, false, false>(m, c, s);
+    hipLaunchKernelGGL(k_res_final_wide, dim3(1), dim3(RF), 0, s, m.partials, m.np_merge,
+                       c.res_to_div, c.target, m.done, m.last_res, m.cycles, 0);
+    if (c.cycles > 1) {
+        // The FP32 iteration continued on the same right-hand side.
+        if ((err = smooth(m, c, 2 * c.fix_sweeps, true, s, false, true)) != hipSuccess) return err;
+        launch_close<false, false, true>(m, c, s);
         hipLaunchKernelGGL(k_res_final_wide, dim3(1), dim3(RF), 0, s, m.partials, m.np_merge,
-                           c.res_to_div, c.target, m.done, m.last_res, m.cycles);
-        for (int cyc = 1; cyc < c.cycles; cyc++) {
+                           c.res_to_div, c.target, m.done, m.last_res, m.cycles, 1);
+    }
+    if (c.cycles > 2) {
+        // Defect correction: P = e and r = b - A P (FP32) as the next
+        // right-hand side, then cycles of a fresh correction added to P.
+        hipLaunchKernelGGL(k_mark_merged, dim3(1), dim3(1), 0, s, m.done, m.merged);
+        launch_close<true, true, true>(m, c, s);
+        for (int cyc = 2; cyc < c.cycles; cyc++) {
             if ((err = smooth(m, c, 2 * c.fix_sweeps, true, s)) != hipSuccess) return err;
             if ((err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black, true,
                                                 m.done, s)) != hipSuccess)
@@ -243,9 +307,15 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
                                c.res_to_div, c.target, m.done, m.last_res, m.cycles, 1);
         }
     }
+    // P = e unless a defect-correction cycle merged the solution already.
+    if (c.merge && (err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red, m.e_black,
+                                                   false, m.merged, s)) != hipSuccess)
+        return err;
     if ((err = hipGetLastError()) != hipSuccess) return err;
-    // Final Neumann fill of the FP64 pressure (project.go:98).
-    return c.merge ? launch_pressure_bc(c.P, G, G.g > 1, s) : hipSuccess;
+    // Final Neumann fill of the FP64 pressure (project.go:98); without merge
+    // only where the defect correction wrote P.
+    return c.merge ? launch_pressure_bc(c.P, G, G.g > 1, s)
+                   : launch_pressure_bc(c.P, G, G.g > 1, s, m.merged, 1);
 }
 
 }  // namespace af

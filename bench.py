@@ -73,20 +73,22 @@ def schedule_bytes(kind: str, iters: int = 24) -> int:
       fewer) and the second writes P from the gradient instead (4 fewer: the
       merge's read of the correction); a batch keeps the merges
       (mixed_batch);
-    * mixed_checked: the one-cycle schedule with the first cycle closed by
-      the fused merge and FP64 residual (24 instead of 12), in both
-      projections (the check reads Div and may add later cycles to P); later
-      cycles, which run only when the check asks for them, are not counted
-      (a lower bound);
+    * mixed_checked: the one-cycle schedule with the second projection's
+      first cycle checked by the FP64 residual of p = e: a single engine's
+      check reads the correction and Div (12 more); a batch's merge also
+      reads Div (8 more, mixed_checked_batch); the first projection runs
+      unchecked.  Later cycles, which run only when the check asks for them,
+      are not counted (a lower bound);
     * direct (FFT passes): divergence folded into the first pass, five
       passes 80, gradient 56, per projection 32 + 80 + 56."""
     batch = kind.endswith("_batch")
     base = kind[:-len("_batch")] if batch else kind
     proj = {"fp64_split": 104 + 24 * iters, "fp64_arena": 80 + 48 * iters,
-            "mixed": 100 + 12 * iters, "mixed_checked": 112 + 12 * iters,
+            "mixed": 100 + 12 * iters, "mixed_checked": 100 + 12 * iters,
             "direct": 32 + 80 + 56}[base]
     saved = {"fp64_split": 8 + (0 if batch else 16 + 8),
-             "mixed": 8 + (0 if batch else (12 + 4) + (4 + 4))}.get(base, 0)
+             "mixed": 8 + (0 if batch else (12 + 4) + (4 + 4)),
+             "mixed_checked": 8 + (-8 if batch else (12 + 4) + (4 + 4) - 12)}.get(base, 0)
     return 288 + 2 * proj - saved
 
 
@@ -232,9 +234,11 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
         the other;
       * an FP64 batch (bit-identical to the plain path);
       * a one-cycle mixed batch (the same 24 sweep pairs in FP32);
-      * a checked mixed batch that iterates every projection until its FP64
-        residual bounds max|div| by that engine's final max|div| on the FP64
-        path (24 FP32 sweep pairs, then checked cycles of 2, at most 4).
+      * a checked mixed batch whose step leaves at most that engine's final
+        max|div| on the FP64 path: the second projection's FP64 residual
+        check after 24 FP32 sweep pairs, then while it is missed the FP32
+        iteration continued by 2 pairs, then FP64 defect-correction cycles
+        of 2 pairs (4 checks at most).
     Each reports cell-updates/s and each engine's final max|div|.  Without
     the plain path (with_plain False, for profiling the batched kernels
     alone) the FP64 batch's final max|div| -- the same bits -- sets the
@@ -301,7 +305,7 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
         # The divergence writes the first red pass; the solve launches the rest.
         npass = 2 * (bc_extra.get("MixedSweeps", 24) if prec == af.PRECISION_MIXED else 24) - 1
         bkind = ("fp64_split_batch" if prec == af.PRECISION_FP64 else
-                 ("mixed_checked" if bc_extra.get("MixedChecked") else "mixed_batch"))
+                 ("mixed_checked_batch" if bc_extra.get("MixedChecked") else "mixed_batch"))
         model = t_vbc + 2 * sum(sw) + 2 * (t_div + npass * t_pass + t_merge + t_pbc + t_grad)
         info = b.info()
         pass_bytes = (6 if prec == af.PRECISION_MIXED else 12) * cells
@@ -359,8 +363,10 @@ def batched_leg(af, ic, n_eng: int, steps: int, warmup: int, with_plain: bool = 
         lambda i: {"MixedSweeps": 24, "MixedFixSweeps": 2, "MixedMaxCycles": 4,
                    "MixedTarget": float(d_plain[i])})
     out["mixed_to_target"]["schedule"] = (
-        "24 FP32 sweep pairs, then FP64-residual-checked cycles of 2 pairs (at most 4) until "
-        "max|div| <= the engine's final max|div| on the plain FP64 path")
+        "24 FP32 sweep pairs per projection; the second projection's FP64 residual check, then "
+        "while max|div| > the engine's final max|div| on the plain FP64 path: the FP32 "
+        "iteration continued by 2 pairs, then FP64 defect-correction cycles of 2 pairs (at "
+        "most 4 checks)")
     out["speedup_mixed_vs_fp64_batch"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out
 
@@ -377,8 +383,9 @@ def big_leg(af, n: int, steps: int) -> dict:
     runs = [("fp64", {}), ("mixed", {"Precision": af.PRECISION_MIXED}), ("mixed_to_target", None)]
     for name, extra in runs:
         if extra is None:
-            # Iterated to the FP64 run's final max|div|: 24 FP32 sweep pairs
-            # closed by the FP64 residual check, then at most one more cycle.
+            # Iterated to the FP64 run's final max|div|: 24 FP32 sweep pairs,
+            # the second projection's FP64 residual check, then at most one
+            # pair of the FP32 iteration continued and one more check.
             extra = {"Precision": af.PRECISION_MIXED, "MixedSweeps": 24, "MixedFixSweeps": 1,
                      "MixedMaxCycles": 2,
                      "MixedTarget": out["fp64"]["final_max_divergence"]}
@@ -478,7 +485,7 @@ def small_leg(af, ic, n: int, steps: int, workers: int, cpu: bool) -> dict:
     return out
 
 
-def serve_summary(jobs, res, st) -> dict:
+def serve_summary(jobs, res, st, kind: str = "mixed_batch") -> dict:
     ok = [r for r in res if r["Status"] == 0]
     lat = [r["Done"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
     que = [r["Admitted"] - j["Arrival"] for j, r in zip(jobs, res) if r["Status"] == 0]
@@ -498,7 +505,7 @@ def serve_summary(jobs, res, st) -> dict:
         "peak_pool_bytes": st["PeakBytes"],
         "preemptions": st["Preemptions"],
         **bandwidth(st["CellUpdates"] / st["Wall"] if st["Wall"] > 0 else 0.0,
-                    schedule_bytes("mixed_batch"), "mixed_batch"),
+                    schedule_bytes(kind), kind),
     }
 
 
@@ -547,8 +554,10 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
     path, each alone (New, init_band_limited, StepN; StepN timed), for its
     cell-updates/s and each job's final max|div|, against which every served
     job's is compared; then those n_plain jobs are served again in a checked
-    batch, each iterated to its own FP64 final max|div| (24 FP32 pairs closed
-    by the FP64 residual check, then checked cycles of 2 pairs, at most 4).
+    batch, each iterated to its own FP64 final max|div| (24 FP32 pairs, the
+    second projection's FP64 residual check, then the FP32 iteration
+    continued by 2 pairs and FP64 defect-correction cycles of 2 pairs, at
+    most 4 checks).
 
     `policies` is a comma list of least (least remaining work admitted
     first and stepped first, up to the step cell budget), fcfs (first come
@@ -600,10 +609,11 @@ def served_leg(af, n_jobs: int, rate: float, budget: float, n_plain: int,
                          for j, r in zip(cjobs, cres) if r["Status"] == 0])
         plain["served_checked"] = {
             "jobs": n_plain,
-            "schedule": "24 FP32 sweep pairs closed by the FP64 residual check, then checked "
-                        "cycles of 2 pairs (at most 4 cycles) until max|div| <= the job's final "
-                        "max|div| on the FP64 path",
-            **serve_summary(cjobs, cres, cst),
+            "schedule": "24 FP32 sweep pairs per projection; the second projection's FP64 "
+                        "residual check, then while max|div| > the job's final max|div| on the "
+                        "FP64 path: the FP32 iteration continued by 2 pairs, then FP64 "
+                        "defect-correction cycles of 2 pairs (at most 4 checks)",
+            **serve_summary(cjobs, cres, cst, "mixed_checked_batch"),
             "speedup_vs_plain_fp64": (cst["CellUpdates"] / cst["Wall"] / plain["value"]
                                       if cst["Wall"] > 0 else None),
             "final_max_divergence_vs_fp64": {
@@ -898,8 +908,9 @@ def main() -> int:
             "speedup_vs_fp64": tval / value,
             "target_max_divergence": d64,
             "final_max_divergence": dt_,
-            "schedule": f"{args.target_sweeps} FP32 sweep pairs closed by the FP64 "
-                        "residual check, then at most one more cycle of 1 pair if max|div| is "
+            "schedule": f"{args.target_sweeps} FP32 sweep pairs per projection; the second "
+                        "projection's FP64 residual check (read-only), then at most one more "
+                        "pair of the same FP32 iteration and one more check if max|div| is "
                         "still above the FP64 run's final value",
             "cycles_last_solve": tcyc,
             "max_rel_error_vs_fp64": terrs,

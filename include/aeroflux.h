@@ -84,14 +84,25 @@ typedef struct af_config {
 
     /* --- extensions --- */
     int32_t Precision;        /* AF_PRECISION_* */
-    int32_t MixedMaxCycles;   /* mixed: max defect-correction cycles (0 -> 4) */
-    int32_t MixedSweeps;      /* mixed: FP32 sweep pairs per cycle (0 ->
-                                 GaussSeidelIters) */
-    int32_t MixedFixSweeps;   /* mixed: FP32 sweep pairs in each later
-                                 correction cycle (0 -> 2) */
-    double MixedTarget;       /* mixed: stop once the post-projection
-                                 max|div| implied by the FP64 residual is
-                                 <= this (0 -> run exactly one cycle) */
+    int32_t MixedMaxCycles;   /* mixed: max FP64 residual checks per
+                                 checked solve (0 -> 4): the first, one
+                                 after the continued FP32 iteration, then
+                                 one per defect-correction cycle */
+    int32_t MixedSweeps;      /* mixed: FP32 sweep pairs of the first cycle
+                                 (0 -> GaussSeidelIters) */
+    int32_t MixedFixSweeps;   /* mixed: FP32 sweep pairs of the continued
+                                 iteration and of each defect-correction
+                                 cycle (0 -> 2) */
+    double MixedTarget;       /* mixed: max|div| the step leaves.  0 -> each
+                                 projection runs exactly one cycle.  > 0 ->
+                                 the step's second projection (and a
+                                 stand-alone af_project) checks the FP64
+                                 residual after the first cycle and, while
+                                 the max|div| it implies exceeds this,
+                                 continues the FP32 iteration by
+                                 MixedFixSweeps pairs once, then runs FP64
+                                 defect-correction cycles; the step's first
+                                 projection runs one cycle */
     int32_t TileK;            /* FP64 red-black schedule: 0 -> one pass per
                                  launch on the colour-split or the arena
                                  layout, whichever is faster on this grid
@@ -299,7 +310,8 @@ int af_time_kernel(af_engine *e, int kernel, int reps, double *avg_ms);
 int af_measure_stream(int kind, int64_t n, int reps, double *gbps);
 
 /* Mixed precision only: max|div| implied by the last FP64 residual check
- * (0 if MixedTarget == 0) and the cycles the last solve ran.  No reference
+ * (0 if MixedTarget == 0) and the cycles the last solve ran (the residual
+ * checks of a checked solve; 1 for a one-cycle solve).  No reference
  * counterpart. */
 int af_mixed_stats(af_engine *e, double *last_div, int32_t *cycles);
 

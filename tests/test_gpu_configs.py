@@ -62,8 +62,8 @@ def test_config2_512cubed_fp64_bitwise_and_mixed_to_target(af, oracle_lib):
     oracle.  Mixed, one cycle of the same 24 sweep pairs: differs from FP64
     by FP32 rounding of the smoothing only (fields within 1e-5 of their
     maximum, max|div| within 1e-5 relative).  Mixed iterated to the FP64
-    path's final max|div|: ends at or below it; its extra correction sweeps
-    (at most 3 cycles of 2 pairs per projection) move the fields away from
+    path's final max|div|: ends at or below it; its extra sweeps (at most 3
+    later cycles of 2 pairs in each step's checked projection) move the fields away from
     the FP64 path's unconverged 24-pair iterate by no more than the FP64 path
     itself moves with 30 pairs (24 + 3 x 2)."""
     n = 512
