@@ -867,8 +867,9 @@ __device__ __forceinline__ int parity(int xi, int yi, int zi, int g)
 // register queue, and a plane needs four more loads: the other x-neighbour,
 // the two y-neighbours and the right-hand side.  The planes go in chunks of
 // CH: a chunk's loads are all issued, then its results formed, then stored,
-// so CH planes' loads are in flight together.  CH holds 32 bytes of results
-// per thread (8 FP32 or 4 FP64 planes).
+// so CH planes' loads are in flight together (in FP64 also the next chunk's,
+// see below).  CH holds 32 bytes of results per thread (8 FP32 or 4 FP64
+// planes).
 //
 // Every neighbour value, wall substitution and the order of the sum are
 // those of gaussSeidelSweep (project.go:103-130) after applyPressureBC
@@ -1008,6 +1009,136 @@ __device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int col
 #pragma unroll
         for (int r = 0; r < CH; r++)
             if (at[r] >= 0) cur[at[r]] = out[r];
+    }
+}
+
+// The same pass with its chunks software-pipelined over two register
+// buffers: chunk c+1's loads are issued before chunk c's arithmetic and
+// stores, so a wave keeps the next chunk's loads in flight while it finishes
+// one.  The values and their order are split_pass_column's.  Used for the
+// single engine's FP64 pass (8 planes, two chunks: 75 -> 128 VGPRs, 32.2 ->
+// 31.5 us at 256^3).  Measured and not used: the batched FP64 pass (92 ->
+// 142 VGPRs, 87 -> 94 us on the 96 engines) and FP32 passes of 16 or 32
+// planes (148-153 VGPRs, 19.5-19.9 us against 17.8 us for 8 planes).
+template <typename T, int MODE, int ZR>
+__device__ __forceinline__ void split_pass_column_pipelined(const SplitArgs<T> &a, int color, int k,
+                                                            int yi, int zb)
+{
+    const int NX = a.NX, NY = a.NY, NZ = a.NZ, NXH = a.NXH;
+    if (yi >= NY || zb >= NZ) return;
+    const auto cur = gp(color ? a.black : a.red);
+    const auto oth = gp((const T *)(color ? a.red : a.black));
+    const auto rhs = gp(color ? a.rhs_black : a.rhs_red);
+    const int P = NY * NXH;
+    constexpr int CH = (int)(ZR * sizeof(T)) > 32 ? 32 / (int)sizeof(T) : ZR;
+    static_assert(ZR % CH == 0 && CH % 2 == 0, "whole, even chunks of planes per thread");
+    const bool ylo = yi == 0, yhi = yi == NY - 1;
+    // Entry k of the other colour in plane zb-1 (zm) and zb (zc): the queue.
+    // Planes outside the interior are read as an in-range stand-in (the
+    // value is replaced by the wall's own value).  Lanes past the row's end
+    // (k >= NXH: the launch covers whole 64-lane rows) store nothing and
+    // read entry NXH-1.
+    const int kq = k < NXH ? k : NXH - 1;
+    const int col = yi * NXH + kq;
+    // The column's two parities: plane zb + r has the first for even r.
+    const int x0 = (color + yi + zb + 3 * a.g) & 1;
+    const SplitCol ca = split_col(NX, NXH, k, kq, yi, x0, ylo, yhi);
+    const SplitCol cb = split_col(NX, NXH, k, kq, yi, x0 ^ 1, ylo, yhi);
+    T zm = T(0), zc = T(0);
+    if (MODE != 1) {
+        zm = oth[(zb > 0 ? zb - 1 : zb) * P + col];
+        zc = oth[zb * P + col];
+    }
+    // One chunk's loads: every plane's index and its loads, all in flight
+    // together.  A plane past the last (zr >= NZ) is computed on the last
+    // plane's indices and not stored.
+    auto load = [&](T *nq, T *nx, T *ny0, T *ny1, T *rh, T *cw, int *at, unsigned char *wall,
+                    int zc0) {
+#pragma unroll
+        for (int r = 0; r < CH; r++) {
+            const SplitCol &cc = (r & 1) ? cb : ca;
+            const int zr = zc0 + r;
+            const int zi = zr < NZ ? zr : NZ - 1;
+            const int i = zi * P + cc.off;
+            at[r] = (zr < NZ && cc.st) ? i : -1;
+            // bit 0..5: x-, x+, y-, y+, z-, z+ outside the interior
+            wall[r] = (unsigned char)(cc.wx | ((zi == 0) << 4) | ((zi == NZ - 1) << 5));
+            // A wall cell's own old value (the Neumann mirror's stand-in for
+            // its outside neighbours); its entry is i, which is k except for
+            // a clamped stand-in cell (not stored).
+            cw[r] = T(0);
+            if (MODE == 0 && wall[r] && cc.live) cw[r] = cur[i];
+            if (MODE != 1) {
+                // The x-neighbour that is not entry k, the y-neighbours, and
+                // entry k of plane zr+1 (the queue's next value).
+                nx[r] = oth[zi * P + cc.xo];
+                ny0[r] = oth[ylo ? i : i - NXH];
+                ny1[r] = oth[yhi ? i : i + NXH];
+                const int zn = zr + 1 < NZ ? zr + 1 : NZ - 1;
+                nq[r] = oth[zn * P + col];
+            }
+            rh[r] = rhs[i];
+        }
+    };
+    // One chunk's results and stores.  A wall cell's own old value stands in
+    // for its outside neighbours (the Neumann mirror applied just before the
+    // pass).
+    auto finish = [&](const T *nq, const T *nx, const T *ny0, const T *ny1, const T *rh,
+                      const T *cw, const int *at, const unsigned char *wall) {
+        T out[CH];
+#pragma unroll
+        for (int r = 0; r < CH; r++) {
+            const SplitCol &cc = (r & 1) ? cb : ca;
+            T sum;
+            if (MODE == 1) {
+                sum = T(0) + T(0) + T(0) + T(0) + T(0) + T(0);
+            } else {
+                // Entry k of this plane (zc) and of the planes below (zm) and
+                // above (nq) are the queue.
+                const T c = cw[r];
+                const unsigned w = wall[r];
+                const T xa = cc.xk0 ? zc : nx[r];
+                const T xb = cc.xk1 ? zc : nx[r];
+                const T xm = (w & 1) ? c : xa;
+                const T xp = (w & 2) ? c : xb;
+                const T ym = (w & 4) ? c : ny0[r];
+                const T yp = (w & 8) ? c : ny1[r];
+                const T zmv = (w & 16) ? c : zm;
+                const T zpv = (w & 32) ? c : nq[r];
+                sum = xm + xp + ym + yp + zmv + zpv;
+                zm = zc;
+                zc = nq[r];
+            }
+            out[r] = relax<T>(sum, rh[r], a.h2);
+        }
+        // Every result is in a register before the first (conditional)
+        // store.  Otherwise each plane's arithmetic is sunk into its store's
+        // branch and every branch then waits for all outstanding memory
+        // operations, earlier stores included, which serialises the stores.
+#pragma unroll
+        for (int r = 0; r < CH; r++) asm volatile("" : "+v"(out[r]));
+#pragma unroll
+        for (int r = 0; r < CH; r++)
+            if (at[r] >= 0) cur[at[r]] = out[r];
+    };
+    constexpr int NCH = ZR / CH;
+    T qa[CH], xa_[CH], y0a[CH], y1a[CH], ra[CH], wa[CH];
+    T qb[CH], xb_[CH], y0b[CH], y1b[CH], rb[CH], wb[CH];
+    int ata[CH], atb[CH];
+    unsigned char fa[CH], fb[CH];
+    load(qa, xa_, y0a, y1a, ra, wa, ata, fa, zb);
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+        if (c + 1 < NCH) {
+            if (c & 1)
+                load(qa, xa_, y0a, y1a, ra, wa, ata, fa, zb + (c + 1) * CH);
+            else
+                load(qb, xb_, y0b, y1b, rb, wb, atb, fb, zb + (c + 1) * CH);
+        }
+        if (c & 1)
+            finish(qb, xb_, y0b, y1b, rb, wb, atb, fb);
+        else
+            finish(qa, xa_, y0a, y1a, ra, wa, ata, fa);
     }
 }
 

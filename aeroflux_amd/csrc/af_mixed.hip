@@ -187,14 +187,14 @@ __global__ __launch_bounds__(RB) void k_res_final(const double *partials, int n,
     }
 }
 
-__global__ void k_mixed_reset(int *done, int *cycles, int *merged)
+// cycles0: 1 for a one-cycle solve (its one cycle counted here), 0 for a
+// checked one (each check counts).
+__global__ void k_mixed_reset(int *done, int *cycles, int *merged, int cycles0)
 {
     *done = 0;
-    *cycles = 0;
+    *cycles = cycles0;
     *merged = 0;
 }
-
-__global__ void k_mixed_count(int *cycles) { *cycles = *cycles + 1; }
 
 // `passes` FP32 red-black passes (red first) on A e = r from e = 0; with
 // first_done the first red pass's values are already in e_red and the
@@ -264,7 +264,8 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
     const bool check = c.target > 0;
     hipError_t err;
 
-    hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles, m.merged);
+    hipLaunchKernelGGL(k_mixed_reset, dim3(1), dim3(1), 0, s, m.done, m.cycles, m.merged,
+                       check ? 0 : 1);
     // Cycle 0: p = 0, r = b = -(H^2 div).
     if (!c.rhs_ready &&
         (err = launch_rhs32_split(c.D.d, c.D.sy, c.D.sz, G, c.h2, m.r_red, m.r_black, s)) !=
@@ -275,7 +276,6 @@ hipError_t enqueue_mixed_solve(MixedState &m, const MixedCtx &c, hipStream_t s)
         if (c.merge && (err = launch_from_split<float>(c.P.d, c.P.sy, c.P.sz, G, m.e_red,
                                                        m.e_black, false, nullptr, s)) != hipSuccess)
             return err;
-        hipLaunchKernelGGL(k_mixed_count, dim3(1), dim3(1), 0, s, m.cycles);
         if ((err = hipGetLastError()) != hipSuccess) return err;
         // Final Neumann fill of the FP64 pressure (project.go:98).
         return c.merge ? launch_pressure_bc(c.P, G, G.g > 1, s) : hipSuccess;

@@ -38,8 +38,13 @@ __global__ __launch_bounds__(BX *BY) void k_split_pass(SplitArgs<T> a, int color
     if (a.done && *a.done) return;
     int bx, by, bz;
     xcd_block3_slab(nb.x, nb.y, nb.z, a.slab, bx, by, bz);
-    body::split_pass_column<T, MODE, zr<T>()>(a, color, bx * BX + threadIdx.x,
-                                              by * BY + threadIdx.y, bz * zr<T>());
+    // FP64: the chunks software-pipelined (body::split_pass_column_pipelined).
+    if constexpr (sizeof(T) == 8)
+        body::split_pass_column_pipelined<T, MODE, zr<T>()>(a, color, bx * BX + threadIdx.x,
+                                                            by * BY + threadIdx.y, bz * zr<T>());
+    else
+        body::split_pass_column<T, MODE, zr<T>()>(a, color, bx * BX + threadIdx.x,
+                                                  by * BY + threadIdx.y, bz * zr<T>());
 }
 
 // Natural interior -> split (both colours).  `nat` uses the padded layout.
