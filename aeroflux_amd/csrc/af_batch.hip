@@ -487,7 +487,13 @@ __global__ __launch_bounds__(BX *BY) void kb_pass(BCtl c, int color, int gated)
         a.h2 = e.h2;
         int k, yi;
         const int zb = plane_point(t, a.NXH, a.NXH * G.NY, BX * BY, k, yi);
-        body::split_pass_column<T, MODE, pass_planes<T>()>(a, color, k, yi,
+        // FP64: two-plane chunks, software-pipelined (87 -> 86 us per pass on the
+        // 96 engines; with four-plane chunks 94 us, 142 VGPRs).
+        if constexpr (sizeof(T) == 8)
+            body::split_pass_column_pipelined<T, MODE, pass_planes<T>(), 2>(a, color, k, yi,
+                                                                           zb * pass_planes<T>());
+        else
+            body::split_pass_column<T, MODE, pass_planes<T>()>(a, color, k, yi,
                                                           zb * pass_planes<T>());
     });
 }

@@ -1017,10 +1017,13 @@ __device__ __forceinline__ void split_pass_column(const SplitArgs<T> &a, int col
 // stores, so a wave keeps the next chunk's loads in flight while it finishes
 // one.  The values and their order are split_pass_column's.  Used for the
 // single engine's FP64 pass (8 planes, two chunks: 75 -> 128 VGPRs, 32.2 ->
-// 31.5 us at 256^3).  Measured and not used: the batched FP64 pass (92 ->
-// 142 VGPRs, 87 -> 94 us on the 96 engines) and FP32 passes of 16 or 32
-// planes (148-153 VGPRs, 19.5-19.9 us against 17.8 us for 8 planes).
-template <typename T, int MODE, int ZR>
+// 31.5 us at 256^3) and the batched FP64 pass with CHP = 2 (four chunks of
+// two planes, 92 VGPRs: 87 -> 86 us on the 96 engines; with the default
+// four-plane chunks 142 VGPRs and 94 us).  CHP: planes per chunk (0: 32
+// bytes of results per thread).  Measured and not used: FP32 passes
+// pipelined (16 or 32 planes: 148-153 VGPRs, 19.5-19.9 us against 17.8 us
+// for 8 planes; 8 planes in chunks of 4 or 2: no change).
+template <typename T, int MODE, int ZR, int CHP = 0>
 __device__ __forceinline__ void split_pass_column_pipelined(const SplitArgs<T> &a, int color, int k,
                                                             int yi, int zb)
 {
@@ -1030,7 +1033,7 @@ __device__ __forceinline__ void split_pass_column_pipelined(const SplitArgs<T> &
     const auto oth = gp((const T *)(color ? a.red : a.black));
     const auto rhs = gp(color ? a.rhs_black : a.rhs_red);
     const int P = NY * NXH;
-    constexpr int CH = (int)(ZR * sizeof(T)) > 32 ? 32 / (int)sizeof(T) : ZR;
+    constexpr int CH = CHP > 0 ? CHP : ((int)(ZR * sizeof(T)) > 32 ? 32 / (int)sizeof(T) : ZR);
     static_assert(ZR % CH == 0 && CH % 2 == 0, "whole, even chunks of planes per thread");
     const bool ylo = yi == 0, yhi = yi == NY - 1;
     // Entry k of the other colour in plane zb-1 (zm) and zb (zc): the queue.
